@@ -1,0 +1,189 @@
+/*
+ * mfcl -- MI355X-native batched narrowphase / BVH collision library, C ABI.
+ *
+ * The boundary replaces per-pair calls into mind-fcl's C++ API with batched
+ * calls over device-resident data.  Every entry point names the reference
+ * interface it stands in for.  No C++ or HIP types cross this ABI: plain
+ * pointers, sizes and PODs only; streams are passed as void* (hipStream_t).
+ *
+ * Conventions
+ *  - every function returns 0 (MFCL_OK) or a negative MFCL_E_* code; the
+ *    message of the last failure is available from mfcl_last_error();
+ *  - "_dev" entry points take device pointers and enqueue on the given
+ *    stream without synchronising; the others take host pointers, copy in,
+ *    run, copy out and synchronise;
+ *  - a context owns its geometry and workspace; one context per stream;
+ *    distinct contexts may be used from distinct threads.
+ *  - poses are FP32 (tx, ty, tz, qw, qx, qy, qz); the quaternion is
+ *    renormalised in FP64 on the device.
+ */
+#ifndef MFCL_H_
+#define MFCL_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFCL_OK 0
+#define MFCL_E_INVALID (-1)
+#define MFCL_E_HIP (-2)
+#define MFCL_E_NOMEM (-3)
+#define MFCL_E_UNSUPPORTED (-4)
+#define MFCL_E_STATE (-5)
+
+/* Shape type codes, equal to cvx_collide::GJKShapeType
+ * (include/fcl/cvx_collide/gjk_shape.h:12-28). */
+#define MFCL_SHAPE_BOX 0      /* params: full side lengths x, y, z (fcl::Box::side) */
+#define MFCL_SHAPE_SPHERE 1   /* params: radius */
+#define MFCL_SHAPE_CAPSULE 3  /* params: radius, full length lz (fcl::Capsule) */
+#define MFCL_SHAPE_CYLINDER 5 /* params: radius, full length lz (fcl::Cylinder) */
+#define MFCL_SHAPE_CONVEX 100 /* vertices in vert_off/vert_cnt (fcl::Convex) */
+
+#define MFCL_MODE_COLLIDE 0         /* fcl::collide semantics */
+#define MFCL_MODE_SIGNED_DISTANCE 1 /* GJKSolver::shapeSignedDistance semantics */
+
+typedef struct mfcl_ctx mfcl_ctx;
+
+/* Query parameters.  Mirrors CollisionRequest (include/fcl/narrowphase/
+ * collision_request.h:51-95) plus the GJKSolver fields that fcl::collide
+ * fills from it (include/fcl/narrowphase/collision_interface-inl.h:18-20,
+ * gjk_solver-inl.h:1121-1130). */
+typedef struct mfcl_request {
+  int32_t mode;               /* MFCL_MODE_* */
+  int32_t max_contacts;       /* collide mode; 0 returns no results (reference warns and returns 0) */
+  int32_t penetration;        /* collide mode: 1 = useDefaultPenetration (GJK/EPA), 0 = boolean only */
+  int32_t gjk_max_iterations; /* default 128 */
+  double gjk_tolerance;       /* default 1e-6 (binaryCollisionTolerance) */
+  int32_t epa_max_faces;      /* default 256 */
+  int32_t epa_max_iterations; /* default 255 */
+  double epa_tolerance;       /* default 1e-6 (distanceTolerance) */
+  int32_t use_guess;          /* 1: per-pair initial GJK guess supplied (GJKSolver::gjk_guess) */
+  int32_t reserved;
+} mfcl_request;
+
+/* Fills *req with the reference defaults for the given mode. */
+void mfcl_request_defaults(mfcl_request* req, int32_t mode);
+
+/* Per-pair result (128 bytes).  World frame throughout.
+ *  collide mode: hit, n_contacts (<= max_contacts), and contact 0 as the
+ *    reference's CollisionResult would hold it first (normal from shape a to
+ *    shape b, pos, depth); distance = -depth when hit.
+ *  signed-distance mode: hit = (distance < 0); distance > 0 separation,
+ *    < 0 minus penetration depth; pa, pb witness points on a and b;
+ *    status codes tell failures apart (ok = 0 means the reference's call
+ *    returned false). */
+typedef struct mfcl_convex_result {
+  int32_t hit;
+  int32_t n_contacts;
+  int8_t method;     /* 0 closed form, 1 GJK/EPA, 2 MPR */
+  int8_t gjk_status; /* cvx_collide::GJK_Status, -1 if not run */
+  int8_t epa_status; /* cvx_collide::EPA_Status, -1 if not run */
+  int8_t ok;
+  uint16_t gjk_iters; /* boolean + distance iterations */
+  uint16_t epa_iters;
+  double distance;
+  double depth;
+  double normal[3];
+  double pos[3];
+  double pa[3];
+  double pb[3];
+} mfcl_convex_result;
+
+/* Mesh query result header, one per pair; contacts go to a separate
+ * (n_pairs x max_contacts) array of (b1, b2) triangle index pairs. */
+typedef struct mfcl_mesh_result {
+  int32_t hit;
+  int32_t n_contacts;
+  int32_t bv_tests;   /* OBB overlap tests performed */
+  int32_t leaf_tests; /* triangle-triangle tests performed */
+} mfcl_mesh_result;
+
+/* ---- context ------------------------------------------------------------ */
+int mfcl_create(mfcl_ctx** out, int device);
+int mfcl_destroy(mfcl_ctx* ctx);
+const char* mfcl_last_error(const mfcl_ctx* ctx);
+int mfcl_sync(mfcl_ctx* ctx);
+/* Records HIP events around each kernel on its stream when enabled; the
+ * durations of the last query are read back with mfcl_last_timing(). */
+int mfcl_set_timing(mfcl_ctx* ctx, int enabled);
+/* ms[0] order/bucketing kernels, ms[1] GJK/closed-form stage, ms[2] EPA
+ * stage, ms[3] mesh traversal, ms[4] broadphase, ms[5] MLP. */
+int mfcl_last_timing(mfcl_ctx* ctx, float* ms6);
+/* Work counters of the last convex query: [0] pairs, [1] hull vertices
+ * scanned by support mapping, [2] EPA pairs. */
+int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4);
+
+/* ---- device memory helpers (so callers need no HIP runtime) ------------- */
+int mfcl_dev_alloc(mfcl_ctx* ctx, uint64_t bytes, void** dptr);
+int mfcl_dev_free(mfcl_ctx* ctx, void* dptr);
+int mfcl_h2d(mfcl_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int mfcl_d2h(mfcl_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- convex / primitive narrowphase ------------------------------------ */
+/* Uploads the shape table (replaces constructing fcl::Box/Sphere/Capsule/
+ * Cylinder/Convex objects, include/fcl/geometry/shape/ headers; the convex
+ * vertex set is fcl::Convex::getVertices()).  params: 4 floats per shape
+ * (unused entries ignored).  verts: xyz triples, shape i owns
+ * verts[vert_off[i] .. vert_off[i] + vert_cnt[i]).  Replaces any previous
+ * shape table. */
+int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
+                       const uint64_t* vert_off, const uint32_t* vert_cnt, uint32_t n_shapes,
+                       const float* verts_xyz, uint64_t n_verts);
+
+/* Batched fcl::collide / GJKSolver::shapeSignedDistance over shape pairs
+ * (include/fcl/narrowphase/collision_interface-inl.h:13-32,
+ * gjk_solver-inl.h:820-874).  pair_a/pair_b: shape ids; pose_a/pose_b: 7
+ * floats per pair; guess3: optional per-pair GJK guess (3 floats, used when
+ * req->use_guess).  Host pointers. */
+int mfcl_convex_query(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                      const float* pose_a7, const float* pose_b7, const float* guess3, uint64_t n,
+                      const mfcl_request* req, mfcl_convex_result* out);
+/* Same over device pointers, enqueued on stream (hipStream_t or NULL). */
+int mfcl_convex_query_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                          const float* pose_a7, const float* pose_b7, const float* guess3,
+                          uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
+                          void* stream);
+
+/* ---- OBBRSS mesh-mesh --------------------------------------------------- */
+/* Builds one OBB tree per mesh on the host and uploads node and triangle
+ * arrays (replaces BVHModel<OBBRSS>::beginModel/addSubModel/endModel,
+ * include/fcl/geometry/bvh/BVH_model-inl.h:136-305).  tris: 3 vertex ids per
+ * triangle, local to the mesh.  Replaces any previous mesh set. */
+int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* vert_off,
+                       const uint32_t* vert_cnt, const uint32_t* tris, const uint64_t* tri_off,
+                       const uint32_t* tri_cnt, uint32_t n_meshes);
+/* Number of BVH nodes of mesh i (host query). */
+int mfcl_mesh_num_nodes(mfcl_ctx* ctx, uint32_t mesh, uint32_t* n_nodes);
+/* Batched fcl::collide(BVHModel<OBBRSS>, BVHModel<OBBRSS>) with
+ * CollisionRequest(max_contacts), penetration disabled
+ * (narrowphase/detail/traversal/collision/bvh_solver-inl.h:75-164).
+ * contacts: n * max_contacts * 2 int32 (b1, b2).  Host pointers. */
+int mfcl_mesh_collide(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                      const float* pose_a7, const float* pose_b7, uint64_t n, int32_t max_contacts,
+                      mfcl_mesh_result* out, int32_t* contacts);
+int mfcl_mesh_collide_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                          const float* pose_a7, const float* pose_b7, uint64_t n,
+                          int32_t max_contacts, mfcl_mesh_result* out, int32_t* contacts,
+                          void* stream);
+
+/* ---- broadphase AABB tree ----------------------------------------------- */
+/* Builds a flattened breadth-first AABB tree over n boxes (replaces
+ * BinaryAABB_Tree::Rebuild, include/fcl/broadphase/binary_AABB_tree-inl.h:19-25). */
+int mfcl_aabbtree_build(mfcl_ctx* ctx, const float* mn3, const float* mx3, const int64_t* ids,
+                        uint32_t n);
+/* For each query box, the ids of the overlapping leaves (up to cap, in no
+ * particular order) and their count (replaces
+ * BinaryAABB_Tree::SingleObjectCollision, binary_AABB_tree-inl.h:400-440).
+ * counts may exceed cap (then only cap ids are written).  Host pointers. */
+int mfcl_aabbtree_query(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uint64_t nq,
+                        int32_t cap, int32_t* counts, int64_t* ids);
+int mfcl_aabbtree_query_dev(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uint64_t nq,
+                            int32_t cap, int32_t* counts, int64_t* ids, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MFCL_H_ */

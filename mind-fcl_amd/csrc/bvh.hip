@@ -1,0 +1,583 @@
+// OBB-tree mesh-mesh collision for gfx950.
+//
+// Host: one OBB tree per mesh, built top-down with covariance (PCA) boxes and
+// a median split of triangle centroids along the box's major axis, laid out
+// breadth-first (siblings adjacent) as 64-byte FP32 nodes; node boxes are
+// inflated when rounded to FP32 so they still contain their triangles.
+//
+// Device: a persistent grid of wavefronts; each wavefront takes one query at
+// a time from an atomic counter and runs the bounding-volume test tree
+// (reference: narrowphase/detail/traversal/collision/bvh_solver-inl.h:75-164)
+// 64 node pairs per step off a wave-private stack in device memory:
+// FP32 OBB separating-axis tests with a rounding margin (never prunes a pair
+// the FP64 test keeps), ballot/prefix compaction of children and contacts,
+// and the 17-axis triangle test in FP64 with contraction off, so a leaf
+// verdict equals the reference's double-precision verdict.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "ctx.h"
+#include "dmath.h"
+
+namespace mfcl {
+
+struct alignas(16) ObbNode {
+  float ax[9];  // box axes, column-major: ax[0..2] = axis 0
+  float c[3];   // centre
+  float e[3];   // half extents
+  int32_t child;  // >= 0: first child (sibling at child + 1); < 0: leaf -(tri + 1)
+};
+static_assert(sizeof(ObbNode) == 64, "node must be 64 bytes");
+
+// ------------------------------------------------------------ host build
+namespace {
+
+struct HV {
+  double x, y, z;
+};
+inline HV hsub(HV a, HV b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double hdot(HV a, HV b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline HV hcross(HV a, HV b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (textbook;
+// Golub & Van Loan 8.5).  Returns eigenvectors as columns of V.
+void symEigen(double A[3][3], double w[3], double V[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) V[i][j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    const double off = std::fabs(A[0][1]) + std::fabs(A[0][2]) + std::fabs(A[1][2]);
+    if (off < 1e-300) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (std::fabs(A[p][q]) < 1e-300) continue;
+        const double theta = (A[q][q] - A[p][p]) / (2 * A[p][q]);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < 3; ++k) {
+          const double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < 3; ++i) w[i] = A[i][i];
+}
+
+struct Builder {
+  const float* v;   // mesh vertices xyz
+  const uint32_t* t;  // mesh triangles (local ids)
+  std::vector<HV> cen;  // centroids
+  std::vector<uint32_t> prim;
+
+  HV vert(uint32_t i) const { return {v[3 * i], v[3 * i + 1], v[3 * i + 2]}; }
+
+  ObbNode fit(const uint32_t* idx, int n, HV axis_out[3]) const {
+    HV m = {0, 0, 0};
+    double S[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const int np = 3 * n;
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const HV p = vert(t[3 * idx[i] + k]);
+        m.x += p.x; m.y += p.y; m.z += p.z;
+        const double q[3] = {p.x, p.y, p.z};
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) S[a][b] += q[a] * q[b];
+      }
+    const double mm[3] = {m.x / np, m.y / np, m.z / np};
+    double C[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) C[a][b] = S[a][b] / np - mm[a] * mm[b];
+    double w[3], V[3][3];
+    symEigen(C, w, V);
+    int order[3] = {0, 1, 2};
+    std::sort(order, order + 3, [&](int a, int b) { return w[a] > w[b]; });
+    HV ax0 = {V[0][order[0]], V[1][order[0]], V[2][order[0]]};
+    HV ax1 = {V[0][order[1]], V[1][order[1]], V[2][order[1]]};
+    const double n0 = std::sqrt(hdot(ax0, ax0));
+    ax0 = {ax0.x / n0, ax0.y / n0, ax0.z / n0};
+    const double d01 = hdot(ax0, ax1);
+    ax1 = {ax1.x - d01 * ax0.x, ax1.y - d01 * ax0.y, ax1.z - d01 * ax0.z};
+    double n1 = std::sqrt(hdot(ax1, ax1));
+    if (n1 < 1e-12) {  // degenerate spread: any perpendicular
+      ax1 = std::fabs(ax0.x) < 0.9 ? hcross(ax0, HV{1, 0, 0}) : hcross(ax0, HV{0, 1, 0});
+      n1 = std::sqrt(hdot(ax1, ax1));
+    }
+    ax1 = {ax1.x / n1, ax1.y / n1, ax1.z / n1};
+    const HV ax2 = hcross(ax0, ax1);
+    const HV axes[3] = {ax0, ax1, ax2};
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const HV p = vert(t[3 * idx[i] + k]);
+        for (int a = 0; a < 3; ++a) {
+          const double pr = hdot(axes[a], p);
+          lo[a] = std::min(lo[a], pr);
+          hi[a] = std::max(hi[a], pr);
+        }
+      }
+    ObbNode nd;
+    HV c = {0, 0, 0};
+    double emax = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double mid = 0.5 * (lo[a] + hi[a]);
+      c.x += axes[a].x * mid; c.y += axes[a].y * mid; c.z += axes[a].z * mid;
+      emax = std::max(emax, 0.5 * (hi[a] - lo[a]));
+    }
+    // FP32 rounding of axes (<= 2^-24 each) and centre moves a contained point
+    // by at most ~4 * 2^-24 * (|c| + |e|); pad by 2^-16 of that scale.
+    const double pad = (std::fabs(c.x) + std::fabs(c.y) + std::fabs(c.z) + 3 * emax) * 1.5e-5 + 1e-12;
+    for (int a = 0; a < 3; ++a) {
+      nd.ax[3 * a] = static_cast<float>(axes[a].x);
+      nd.ax[3 * a + 1] = static_cast<float>(axes[a].y);
+      nd.ax[3 * a + 2] = static_cast<float>(axes[a].z);
+      nd.e[a] = static_cast<float>(0.5 * (hi[a] - lo[a]) + pad);
+      axis_out[a] = axes[a];
+    }
+    nd.c[0] = static_cast<float>(c.x);
+    nd.c[1] = static_cast<float>(c.y);
+    nd.c[2] = static_cast<float>(c.z);
+    nd.child = 0;
+    return nd;
+  }
+
+  // Breadth-first build: a FIFO of (node, first, count) ranges.
+  void build(uint32_t n_tris, std::vector<ObbNode>& nodes) {
+    cen.resize(n_tris);
+    prim.resize(n_tris);
+    for (uint32_t i = 0; i < n_tris; ++i) {
+      const HV a = vert(t[3 * i]), b = vert(t[3 * i + 1]), c = vert(t[3 * i + 2]);
+      cen[i] = {(a.x + b.x + c.x) / 3, (a.y + b.y + c.y) / 3, (a.z + b.z + c.z) / 3};
+      prim[i] = i;
+    }
+    nodes.clear();
+    nodes.reserve(2 * n_tris);
+    struct R { uint32_t node, first, count; };
+    std::vector<R> fifo;
+    fifo.reserve(2 * n_tris);
+    nodes.emplace_back();
+    fifo.push_back({0, 0, n_tris});
+    for (size_t h = 0; h < fifo.size(); ++h) {
+      const R r = fifo[h];
+      HV axes[3];
+      ObbNode nd = fit(prim.data() + r.first, static_cast<int>(r.count), axes);
+      if (r.count == 1) {
+        nd.child = -static_cast<int32_t>(prim[r.first]) - 1;
+        nodes[r.node] = nd;
+        continue;
+      }
+      const uint32_t half = r.count / 2;
+      const HV ax = axes[0];
+      std::nth_element(prim.begin() + r.first, prim.begin() + r.first + half, prim.begin() + r.first + r.count,
+                       [&](uint32_t x, uint32_t y) { return hdot(ax, cen[x]) < hdot(ax, cen[y]); });
+      const uint32_t left = static_cast<uint32_t>(nodes.size());
+      nodes.emplace_back();
+      nodes.emplace_back();
+      nd.child = static_cast<int32_t>(left);
+      nodes[r.node] = nd;
+      fifo.push_back({left, r.first, half});
+      fifo.push_back({left + 1, r.first + half, r.count - half});
+    }
+  }
+};
+
+}  // namespace
+
+int buildAndUploadMeshes(mfcl_ctx* ctx, const float* verts, const uint64_t* voff, const uint32_t* vcnt,
+                         const uint32_t* tris, const uint64_t* toff, const uint32_t* tcnt, uint32_t n) {
+  std::vector<std::vector<ObbNode>> per(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (tcnt[i] == 0 || vcnt[i] == 0) {
+      ctx->err = "mesh " + std::to_string(i) + " is empty";
+      return MFCL_E_INVALID;
+    }
+    for (uint64_t k = 0; k < 3ull * tcnt[i]; ++k)
+      if (tris[3 * toff[i] + k] >= vcnt[i]) {
+        ctx->err = "mesh " + std::to_string(i) + " has a triangle vertex id out of range";
+        return MFCL_E_INVALID;
+      }
+  }
+  const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  {
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nth; ++w)
+      th.emplace_back([&, w]() {
+        for (uint32_t i = w; i < n; i += nth) {
+          Builder b{verts + 3 * voff[i], tris + 3 * toff[i], {}, {}};
+          b.build(tcnt[i], per[i]);
+        }
+      });
+    for (auto& x : th) x.join();
+  }
+  std::vector<uint32_t> node_base(n), tri_base(n), node_count(n);
+  uint64_t total_nodes = 0, total_tris = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    node_base[i] = static_cast<uint32_t>(total_nodes);
+    node_count[i] = static_cast<uint32_t>(per[i].size());
+    tri_base[i] = static_cast<uint32_t>(total_tris);
+    total_nodes += per[i].size();
+    total_tris += tcnt[i];
+  }
+  if (total_nodes > 0x7FFFFFFFull || total_tris > 0x7FFFFFFFull) {
+    ctx->err = "mesh set too large (more than 2^31 nodes or triangles)";
+    return MFCL_E_INVALID;
+  }
+  std::vector<ObbNode> nodes(total_nodes);
+  std::vector<float4> tri4(3 * total_tris);
+  for (uint32_t i = 0; i < n; ++i) {
+    std::memcpy(nodes.data() + node_base[i], per[i].data(), per[i].size() * sizeof(ObbNode));
+    for (uint32_t k = 0; k < tcnt[i]; ++k)
+      for (int j = 0; j < 3; ++j) {
+        const float* p = verts + 3 * (voff[i] + tris[3 * (toff[i] + k) + j]);
+        tri4[3 * (tri_base[i] + k) + j] = float4{p[0], p[1], p[2], 0.f};
+      }
+  }
+  MFCL_HIP_TRY(ctx, hipMalloc(&ctx->d_nodes, nodes.size() * sizeof(ObbNode)));
+  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_nodes, nodes.data(), nodes.size() * sizeof(ObbNode), hipMemcpyHostToDevice));
+  MFCL_HIP_TRY(ctx, hipMalloc(&ctx->d_tris, tri4.size() * sizeof(float4)));
+  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_tris, tri4.data(), tri4.size() * sizeof(float4), hipMemcpyHostToDevice));
+  MFCL_HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_mesh_node_base), n * sizeof(uint32_t)));
+  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_mesh_node_base, node_base.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  MFCL_HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_mesh_node_count), n * sizeof(uint32_t)));
+  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_mesh_node_count, node_count.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  MFCL_HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_mesh_tri_base), n * sizeof(uint32_t)));
+  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_mesh_tri_base, tri_base.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  ctx->mesh_host.node_base = node_base;
+  ctx->mesh_host.node_count = node_count;
+  ctx->n_meshes = n;
+  return MFCL_OK;
+}
+
+// --------------------------------------------------------------- device
+struct MeshArgs {
+  const ObbNode* nodes;
+  const float4* tris;
+  const uint32_t* node_base;
+  const uint32_t* node_count;
+  const uint32_t* tri_base;
+  const uint32_t* pa;
+  const uint32_t* pb;
+  const float* posea;
+  const float* poseb;
+  uint64_t n;
+  uint32_t n_meshes;
+  int max_contacts;
+  mfcl_mesh_result* out;
+  int32_t* contacts;
+  uint64_t* stacks;
+  uint32_t stack_cap;
+  uint32_t* next_query;
+  unsigned long long* totals;  // [0] bv tests, [1] leaf tests
+};
+
+struct F3 {
+  float x, y, z;
+};
+MFCL_D F3 f3(const float* p) { return {p[0], p[1], p[2]}; }
+MFCL_D float fdot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+#pragma clang fp contract(fast)
+// FP32 OBB overlap with a margin: returns false only when the separating
+// axis clears the boxes by more than the accumulated FP32 rounding error,
+// so the FP64 test (OBB-inl.h:306-436, |B| padded by 1e-6) would also
+// report disjoint.  Rq (3x3 row-major) and Tq: pose of tree-2 frame in
+// tree-1 frame, FP32.
+MFCL_D bool obbOverlapF(const float* Rq, const float* Tq, const ObbNode& a, const ObbNode& b, float scale_err) {
+  // B = a.axes^T * Rq * b.axes ; T = a.axes^T * (Rq * b.c + Tq - a.c)
+  float RB[9];  // Rq * b.axes (column-major columns)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const float bx = b.ax[3 * j], by = b.ax[3 * j + 1], bz = b.ax[3 * j + 2];
+    RB[3 * j] = Rq[0] * bx + Rq[1] * by + Rq[2] * bz;
+    RB[3 * j + 1] = Rq[3] * bx + Rq[4] * by + Rq[5] * bz;
+    RB[3 * j + 2] = Rq[6] * bx + Rq[7] * by + Rq[8] * bz;
+  }
+  float B[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      B[i][j] = a.ax[3 * i] * RB[3 * j] + a.ax[3 * i + 1] * RB[3 * j + 1] + a.ax[3 * i + 2] * RB[3 * j + 2];
+  const float tx = Rq[0] * b.c[0] + Rq[1] * b.c[1] + Rq[2] * b.c[2] + Tq[0] - a.c[0];
+  const float ty = Rq[3] * b.c[0] + Rq[4] * b.c[1] + Rq[5] * b.c[2] + Tq[1] - a.c[1];
+  const float tz = Rq[6] * b.c[0] + Rq[7] * b.c[1] + Rq[8] * b.c[2] + Tq[2] - a.c[2];
+  float T[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) T[i] = a.ax[3 * i] * tx + a.ax[3 * i + 1] * ty + a.ax[3 * i + 2] * tz;
+  float Bf[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Bf[i][j] = fabsf(B[i][j]) + 1e-6f;
+  const float* ea = a.e;
+  const float* eb = b.e;
+  // translational rounding error bound (absolute, in length units)
+  const float m = scale_err * (fabsf(a.c[0]) + fabsf(a.c[1]) + fabsf(a.c[2]) + fabsf(b.c[0]) + fabsf(b.c[1]) +
+                               fabsf(b.c[2]) + ea[0] + ea[1] + ea[2] + eb[0] + eb[1] + eb[2]) +
+                  scale_err * (fabsf(Tq[0]) + fabsf(Tq[1]) + fabsf(Tq[2]));
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (fabsf(T[i]) > ea[i] + Bf[i][0] * eb[0] + Bf[i][1] * eb[1] + Bf[i][2] * eb[2] + m) return false;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const float s = B[0][j] * T[0] + B[1][j] * T[1] + B[2][j] * T[2];
+    if (fabsf(s) > eb[j] + Bf[0][j] * ea[0] + Bf[1][j] * ea[1] + Bf[2][j] * ea[2] + m) return false;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      const float s = T[i2] * B[i1][j] - T[i1] * B[i2][j];
+      const float r = ea[i1] * Bf[i2][j] + ea[i2] * Bf[i1][j] + eb[j1] * Bf[i][j2] + eb[j2] * Bf[i][j1];
+      if (fabsf(s) > r + 2.0f * m) return false;
+    }
+  }
+  return true;
+}
+
+#pragma clang fp contract(off)
+// intersect-inl.h:1082-1103 and :594-790 in FP64 without FMA contraction.
+MFCL_D bool project6d(d3 ax, d3 p1, d3 p2, d3 p3, d3 q1, d3 q2, d3 q3) {
+  const double P1 = dot(ax, p1), P2 = dot(ax, p2), P3 = dot(ax, p3);
+  const double Q1 = dot(ax, q1), Q2 = dot(ax, q2), Q3 = dot(ax, q3);
+  const double mn1 = fmin(P1, fmin(P2, P3)), mx2 = fmax(Q1, fmax(Q2, Q3));
+  if (mn1 > mx2) return false;
+  const double mx1 = fmax(P1, fmax(P2, P3)), mn2 = fmin(Q1, fmin(Q2, Q3));
+  if (mn2 > mx1) return false;
+  return true;
+}
+
+MFCL_D bool triTriD(const float4* P, const float4* Qm, const m33& R, d3 T) {
+  const d3 P1 = {P[0].x, P[0].y, P[0].z}, P2 = {P[1].x, P[1].y, P[1].z}, P3 = {P[2].x, P[2].y, P[2].z};
+  const d3 Q1 = mul(R, d3{Qm[0].x, Qm[0].y, Qm[0].z}) + T;
+  const d3 Q2 = mul(R, d3{Qm[1].x, Qm[1].y, Qm[1].z}) + T;
+  const d3 Q3 = mul(R, d3{Qm[2].x, Qm[2].y, Qm[2].z}) + T;
+  const d3 p1 = P1 - P1, p2 = P2 - P1, p3 = P3 - P1;
+  const d3 q1 = Q1 - P1, q2 = Q2 - P1, q3 = Q3 - P1;
+  const d3 e1 = p2 - p1, e2 = p3 - p2, e3 = p1 - p3;
+  const d3 f1 = q2 - q1, f2 = q3 - q2, f3 = q1 - q3;
+  const d3 n1 = cross(e1, e2);
+  if (!project6d(n1, p1, p2, p3, q1, q2, q3)) return false;
+  const d3 m1 = cross(f1, f2);
+  if (!project6d(m1, p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e1, f1), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e1, f2), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e1, f3), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e2, f1), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e2, f2), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e2, f3), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e3, f1), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e3, f2), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e3, f3), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e1, n1), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e2, n1), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(e3, n1), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(f1, m1), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(f2, m1), p1, p2, p3, q1, q2, q3)) return false;
+  if (!project6d(cross(f3, m1), p1, p2, p3, q1, q2, q3)) return false;
+  return true;
+}
+
+MFCL_D void relPose(const float* pa, const float* pb, m33& R, d3& T) {
+  const tf3 ta = poseFromQuat(pa), tb = poseFromQuat(pb);
+  R = tmulm(ta.R, tb.R);
+  T = tmul(ta.R, tb.t - ta.t);
+}
+
+__device__ __forceinline__ unsigned long long ballot64(bool p) { return __ballot(p); }
+__device__ __forceinline__ int laneId() { return threadIdx.x & 63; }
+__device__ __forceinline__ int prefixBelow(unsigned long long m) {
+  return __popcll(m & ((1ull << laneId()) - 1ull));
+}
+
+__global__ void __launch_bounds__(256) meshKernel(MeshArgs A) {
+  const int lane = laneId();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  uint64_t* stk = A.stacks + static_cast<uint64_t>(wave) * A.stack_cap;
+  const float kScaleErr = 16.0f * 5.9604645e-08f;
+  unsigned long long tot_bv = 0, tot_leaf = 0;
+  while (true) {
+    uint32_t q = 0;
+    if (lane == 0) q = atomicAdd(A.next_query, 1u);
+    q = __shfl(q, 0);
+    if (q >= A.n) break;
+    const uint32_t ma = A.pa[q], mb = A.pb[q];
+    if (ma >= A.n_meshes || mb >= A.n_meshes) {
+      if (lane == 0) A.out[q] = mfcl_mesh_result{-1, -1, 0, 0};
+      continue;
+    }
+    const uint32_t base1 = A.node_base[ma], base2 = A.node_base[mb];
+    const uint32_t cnt1 = A.node_count[ma], cnt2 = A.node_count[mb];
+    const float4* tri1 = A.tris + 3ull * A.tri_base[ma];
+    const float4* tri2 = A.tris + 3ull * A.tri_base[mb];
+    m33 R;
+    d3 T;
+    relPose(A.posea + 7ull * q, A.poseb + 7ull * q, R, T);
+    const float Rq[9] = {(float)R.m00, (float)R.m01, (float)R.m02, (float)R.m10, (float)R.m11,
+                         (float)R.m12, (float)R.m20, (float)R.m21, (float)R.m22};
+    const float Tq[3] = {(float)T.x, (float)T.y, (float)T.z};
+    int top = 1;
+    if (lane == 0) stk[0] = 0ull;
+    int ncont = 0;
+    int bv = 0, leaf = 0;
+    bool overflow = false;
+    int32_t* cout = A.contacts + 2ull * A.max_contacts * q;
+    while (top > 0 && ncont < A.max_contacts) {
+      // make this wave's stack stores visible to its next loads
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+      const int k = top < 64 ? top : 64;
+      const int b0 = top - k;
+      const bool act = lane < k;
+      uint64_t e = act ? stk[b0 + lane] : 0ull;
+      top = b0;
+      const uint32_t n1 = static_cast<uint32_t>(e >> 32), n2 = static_cast<uint32_t>(e & 0xFFFFFFFFu);
+      bool is_leafpair = false, push1 = false, push2 = false;
+      int32_t c1 = 0, c2 = 0;
+      const bool bad = act && (n1 >= cnt1 || n2 >= cnt2);
+      if (ballot64(bad)) {
+        overflow = true;
+        break;
+      }
+      if (act) {
+        const ObbNode a = A.nodes[base1 + n1];
+        const ObbNode b = A.nodes[base2 + n2];
+        if (obbOverlapF(Rq, Tq, a, b, kScaleErr)) {
+          const bool la = a.child < 0, lb = b.child < 0;
+          c1 = a.child;
+          c2 = b.child;
+          if (la && lb) {
+            is_leafpair = true;
+          } else {
+            const float sa = a.e[0] * a.e[0] + a.e[1] * a.e[1] + a.e[2] * a.e[2];
+            const float sb = b.e[0] * b.e[0] + b.e[1] * b.e[1] + b.e[2] * b.e[2];
+            if (lb || (!la && sa > sb)) push1 = true;
+            else push2 = true;
+          }
+        }
+      }
+      bv += k;
+      // leaf tests (FP64), then compact hits into the contact list
+      bool hit = false;
+      int t1 = 0, t2 = 0;
+      if (is_leafpair) {
+        t1 = -c1 - 1;
+        t2 = -c2 - 1;
+        hit = triTriD(tri1 + 3 * t1, tri2 + 3 * t2, R, T);
+      }
+      leaf += __popcll(ballot64(is_leafpair));
+      const unsigned long long hm = ballot64(hit);
+      if (hm) {
+        const int slot = ncont + prefixBelow(hm);
+        if (hit && slot < A.max_contacts) {
+          cout[2 * slot] = t1;
+          cout[2 * slot + 1] = t2;
+        }
+        ncont += __popcll(hm);
+        if (ncont > A.max_contacts) ncont = A.max_contacts;
+      }
+      // push children (two per expanding lane)
+      const unsigned long long pm = ballot64(push1 || push2);
+      const int total = 2 * __popcll(pm);
+      if (top + total > static_cast<int>(A.stack_cap)) {
+        overflow = true;
+        break;
+      }
+      if (push1 || push2) {
+        const int pos = top + 2 * prefixBelow(pm);
+        uint64_t x, y;
+        if (push1) {
+          x = (static_cast<uint64_t>(c1) << 32) | n2;
+          y = (static_cast<uint64_t>(c1 + 1) << 32) | n2;
+        } else {
+          x = (static_cast<uint64_t>(n1) << 32) | static_cast<uint32_t>(c2);
+          y = (static_cast<uint64_t>(n1) << 32) | static_cast<uint32_t>(c2 + 1);
+        }
+        stk[pos] = x;
+        stk[pos + 1] = y;
+      }
+      top += total;
+    }
+    if (lane == 0) {
+      mfcl_mesh_result r;
+      r.hit = overflow ? -1 : (ncont > 0 ? 1 : 0);
+      r.n_contacts = overflow ? -1 : ncont;
+      r.bv_tests = bv;
+      r.leaf_tests = leaf;
+      A.out[q] = r;
+    }
+    tot_bv += bv;
+    tot_leaf += leaf;
+  }
+  if (lane == 0) {
+    atomicAdd(&A.totals[0], tot_bv);
+    atomicAdd(&A.totals[1], tot_leaf);
+  }
+}
+
+__global__ void zeroU32(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                      const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
+                      int32_t* contacts, hipStream_t s) {
+  if (n == 0) return MFCL_OK;
+  if (ctx->n_meshes == 0) {
+    ctx->err = "no meshes uploaded";
+    return MFCL_E_STATE;
+  }
+  hipDeviceProp_t prop;
+  MFCL_HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
+  const int blocks = prop.multiProcessorCount * 4;
+  const int waves = blocks * 4;
+  const uint32_t cap = 16384;
+  if (int rc = ensure(ctx, ctx->ws_arena, static_cast<size_t>(waves) * cap * sizeof(uint64_t))) return rc;
+  if (int rc = ensure(ctx, ctx->ws_misc, 2048)) return rc;
+  uint32_t* next = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->ws_misc.p) + 1024);
+  unsigned long long* totals = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->ws_misc.p) + 1536);
+  MeshArgs A;
+  A.nodes = static_cast<const ObbNode*>(ctx->d_nodes);
+  A.tris = static_cast<const float4*>(ctx->d_tris);
+  A.node_base = ctx->d_mesh_node_base;
+  A.node_count = ctx->d_mesh_node_count;
+  A.tri_base = ctx->d_mesh_tri_base;
+  A.pa = pa;
+  A.pb = pb;
+  A.posea = posea;
+  A.poseb = poseb;
+  A.n = n;
+  A.n_meshes = ctx->n_meshes;
+  A.max_contacts = max_contacts;
+  A.out = out;
+  A.contacts = contacts;
+  A.stacks = static_cast<uint64_t*>(ctx->ws_arena.p);
+  A.stack_cap = cap;
+  A.next_query = next;
+  A.totals = totals;
+  zeroU32<<<1, 512, 0, s>>>(static_cast<uint32_t*>(ctx->ws_misc.p), 512);
+  if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+  meshKernel<<<blocks, 256, 0, s>>>(A);
+  if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  MFCL_HIP_TRY(ctx, hipGetLastError());
+  ctx->d_counters = reinterpret_cast<uint64_t*>(totals);
+  return MFCL_OK;
+}
+
+}  // namespace mfcl

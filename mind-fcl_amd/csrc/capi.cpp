@@ -1,0 +1,448 @@
+// Host side of the C ABI declared in include/mfcl.h: context lifetime,
+// geometry upload, workspace management, and host-pointer convenience
+// wrappers around the device-pointer launchers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "ctx.h"
+
+namespace mfcl {
+
+int ensure(mfcl_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (b.bytes >= bytes) return MFCL_OK;
+  if (b.p) {
+    MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    MFCL_HIP_TRY(ctx, hipDeviceSynchronize());
+    MFCL_HIP_TRY(ctx, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  const size_t grown = bytes + bytes / 4;
+  hipError_t e = hipMalloc(&b.p, grown);
+  if (e != hipSuccess) {
+    ctx->err = std::string("workspace allocation failed: ") + hipGetErrorString(e);
+    b.p = nullptr;
+    return MFCL_E_NOMEM;
+  }
+  b.bytes = grown;
+  return MFCL_OK;
+}
+
+static void freePtr(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void freeShapes(mfcl_ctx* c) {
+  freePtr(reinterpret_cast<void*&>(c->d_type));
+  freePtr(reinterpret_cast<void*&>(c->d_param));
+  freePtr(reinterpret_cast<void*&>(c->d_voff));
+  freePtr(reinterpret_cast<void*&>(c->d_vcnt));
+  freePtr(reinterpret_cast<void*&>(c->d_verts));
+  freePtr(reinterpret_cast<void*&>(c->d_interior));
+  c->n_shapes = 0;
+}
+
+void freeMeshes(mfcl_ctx* c) {
+  freePtr(c->d_nodes);
+  freePtr(c->d_tris);
+  freePtr(reinterpret_cast<void*&>(c->d_mesh_node_base));
+  freePtr(reinterpret_cast<void*&>(c->d_mesh_node_count));
+  freePtr(reinterpret_cast<void*&>(c->d_mesh_tri_base));
+  c->n_meshes = 0;
+  c->mesh_host = MeshHost();
+}
+
+void freeAll(mfcl_ctx* c) {
+  freeShapes(c);
+  freeMeshes(c);
+  freePtr(c->d_bp_nodes);
+  freePtr(reinterpret_cast<void*&>(c->d_bp_ids));
+  freePtr(c->ws_order.p);
+  freePtr(c->ws_queue.p);
+  freePtr(c->ws_arena.p);
+  freePtr(c->ws_misc.p);
+  freePtr(c->ws_io.p);
+}
+
+// Host builders (bvh_build.cpp, broadphase.hip).
+int buildAndUploadMeshes(mfcl_ctx* ctx, const float* verts, const uint64_t* voff, const uint32_t* vcnt,
+                         const uint32_t* tris, const uint64_t* toff, const uint32_t* tcnt, uint32_t n);
+int buildAndUploadAabbTree(mfcl_ctx* ctx, const float* mn, const float* mx, const int64_t* ids, uint32_t n);
+
+}  // namespace mfcl
+
+using namespace mfcl;
+
+template <class T>
+static int upload(mfcl_ctx* ctx, T** dst, const T* src, size_t count) {
+  if (count == 0) count = 1;
+  MFCL_HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * count));
+  if (src) MFCL_HIP_TRY(ctx, hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  return MFCL_OK;
+}
+
+extern "C" {
+
+void mfcl_request_defaults(mfcl_request* req, int32_t mode) {
+  std::memset(req, 0, sizeof(*req));
+  req->mode = mode;
+  req->max_contacts = 1;
+  req->penetration = mode == MFCL_MODE_SIGNED_DISTANCE ? 1 : 0;
+  req->gjk_max_iterations = 128;
+  req->gjk_tolerance = 1e-6;
+  req->epa_max_faces = 256;
+  req->epa_max_iterations = 255;
+  req->epa_tolerance = 1e-6;
+  req->use_guess = 0;
+}
+
+int mfcl_create(mfcl_ctx** out, int device) {
+  if (!out) return MFCL_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MFCL_E_HIP;
+  if (device < 0 || device >= n) return MFCL_E_INVALID;
+  auto* c = new (std::nothrow) mfcl_ctx();
+  if (!c) return MFCL_E_NOMEM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return MFCL_E_HIP;
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete c;
+      return MFCL_E_HIP;
+    }
+  *out = c;
+  return MFCL_OK;
+}
+
+int mfcl_destroy(mfcl_ctx* ctx) {
+  if (!ctx) return MFCL_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  freeAll(ctx);
+  for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MFCL_OK;
+}
+
+const char* mfcl_last_error(const mfcl_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mfcl_sync(mfcl_ctx* ctx) {
+  if (!ctx) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  MFCL_HIP_TRY(ctx, hipDeviceSynchronize());
+  return MFCL_OK;
+}
+
+int mfcl_set_timing(mfcl_ctx* ctx, int enabled) {
+  if (!ctx) return MFCL_E_INVALID;
+  ctx->timing = enabled != 0;
+  return MFCL_OK;
+}
+
+int mfcl_last_timing(mfcl_ctx* ctx, float* ms6) {
+  if (!ctx || !ms6) return MFCL_E_INVALID;
+  std::memcpy(ms6, ctx->last_ms, sizeof(ctx->last_ms));
+  return MFCL_OK;
+}
+
+int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4) {
+  if (!ctx || !c4) return MFCL_E_INVALID;
+  if (ctx->d_counters) {
+    MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t dev[2] = {0, 0};
+    uint32_t q = 0;
+    MFCL_HIP_TRY(ctx, hipMemcpy(dev, ctx->d_counters, sizeof(dev), hipMemcpyDeviceToHost));
+    MFCL_HIP_TRY(ctx, hipMemcpy(&q, static_cast<char*>(ctx->ws_misc.p) + 1024, sizeof(q), hipMemcpyDeviceToHost));
+    ctx->counters[1] = dev[0] + dev[1];
+    ctx->counters[2] = q;
+    ctx->counters[3] = dev[1];
+  }
+  std::memcpy(c4, ctx->counters, sizeof(ctx->counters));
+  return MFCL_OK;
+}
+
+int mfcl_dev_alloc(mfcl_ctx* ctx, uint64_t bytes, void** dptr) {
+  if (!ctx || !dptr) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipMalloc(dptr, bytes ? bytes : 1));
+  return MFCL_OK;
+}
+int mfcl_dev_free(mfcl_ctx* ctx, void* dptr) {
+  if (!ctx) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipFree(dptr));
+  return MFCL_OK;
+}
+int mfcl_h2d(mfcl_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return MFCL_OK;
+}
+int mfcl_d2h(mfcl_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  MFCL_HIP_TRY(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return MFCL_OK;
+}
+
+int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4, const uint64_t* vert_off,
+                       const uint32_t* vert_cnt, uint32_t n_shapes, const float* verts_xyz,
+                       uint64_t n_verts) {
+  if (!ctx || !type || !params4 || n_shapes == 0) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  freeShapes(ctx);
+  std::vector<float4> param(n_shapes);
+  std::vector<uint32_t> voff(n_shapes, 0), vcnt(n_shapes, 0);
+  std::vector<double4> interior(n_shapes, double4{0, 0, 0, 0});
+  std::vector<float4> verts(std::max<uint64_t>(n_verts, 1));
+  for (uint64_t k = 0; k < n_verts; ++k)
+    verts[k] = float4{verts_xyz[3 * k], verts_xyz[3 * k + 1], verts_xyz[3 * k + 2], 0.f};
+  for (uint32_t i = 0; i < n_shapes; ++i) {
+    const int t = type[i];
+    param[i] = float4{params4[4 * i], params4[4 * i + 1], params4[4 * i + 2], 0.f};
+    if (t == MFCL_SHAPE_CONVEX) {
+      if (!vert_off || !vert_cnt || vert_cnt[i] == 0 || vert_off[i] + vert_cnt[i] > n_verts ||
+          vert_off[i] > 0xFFFFFFFFull) {
+        ctx->err = "convex shape " + std::to_string(i) + " has an invalid vertex range";
+        return MFCL_E_INVALID;
+      }
+      voff[i] = static_cast<uint32_t>(vert_off[i]);
+      vcnt[i] = vert_cnt[i];
+      // interior point = vertex mean in double (convex-inl.h:64-70); rbound
+      // = max |coordinate| for the FP32 support-scan error bound.
+      double sx = 0, sy = 0, sz = 0;
+      float rb = 0;
+      for (uint32_t k = 0; k < vcnt[i]; ++k) {
+        const float* p = verts_xyz + 3 * (vert_off[i] + k);
+        sx += p[0];
+        sy += p[1];
+        sz += p[2];
+        rb = std::max(rb, std::max(std::fabs(p[0]), std::max(std::fabs(p[1]), std::fabs(p[2]))));
+      }
+      const double inv = 1.0 / vcnt[i];
+      interior[i] = double4{sx * inv, sy * inv, sz * inv, 0};
+      param[i].w = rb;
+    } else if (t != MFCL_SHAPE_BOX && t != MFCL_SHAPE_SPHERE && t != MFCL_SHAPE_CAPSULE &&
+               t != MFCL_SHAPE_CYLINDER) {
+      ctx->err = "unsupported shape type " + std::to_string(t) + " for shape " + std::to_string(i);
+      return MFCL_E_UNSUPPORTED;
+    }
+  }
+  int rc;
+  if ((rc = upload(ctx, &ctx->d_type, type, n_shapes))) return rc;
+  if ((rc = upload(ctx, &ctx->d_param, param.data(), n_shapes))) return rc;
+  if ((rc = upload(ctx, &ctx->d_voff, voff.data(), n_shapes))) return rc;
+  if ((rc = upload(ctx, &ctx->d_vcnt, vcnt.data(), n_shapes))) return rc;
+  if ((rc = upload(ctx, &ctx->d_verts, verts.data(), verts.size()))) return rc;
+  if ((rc = upload(ctx, &ctx->d_interior, interior.data(), n_shapes))) return rc;
+  ctx->h_vcnt = vcnt;
+  ctx->n_shapes = n_shapes;
+  return MFCL_OK;
+}
+
+static int finishTiming(mfcl_ctx* ctx, int first_ev, int n_ev, int slot0) {
+  if (!ctx->timing) return MFCL_OK;
+  MFCL_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[first_ev + n_ev - 1]));
+  for (int k = 0; k + 1 < n_ev; ++k) {
+    float ms = 0;
+    MFCL_HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[first_ev + k], ctx->ev[first_ev + k + 1]));
+    ctx->last_ms[slot0 + k] = ms;
+  }
+  return MFCL_OK;
+}
+
+int mfcl_convex_query_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                          const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
+                          mfcl_convex_result* out, void* stream) {
+  if (!ctx || !req) return MFCL_E_INVALID;
+  if (n && (!pa || !pb || !posea || !poseb || !out)) {
+    ctx->err = "null device pointer";
+    return MFCL_E_INVALID;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  ctx->counters[0] = n;
+  const int rc = convexQueryLaunch(ctx, pa, pb, posea, poseb, guess, n, req, out, s);
+  if (rc) return rc;
+  if (ctx->timing && n) return finishTiming(ctx, 0, 4, 0);
+  return MFCL_OK;
+}
+
+int mfcl_convex_query(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                      const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
+                      mfcl_convex_result* out) {
+  if (!ctx || !req) return MFCL_E_INVALID;
+  if (n == 0) return MFCL_OK;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (uint64_t i = 0; i < n; ++i)
+    if (pa[i] >= ctx->n_shapes || pb[i] >= ctx->n_shapes) {
+      ctx->err = "pair " + std::to_string(i) + " references a shape id out of range";
+      return MFCL_E_INVALID;
+    }
+  const size_t b_ids = 4 * n, b_pose = 28 * n, b_guess = req->use_guess ? 12 * n : 0,
+               b_out = sizeof(mfcl_convex_result) * n;
+  const size_t total = 2 * b_ids + 2 * b_pose + b_guess + b_out + 6 * 256;
+  if (int rc = ensure(ctx, ctx->ws_io, total)) return rc;
+  char* base = static_cast<char*>(ctx->ws_io.p);
+  auto carve = [&](size_t bytes) {
+    char* p = base;
+    base += (bytes + 255) & ~static_cast<size_t>(255);
+    return p;
+  };
+  uint32_t* dpa = reinterpret_cast<uint32_t*>(carve(b_ids));
+  uint32_t* dpb = reinterpret_cast<uint32_t*>(carve(b_ids));
+  float* dqa = reinterpret_cast<float*>(carve(b_pose));
+  float* dqb = reinterpret_cast<float*>(carve(b_pose));
+  float* dg = b_guess ? reinterpret_cast<float*>(carve(b_guess)) : nullptr;
+  mfcl_convex_result* dout = reinterpret_cast<mfcl_convex_result*>(carve(b_out));
+  hipStream_t s = ctx->stream;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpa, pa, b_ids, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpb, pb, b_ids, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqa, posea, b_pose, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqb, poseb, b_pose, hipMemcpyHostToDevice, s));
+  if (dg) MFCL_HIP_TRY(ctx, hipMemcpyAsync(dg, guess, b_guess, hipMemcpyHostToDevice, s));
+  int rc = mfcl_convex_query_dev(ctx, dpa, dpb, dqa, dqb, dg, n, req, dout, s);
+  if (rc) return rc;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(out, dout, b_out, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return MFCL_OK;
+}
+
+int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* vert_off, const uint32_t* vert_cnt,
+                       const uint32_t* tris, const uint64_t* tri_off, const uint32_t* tri_cnt,
+                       uint32_t n_meshes) {
+  if (!ctx || !verts_xyz || !vert_off || !vert_cnt || !tris || !tri_off || !tri_cnt || n_meshes == 0)
+    return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  freeMeshes(ctx);
+  return buildAndUploadMeshes(ctx, verts_xyz, vert_off, vert_cnt, tris, tri_off, tri_cnt, n_meshes);
+}
+
+int mfcl_mesh_num_nodes(mfcl_ctx* ctx, uint32_t mesh, uint32_t* n_nodes) {
+  if (!ctx || !n_nodes || mesh >= ctx->n_meshes) return MFCL_E_INVALID;
+  *n_nodes = ctx->mesh_host.node_count[mesh];
+  return MFCL_OK;
+}
+
+int mfcl_mesh_collide_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                          const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
+                          int32_t* contacts, void* stream) {
+  if (!ctx) return MFCL_E_INVALID;
+  if (max_contacts < 1 || max_contacts > 1024) {
+    ctx->err = "max_contacts must be in [1, 1024]";
+    return MFCL_E_INVALID;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  const int rc = meshCollideLaunch(ctx, pa, pb, posea, poseb, n, max_contacts, out, contacts, s);
+  if (rc) return rc;
+  if (ctx->timing && n) return finishTiming(ctx, 4, 2, 3);
+  return MFCL_OK;
+}
+
+int mfcl_mesh_collide(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                      const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
+                      int32_t* contacts) {
+  if (!ctx) return MFCL_E_INVALID;
+  if (n == 0) return MFCL_OK;
+  if (max_contacts < 1 || max_contacts > 1024) {
+    ctx->err = "max_contacts must be in [1, 1024]";
+    return MFCL_E_INVALID;
+  }
+  for (uint64_t i = 0; i < n; ++i)
+    if (pa[i] >= ctx->n_meshes || pb[i] >= ctx->n_meshes) {
+      ctx->err = "pair " + std::to_string(i) + " references a mesh id out of range";
+      return MFCL_E_INVALID;
+    }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t b_ids = 4 * n, b_pose = 28 * n, b_out = sizeof(mfcl_mesh_result) * n,
+               b_c = sizeof(int32_t) * 2 * max_contacts * n;
+  if (int rc = ensure(ctx, ctx->ws_io, 2 * b_ids + 2 * b_pose + b_out + b_c + 6 * 256)) return rc;
+  char* base = static_cast<char*>(ctx->ws_io.p);
+  auto carve = [&](size_t bytes) {
+    char* p = base;
+    base += (bytes + 255) & ~static_cast<size_t>(255);
+    return p;
+  };
+  uint32_t* dpa = reinterpret_cast<uint32_t*>(carve(b_ids));
+  uint32_t* dpb = reinterpret_cast<uint32_t*>(carve(b_ids));
+  float* dqa = reinterpret_cast<float*>(carve(b_pose));
+  float* dqb = reinterpret_cast<float*>(carve(b_pose));
+  mfcl_mesh_result* dout = reinterpret_cast<mfcl_mesh_result*>(carve(b_out));
+  int32_t* dc = reinterpret_cast<int32_t*>(carve(b_c));
+  hipStream_t s = ctx->stream;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpa, pa, b_ids, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpb, pb, b_ids, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqa, posea, b_pose, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqb, poseb, b_pose, hipMemcpyHostToDevice, s));
+  int rc = mfcl_mesh_collide_dev(ctx, dpa, dpb, dqa, dqb, n, max_contacts, dout, dc, s);
+  if (rc) return rc;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(out, dout, b_out, hipMemcpyDeviceToHost, s));
+  if (contacts) MFCL_HIP_TRY(ctx, hipMemcpyAsync(contacts, dc, b_c, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return MFCL_OK;
+}
+
+int mfcl_aabbtree_build(mfcl_ctx* ctx, const float* mn3, const float* mx3, const int64_t* ids, uint32_t n) {
+  if (!ctx || !mn3 || !mx3 || !ids || n == 0) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return buildAndUploadAabbTree(ctx, mn3, mx3, ids, n);
+}
+
+int mfcl_aabbtree_query_dev(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uint64_t nq, int32_t cap,
+                            int32_t* counts, int64_t* ids, void* stream) {
+  if (!ctx) return MFCL_E_INVALID;
+  if (cap < 0) {
+    ctx->err = "cap must be >= 0";
+    return MFCL_E_INVALID;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  const int rc = aabbQueryLaunch(ctx, qmn3, qmx3, nq, cap, counts, ids, s);
+  if (rc) return rc;
+  if (ctx->timing && nq) return finishTiming(ctx, 6, 2, 4);
+  return MFCL_OK;
+}
+
+int mfcl_aabbtree_query(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uint64_t nq, int32_t cap,
+                        int32_t* counts, int64_t* ids) {
+  if (!ctx) return MFCL_E_INVALID;
+  if (nq == 0) return MFCL_OK;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t b_box = 12 * nq, b_cnt = 4 * nq, b_ids = sizeof(int64_t) * (cap > 0 ? cap : 1) * nq;
+  if (int rc = ensure(ctx, ctx->ws_io, 2 * b_box + b_cnt + b_ids + 4 * 256)) return rc;
+  char* base = static_cast<char*>(ctx->ws_io.p);
+  auto carve = [&](size_t bytes) {
+    char* p = base;
+    base += (bytes + 255) & ~static_cast<size_t>(255);
+    return p;
+  };
+  float* dmn = reinterpret_cast<float*>(carve(b_box));
+  float* dmx = reinterpret_cast<float*>(carve(b_box));
+  int32_t* dcnt = reinterpret_cast<int32_t*>(carve(b_cnt));
+  int64_t* dids = reinterpret_cast<int64_t*>(carve(b_ids));
+  hipStream_t s = ctx->stream;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dmn, qmn3, b_box, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dmx, qmx3, b_box, hipMemcpyHostToDevice, s));
+  int rc = mfcl_aabbtree_query_dev(ctx, dmn, dmx, nq, cap, dcnt, dids, s);
+  if (rc) return rc;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(counts, dcnt, b_cnt, hipMemcpyDeviceToHost, s));
+  if (ids && cap > 0) MFCL_HIP_TRY(ctx, hipMemcpyAsync(ids, dids, b_ids, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return MFCL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,706 @@
+// Device EPA (include/fcl/cvx_collide/epa.hpp, epa_polytope*.hpp,
+// epa_simplex2polytope.hpp, epa_polytope_utils.h).
+//
+// GPU representation: each worker lane owns an arena in device memory that
+// holds the polytope as three index-linked element lists (newest element
+// first) plus free lists, 16-bit indices, FP64 geometry.  Scans visit the
+// elements in the reference's list order, so ties resolve identically.
+// The arena also carries the LIFO stack of the visibility flood fill.
+#pragma once
+
+#include "narrow_dev.h"
+
+namespace mfcl {
+
+constexpr int16_t kNil = -1;
+
+struct EVert {
+  double vx, vy, vz;  // vertex on the Minkowski difference
+  double dx, dy, dz;  // direction that produced it
+  int16_t next, mnext, new_edge;
+  uint8_t remove, pad;
+};
+struct EEdge {
+  double d2, wx, wy, wz;  // squared distance to origin, witness
+  int16_t v0, v1, f0, f1, next, mnext;
+  uint8_t in, vis;
+  uint16_t pad;
+};
+struct EFace {
+  double d2, wx, wy, wz;
+  int16_t e0, e1, e2, next, mnext;
+  uint8_t in, vis;
+  uint32_t pad;
+};
+
+// Arena sizing for a given face capacity: vertices = edges = floor(1.51 F)
+// (epa_polytope.hpp:90-95), stack = 3F + 3 entries.
+struct ArenaDims {
+  int nv, nf, nstack;
+};
+__host__ __device__ inline ArenaDims arenaDims(int max_faces) {
+  ArenaDims d;
+  d.nv = static_cast<int>(1.51 * static_cast<double>(max_faces));
+  d.nf = max_faces;
+  d.nstack = 3 * max_faces + 3;
+  return d;
+}
+__host__ __device__ inline size_t arenaBytes(int max_faces) {
+  const ArenaDims d = arenaDims(max_faces);
+  size_t b = sizeof(EVert) * d.nv + sizeof(EEdge) * d.nv + sizeof(EFace) * d.nf + sizeof(int32_t) * d.nstack;
+  return (b + 255) & ~static_cast<size_t>(255);
+}
+
+enum EVis : uint8_t { kUnknown = 0, kBorder = 1, kInternal = 2 };
+enum FVis : uint8_t { kFUnknown = 0, kVisible = 1, kHidden = 2 };
+enum EpaStatus : int { kEpaFailed = 0, kEpaOK = 1, kEpaTouching = 2, kEpaIterLimit = 3, kEpaMallocFailed = 4 };
+
+struct MinD {
+  bool in;
+  double d2;
+  d3 w;
+};
+
+// epa_polytope_utils.h:19-52
+MFCL_D void segDist(d3 p, d3 x0, d3 x1, MinD& o) {
+  const d3 d = x1 - x0;
+  const d3 a = x0 - p;
+  const double l2 = sqnorm(d);
+  const double t = -1.0 * dot(a, d) / l2;
+  if (t <= 0.0) {
+    o.w = x0;
+    o.in = false;
+    o.d2 = sqnorm(x0 - p);
+  } else if (t >= 1.0) {
+    o.w = x1;
+    o.in = false;
+    o.d2 = sqnorm(x1 - p);
+  } else {
+    o.w = x0 + t * d;
+    o.in = true;
+    o.d2 = sqnorm(o.w);
+  }
+}
+MFCL_D double lineDist(d3 p, d3 a, d3 b) {
+  const d3 ab = b - a;
+  const double l = norm(ab);
+  if (l <= 0.0) return norm(p - a);
+  return norm(cross(p - a, ab)) / l;
+}
+MFCL_D double planeDist(d3 p, d3 a, d3 b, d3 c) {
+  const d3 n = cross(a - b, b - c);
+  const double l = norm(n);
+  if (l <= 0.0) return lineDist(p, a, b);
+  return fabs(dot(n / l, a - p));
+}
+// epa_polytope_utils.h:92-166 with p = origin.
+MFCL_D void triDist0(d3 a, d3 b, d3 c, MinD& o) {
+  const d3 dl0 = a - b, dl1 = b - c, dl2 = c - a;
+  const d3 n = cross(dl0, dl1);
+  const double n2 = sqnorm(n);
+  const double area = sqrt(n2);
+  bool sub = fabs(area) <= 1e-16;
+  if (!sub) {
+    const d3 proj = n * (dot(a, n) / n2);  // p + n * ((a - p).n / n2) with p = 0
+    const double w0 = norm(cross(dl1, b - proj)) / area;
+    const double w1 = norm(cross(dl2, c - proj)) / area;
+    const double w2 = 1.0 - w0 - w1;
+    bool in = !(w0 < 0.0 || w0 > 1.0) && !(w1 < 0.0 || w1 > 1.0) && !(w2 < 0.0 || w2 > 1.0);
+    const double chk = norm(cross(dl0, a - proj)) / area;
+    if (fabs(chk - w2) > 1e-3) in = false;
+    if (in) {
+      o.in = true;
+      o.d2 = sqnorm(proj);
+      o.w = a * w0 + b * w1 + c * w2;
+      return;
+    }
+    sub = true;
+  }
+  const d3 z = {0, 0, 0};
+  MinD s0, s1, s2;
+  segDist(z, a, b, s0);
+  segDist(z, b, c, s1);
+  segDist(z, c, a, s2);
+  MinD best = s0;
+  if (s1.d2 < best.d2) best = s1;
+  if (s2.d2 < best.d2) best = s2;
+  o.in = false;
+  o.d2 = best.d2;
+  o.w = best.w;
+}
+
+struct Poly {
+  EVert* V;
+  EEdge* E;
+  EFace* F;
+  int32_t* stk;
+  int nv, nf;
+  int16_t vl, el, fl, vfree, efree, ffree;
+
+  MFCL_D d3 vloc(int16_t i) const { return {V[i].vx, V[i].vy, V[i].vz}; }
+  MFCL_D d3 vdir(int16_t i) const { return {V[i].dx, V[i].dy, V[i].dz}; }
+
+  // epa_polytope.hpp:77-125
+  MFCL_D void reset() {
+    vl = el = fl = kNil;
+    vfree = efree = ffree = kNil;
+    for (int i = nv - 1; i >= 0; --i) { V[i].mnext = vfree; vfree = static_cast<int16_t>(i); }
+    for (int i = nv - 1; i >= 0; --i) { E[i].mnext = efree; efree = static_cast<int16_t>(i); }
+    for (int i = nf - 1; i >= 0; --i) { F[i].mnext = ffree; ffree = static_cast<int16_t>(i); }
+  }
+  MFCL_D int16_t addVertex(d3 v, d3 d) {
+    if (vfree == kNil) return kNil;
+    const int16_t i = vfree;
+    EVert& x = V[i];
+    vfree = x.mnext;
+    x.vx = v.x; x.vy = v.y; x.vz = v.z;
+    x.dx = d.x; x.dy = d.y; x.dz = d.z;
+    x.next = vl;
+    vl = i;
+    return i;
+  }
+  MFCL_D int16_t addEdge(int16_t a, int16_t b) {
+    if (a == kNil || b == kNil || efree == kNil) return kNil;
+    const int16_t i = efree;
+    EEdge& e = E[i];
+    efree = e.mnext;
+    e.v0 = a;
+    e.v1 = b;
+    e.f0 = e.f1 = kNil;
+    MinD m;
+    segDist(d3{0, 0, 0}, vloc(a), vloc(b), m);
+    e.in = m.in; e.d2 = m.d2; e.wx = m.w.x; e.wy = m.w.y; e.wz = m.w.z;
+    e.next = el;
+    el = i;
+    return i;
+  }
+  MFCL_D int16_t addFace(int16_t e1, int16_t e2, int16_t e3) {
+    if (e1 == kNil || e2 == kNil || e3 == kNil || ffree == kNil) return kNil;
+    const int16_t i = ffree;
+    EFace& f = F[i];
+    ffree = f.mnext;
+    f.e0 = e1; f.e1 = e2; f.e2 = e3;
+    const int16_t a = E[e1].v0, b = E[e1].v1;
+    const int16_t c = (E[e2].v0 != a && E[e2].v0 != b) ? E[e2].v0 : E[e2].v1;
+    MinD m;
+    triDist0(vloc(a), vloc(b), vloc(c), m);
+    f.in = m.in; f.d2 = m.d2; f.wx = m.w.x; f.wy = m.w.y; f.wz = m.w.z;
+    f.next = fl;
+    fl = i;
+    const int16_t es[3] = {e1, e2, e3};
+    for (int k = 0; k < 3; ++k) {
+      EEdge& ek = E[es[k]];
+      if (ek.f0 == kNil) {
+        ek.f0 = i;
+      } else {
+        if (ek.f1 != kNil) return kNil;
+        ek.f1 = i;
+      }
+    }
+    return i;
+  }
+  MFCL_D void faceVerts(int16_t fi, int16_t& a, int16_t& b, int16_t& c) const {
+    const EFace& f = F[fi];
+    a = E[f.e0].v0;
+    b = E[f.e0].v1;
+    c = (E[f.e1].v0 != a && E[f.e1].v0 != b) ? E[f.e1].v0 : E[f.e1].v1;
+  }
+  // epa_polytope.hpp:292-345; kind: 0 none, 1 vertex, 2 edge, 3 face
+  MFCL_D void minToOrigin(bool excl_vertex, int& kind, int16_t& idx) const {
+    kind = 0;
+    idx = kNil;
+    if (vl == kNil) return;
+    double best = INFINITY;
+    if (!excl_vertex)
+      for (int16_t i = vl; i != kNil; i = V[i].next) {
+        const double d2 = sqnorm(vloc(i));
+        if (d2 < best) { best = d2; kind = 1; idx = i; }
+      }
+    for (int16_t i = el; i != kNil; i = E[i].next) {
+      if (!excl_vertex && !E[i].in) continue;
+      if (E[i].d2 < best) { best = E[i].d2; kind = 2; idx = i; }
+    }
+    for (int16_t i = fl; i != kNil; i = F[i].next) {
+      if (!F[i].in) continue;
+      if (F[i].d2 < best) { best = F[i].d2; kind = 3; idx = i; }
+    }
+  }
+  // epa_polytope.hpp:347-405
+  MFCL_D bool faceNormalOut(int16_t fi, d3& normal, double* area) const {
+    int16_t ia, ib, ic;
+    faceVerts(fi, ia, ib, ic);
+    const d3 a = vloc(ia), b = vloc(ib), c = vloc(ic);
+    const d3 n = cross(a - b, b - c);
+    const double nn = norm(n);
+    if (area) *area = 0.5 * nn;
+    if (nn <= 0.0) return false;
+    const d3 dir = n / nn;
+    const double od = dot(a, dir);
+    if (fabs(od) >= 1e-4) {
+      normal = od > 0 ? dir : -dir;
+      return true;
+    }
+    double maxp = 0.0, minn = 0.0;
+    for (int16_t i = vl; i != kNil; i = V[i].next) {
+      const double dv = dot(vloc(i), dir);
+      if (dv > maxp) maxp = dv;
+      if (dv < minn) minn = dv;
+    }
+    normal = fabs(maxp) > fabs(minn) ? -dir : dir;
+    return true;
+  }
+  MFCL_D d3 faceAnchor(int16_t fi) const { return vloc(E[F[fi].e0].v0); }
+  // epa_polytope_expand.hpp:12-30
+  MFCL_D bool outside(int16_t fi, d3 p) const {
+    d3 n;
+    double area = 0;
+    if (!faceNormalOut(fi, n, &area)) return area <= 0.0;
+    return dot(n, p - faceAnchor(fi)) >= 0.0;
+  }
+
+  // epa_polytope_expand.hpp:32-281. 0 ok, 1 failed, 2 malloc failed
+  MFCL_D int expand(d3 nv, d3 nd, int16_t start) {
+    for (int16_t i = vl; i != kNil; i = V[i].next) { V[i].remove = 1; V[i].new_edge = kNil; }
+    for (int16_t i = el; i != kNil; i = E[i].next) E[i].vis = kUnknown;
+    for (int16_t i = fl; i != kNil; i = F[i].next) F[i].vis = kFUnknown;
+    int sp = 0;
+    F[start].vis = kVisible;
+    for (int k = 0; k < 3; ++k) stk[sp++] = (static_cast<int32_t>(start) << 2) | k;
+    while (sp > 0) {
+      const int32_t t = stk[--sp];
+      const int16_t fi = static_cast<int16_t>(t >> 2);
+      const int k = t & 3;
+      const int16_t ei = k == 0 ? F[fi].e0 : (k == 1 ? F[fi].e1 : F[fi].e2);
+      if (ei == kNil) return 1;
+      EEdge& e = E[ei];
+      const int16_t g = (e.f0 == fi) ? e.f1 : e.f0;
+      if (g == kNil) return 1;
+      if (F[g].vis == kVisible) { e.vis = kInternal; continue; }
+      if (F[g].vis == kHidden) { e.vis = kBorder; continue; }
+      if (outside(g, nv)) {
+        F[g].vis = kVisible;
+        e.vis = kInternal;
+        if (sp + 3 > 3 * nf + 3) return 1;  // cannot happen: each face is pushed once
+        for (int q = 0; q < 3; ++q) stk[sp++] = (static_cast<int32_t>(g) << 2) | q;
+      } else {
+        F[g].vis = kHidden;
+        e.vis = kBorder;
+      }
+    }
+    for (int16_t i = el; i != kNil; i = E[i].next)
+      if (E[i].vis != kInternal) { V[E[i].v0].remove = 0; V[E[i].v1].remove = 0; }
+    {
+      int16_t prev = kNil, cur = fl;
+      while (cur != kNil) {
+        EFace& f = F[cur];
+        const int16_t nx = f.next;
+        if (f.vis == kVisible) {
+          const int16_t es[3] = {f.e0, f.e1, f.e2};
+          for (int q = 0; q < 3; ++q) {
+            EEdge& ek = E[es[q]];
+            if (ek.f0 == cur) ek.f0 = ek.f1;
+            ek.f1 = kNil;
+          }
+          if (prev == kNil) fl = nx; else F[prev].next = nx;
+          f.next = kNil;
+          f.mnext = ffree;
+          ffree = cur;
+        } else {
+          prev = cur;
+        }
+        cur = nx;
+      }
+    }
+    {
+      int16_t prev = kNil, cur = el;
+      while (cur != kNil) {
+        EEdge& e = E[cur];
+        const int16_t nx = e.next;
+        if (e.vis == kInternal) {
+          if (prev == kNil) el = nx; else E[prev].next = nx;
+          e.next = kNil;
+          e.mnext = efree;
+          efree = cur;
+        } else {
+          prev = cur;
+        }
+        cur = nx;
+      }
+    }
+    {
+      int16_t prev = kNil, cur = vl;
+      while (cur != kNil) {
+        EVert& v = V[cur];
+        const int16_t nx = v.next;
+        if (v.remove) {
+          if (prev == kNil) vl = nx; else V[prev].next = nx;
+          v.next = kNil;
+          v.mnext = vfree;
+          vfree = cur;
+        } else {
+          prev = cur;
+        }
+        cur = nx;
+      }
+    }
+    const int16_t nvi = addVertex(nv, nd);
+    if (nvi == kNil) return 2;
+    bool ok = true;
+    for (int16_t i = el; i != kNil; i = E[i].next) {
+      if (E[i].vis != kBorder) continue;
+      int16_t ne[2];
+      const int16_t vs[2] = {E[i].v0, E[i].v1};
+      for (int q = 0; q < 2; ++q) {
+        EVert& vq = V[vs[q]];
+        if (vq.new_edge == kNil) vq.new_edge = addEdge(nvi, vs[q]);
+        ne[q] = vq.new_edge;
+      }
+      if (addFace(i, ne[0], ne[1]) == kNil) ok = false;
+    }
+    return ok ? 0 : 2;
+  }
+};
+
+// Result of the EPA stage in shape-0 frame.
+struct EpaOut {
+  int status;
+  double depth;
+  d3 p0, p1;
+  int iters;
+};
+
+MFCL_D void touchingPoint(Mink& md, d3 d, EpaOut& o) {
+  const d3 mid = (md.support0(d) + md.support1(-d)) / 2;
+  o.p0 = mid;
+  o.p1 = mid;
+}
+
+// epa_simplex2polytope.hpp:184-217; returns 0 ok, 2 failed
+MFCL_D int tetra(Poly& P, const MV& a, const MV& b, const MV& c, const MV& d) {
+  P.reset();
+  int16_t v[4], e[6], f[4];
+  v[0] = P.addVertex(a.v, a.d);
+  v[1] = P.addVertex(b.v, b.d);
+  v[2] = P.addVertex(c.v, c.d);
+  v[3] = P.addVertex(d.v, d.d);
+  e[0] = P.addEdge(v[0], v[1]);
+  e[1] = P.addEdge(v[1], v[2]);
+  e[2] = P.addEdge(v[2], v[0]);
+  e[3] = P.addEdge(v[3], v[0]);
+  e[4] = P.addEdge(v[3], v[1]);
+  e[5] = P.addEdge(v[3], v[2]);
+  f[0] = P.addFace(e[0], e[1], e[2]);
+  f[1] = P.addFace(e[3], e[4], e[0]);
+  f[2] = P.addFace(e[4], e[5], e[1]);
+  f[3] = P.addFace(e[5], e[3], e[2]);
+  for (int i = 0; i < 4; ++i)
+    if (f[i] == kNil) return 2;
+  return 0;
+}
+
+// s2p status: 0 OK, 1 touching, 2 failed
+MFCL_D int s2p3(Poly& P, Mink& md, const MV& a, const MV& b, const MV& c, double thr, EpaOut& o) {
+  d3 n = cross(b.v - a.v, c.v - a.v);
+  if (sqnorm(n) <= 0.0) return 2;
+  n = normalized(n);
+  const d3 d0 = md.support(n);
+  const double t0 = planeDist(d0, a.v, b.v, c.v);
+  const d3 dir1 = -n;
+  const d3 d1 = md.support(dir1);
+  const double t1 = planeDist(d1, a.v, b.v, c.v);
+  if (t0 <= thr) { touchingPoint(md, n, o); return 1; }
+  if (t1 <= thr) { touchingPoint(md, dir1, o); return 1; }
+  if (t0 > t1) return tetra(P, a, b, c, MV{d0, n});
+  return tetra(P, a, b, c, MV{d1, dir1});
+}
+
+MFCL_D int s2p2(Poly& P, Mink& md, const MV& a, const MV& b, double thr, EpaOut& o) {
+  const double thr2 = thr * thr;
+  const d3 ab = b.v - a.v;
+  if (sqnorm(ab) < thr2) return 2;
+  const d3 abu = normalized(ab);
+  d3 dinit = {abu.y, -abu.x, 0};
+  bool valid = sqnorm(dinit) > thr2;
+  if (!valid) { dinit = d3{abu.z, 0, -abu.x}; valid = sqnorm(dinit) > thr2; }
+  if (!valid) { dinit = d3{0, abu.z, -abu.y}; valid = sqnorm(dinit) > thr2; }
+  if (!valid) return 2;
+  dinit = normalized(dinit);
+  d3 dcur = dinit, dfound = {0, 0, 0}, vfound = {0, 0, 0};
+  bool found = false;
+  for (int i = 0; i < 72; ++i) {
+    const d3 vcur = md.support(dcur);
+    if (lineDist(vcur, a.v, b.v) > thr) {
+      vfound = vcur;
+      dfound = dcur;
+      found = true;
+      break;
+    }
+    const double step = 2.0 * 3.145926 / 72;  // the reference's pi constant (sic)
+    dcur = normalized(dcur + step * cross(abu, vcur));
+  }
+  if (!found) { touchingPoint(md, dinit, o); return 1; }
+  const MV v0 = {vfound, dfound};
+  MV v1;
+  v1.d = -v0.d;
+  v1.v = md.support(v1.d);
+  if (lineDist(v1.v, a.v, b.v) < thr) { touchingPoint(md, v1.d, o); return 1; }
+  const d3 n = cross(v0.v - a.v, v1.v - a.v);
+  if (sqnorm(n) < thr2) return 2;
+  MV v2;
+  v2.d = normalized(n);
+  v2.v = md.support(v2.d);
+  if (lineDist(v2.v, a.v, b.v) < thr) { touchingPoint(md, v2.d, o); return 1; }
+  MV v3;
+  v3.d = -v2.d;
+  v3.v = md.support(v3.d);
+  if (lineDist(v3.v, a.v, b.v) < thr) { touchingPoint(md, v3.d, o); return 1; }
+  P.reset();
+  int16_t v[6], e[12], f[8];
+  v[0] = P.addVertex(a.v, a.d);
+  v[1] = P.addVertex(v0.v, v0.d);
+  v[2] = P.addVertex(b.v, b.d);
+  v[3] = P.addVertex(v1.v, v1.d);
+  v[4] = P.addVertex(v2.v, v2.d);
+  v[5] = P.addVertex(v3.v, v3.d);
+  e[0] = P.addEdge(v[0], v[1]);
+  e[1] = P.addEdge(v[1], v[2]);
+  e[2] = P.addEdge(v[2], v[3]);
+  e[3] = P.addEdge(v[3], v[0]);
+  e[4] = P.addEdge(v[4], v[0]);
+  e[5] = P.addEdge(v[4], v[1]);
+  e[6] = P.addEdge(v[4], v[2]);
+  e[7] = P.addEdge(v[4], v[3]);
+  e[8] = P.addEdge(v[5], v[0]);
+  e[9] = P.addEdge(v[5], v[1]);
+  e[10] = P.addEdge(v[5], v[2]);
+  e[11] = P.addEdge(v[5], v[3]);
+  f[0] = P.addFace(e[4], e[5], e[0]);
+  f[1] = P.addFace(e[5], e[6], e[1]);
+  f[2] = P.addFace(e[6], e[7], e[2]);
+  f[3] = P.addFace(e[7], e[4], e[3]);
+  f[4] = P.addFace(e[8], e[9], e[0]);
+  f[5] = P.addFace(e[9], e[10], e[1]);
+  f[6] = P.addFace(e[10], e[11], e[2]);
+  f[7] = P.addFace(e[11], e[8], e[3]);
+  for (int i = 0; i < 8; ++i)
+    if (f[i] == kNil) return 2;
+  return 0;
+}
+
+// epa_simplex2polytope.hpp:46-110
+MFCL_D int simplexToPolytope(Poly& P, Mink& md, const Smp& s, double thr, EpaOut& o) {
+  const double thr2 = thr * thr;
+  if (s.rank > 0 && sqnorm(s.a.v) <= thr2) { touchingPoint(md, s.a.d, o); return 1; }
+  if (s.rank > 1 && sqnorm(s.b.v) <= thr2) { touchingPoint(md, s.b.d, o); return 1; }
+  if (s.rank > 2 && sqnorm(s.c.v) <= thr2) { touchingPoint(md, s.c.d, o); return 1; }
+  if (s.rank > 3 && sqnorm(s.e.v) <= thr2) { touchingPoint(md, s.e.d, o); return 1; }
+  if (s.rank == 4) {
+    const d3 z = {0, 0, 0};
+    if (planeDist(z, s.a.v, s.b.v, s.c.v) < thr) return s2p3(P, md, s.a, s.b, s.c, thr, o);
+    if (planeDist(z, s.a.v, s.c.v, s.e.v) < thr) return s2p3(P, md, s.a, s.c, s.e, thr, o);
+    if (planeDist(z, s.a.v, s.b.v, s.e.v) < thr) return s2p3(P, md, s.a, s.b, s.e, thr, o);
+    if (planeDist(z, s.b.v, s.c.v, s.e.v) < thr) return s2p3(P, md, s.b, s.c, s.e, thr, o);
+    return tetra(P, s.a, s.b, s.c, s.e);
+  }
+  if (s.rank == 3) return s2p3(P, md, s.a, s.b, s.c, thr, o);
+  if (s.rank == 2) return s2p2(P, md, s.a, s.b, thr, o);
+  o.p0 = md.support0(s.a.d);
+  o.p1 = md.support1(-s.a.d);
+  return 1;
+}
+
+struct RawF {
+  int kind;
+  MinD m;
+  MV a, b, c;
+};
+
+MFCL_D void toRaw(const Poly& P, int kind, int16_t i, RawF& r) {
+  r.kind = kind;
+  if (kind == 1) {
+    const d3 v = P.vloc(i);
+    r.m.in = true; r.m.d2 = sqnorm(v); r.m.w = v;
+    r.a = MV{v, P.vdir(i)};
+  } else if (kind == 2) {
+    const EEdge& e = P.E[i];
+    r.m.in = e.in; r.m.d2 = e.d2; r.m.w = d3{e.wx, e.wy, e.wz};
+    r.a = MV{P.vloc(e.v0), P.vdir(e.v0)};
+    r.b = MV{P.vloc(e.v1), P.vdir(e.v1)};
+  } else {
+    const EFace& f = P.F[i];
+    r.m.in = f.in; r.m.d2 = f.d2; r.m.w = d3{f.wx, f.wy, f.wz};
+    int16_t ia, ib, ic;
+    P.faceVerts(i, ia, ib, ic);
+    r.a = MV{P.vloc(ia), P.vdir(ia)};
+    r.b = MV{P.vloc(ib), P.vdir(ib)};
+    r.c = MV{P.vloc(ic), P.vdir(ic)};
+  }
+}
+
+// epa.hpp:11-113; returns 0 ok, 1 failed, 2 converge
+MFCL_D int nextDir(const Poly& P, Mink& md, int kind, int16_t fi, double tol, d3& nd, d3& nv, int16_t& start) {
+  constexpr double kOuter = 1e-3 * 1e-3;
+  auto face_cand = [&](int16_t f, bool try_w) -> int {
+    d3 fn;
+    if (!P.faceNormalOut(f, fn, nullptr)) return 1;
+    const d3 on = P.faceAnchor(f);
+    const EFace& F = P.F[f];
+    if (try_w && F.d2 > kOuter) {
+      nd = normalized(d3{F.wx, F.wy, F.wz});
+      nv = md.support(nd);
+      if (dot(fn, nv - on) >= 0.0) { start = f; return 0; }
+    }
+    nd = fn;
+    nv = md.support(nd);
+    if (dot(fn, nv - on) > tol) { start = f; return 0; }
+    return 2;
+  };
+  if (kind == 3) return face_cand(fi, true);
+  if (kind == 2) {
+    const EEdge& e = P.E[fi];
+    const int16_t f0 = e.f0, f1 = e.f1;
+    if (f0 == kNil || f1 == kNil) return 1;  // open edge: malformed polytope
+    if (e.d2 > kOuter) {
+      nd = normalized(d3{e.wx, e.wy, e.wz});
+      nv = md.support(nd);
+      if (P.outside(f0, nv)) { start = f0; return 0; }
+      if (P.outside(f1, nv)) { start = f1; return 0; }
+    }
+    const int s0 = face_cand(f0, false);
+    if (s0 == 0) return 0;
+    return face_cand(f1, false);
+  }
+  return 1;
+}
+
+// epa.hpp:269-299 (edge case: squared norm of the segment projection, as in the reference)
+MFCL_D bool epaTerminate(const RawF& f, d3 nv, double tol) {
+  double d2;
+  if (f.kind == 2) {
+    MinD m;
+    segDist(nv, f.a.v, f.b.v, m);
+    d2 = m.d2;
+  } else {
+    const double t = planeDist(nv, f.a.v, f.b.v, f.c.v);
+    d2 = t * t;
+  }
+  return d2 < tol * tol;
+}
+
+// epa.hpp:341-498
+MFCL_D void assignPair(Mink& md, d3 cand_d, const RawF& f, EpaOut& o) {
+  if (f.kind == 1) {
+    o.p0 = md.support0(f.a.d);
+    o.p1 = md.support1(-f.a.d);
+    o.depth = norm(f.a.v);
+    return;
+  }
+  if (f.kind == 2) {
+    o.depth = sqrt(f.m.d2);
+    const d3 p = f.m.w, a = f.a.v, b = f.b.v;
+    const d3 ab = b - a;
+    int ax = 0;
+    double mx = fabs(ab.x);
+    if (fabs(ab.y) > mx) { ax = 1; mx = fabs(ab.y); }
+    if (fabs(ab.z) > mx) { ax = 2; mx = fabs(ab.z); }
+    if (mx <= 1e-10) {
+      o.p0 = md.support0(f.a.d);
+      o.p1 = md.support1(-f.a.d);
+      return;
+    }
+    const double s = (comp(p, ax) - comp(a, ax)) / (comp(b, ax) - comp(a, ax));
+    if (!(s < 0 || s > 1.0)) {
+      const double wa = 1.0 - s, wb = s;
+      o.p0 = wa * md.support0(f.a.d) + wb * md.support0(f.b.d);
+      o.p1 = wa * md.support1(-f.a.d) + wb * md.support1(-f.b.d);
+      return;
+    }
+  } else if (f.kind == 3) {
+    o.depth = sqrt(f.m.d2);
+    const d3 a = f.a.v, b = f.b.v, c = f.c.v, p = f.m.w;
+    const d3 dl0 = a - b, dl1 = b - c, dl2 = c - a;
+    const double area = sqrt(sqnorm(cross(dl0, dl1)));
+    const double w0 = norm(cross(dl1, b - p)) / area;
+    const double w1 = norm(cross(dl2, c - p)) / area;
+    const double chk = norm(cross(dl0, a - p)) / area;
+    const double w2 = 1.0 - w0 - w1;
+    if (!(fabs(chk - w2) > 0.01)) {
+      o.p0 = w0 * md.support0(f.a.d) + w1 * md.support0(f.b.d) + w2 * md.support0(f.c.d);
+      o.p1 = w0 * md.support1(-f.a.d) + w1 * md.support1(-f.b.d) + w2 * md.support1(-f.c.d);
+      return;
+    }
+  }
+  const d3 q0 = md.support0(cand_d);
+  const d3 q1 = md.support1(-cand_d);
+  o.p0 = q0;
+  o.p1 = q1;
+  o.depth = norm(q0 - q1);
+}
+
+// epa.hpp:137-257
+MFCL_D void epaEvaluate(Poly& P, Mink& md, const Smp& s, double tol, int max_iter, EpaOut& o) {
+  o.iters = 0;
+  o.depth = 0;
+  o.p0 = d3{0, 0, 0};
+  o.p1 = d3{0, 0, 0};
+  const int s2p = simplexToPolytope(P, md, s, tol, o);
+  if (s2p == 2) { o.status = kEpaFailed; return; }
+  if (s2p == 1) { o.depth = 0.0; o.status = kEpaTouching; return; }
+  int it = 0;
+  RawF raw;
+  while (true) {
+    int kind;
+    int16_t fi;
+    P.minToOrigin(false, kind, fi);
+    if (kind == 0) { o.status = kEpaFailed; return; }
+    if (kind == 1) {
+      const double vd2 = sqnorm(P.vloc(fi));
+      constexpr double kRatio = 1.0 - 1e-3;
+      int k2;
+      int16_t i2;
+      P.minToOrigin(true, k2, i2);
+      if (k2 == 0) { o.status = kEpaFailed; return; }
+      if (k2 == 2) {
+        if (kRatio * P.E[i2].d2 < vd2) { kind = 2; fi = i2; }
+      } else if (k2 == 3) {
+        if (kRatio * P.F[i2].d2 < vd2) { kind = 3; fi = i2; }
+      }
+    }
+    if (kind != 2 && kind != 3) { o.status = kEpaFailed; return; }
+    toRaw(P, kind, fi, raw);
+    d3 nd = {0, 0, 0}, nv = {0, 0, 0};
+    int16_t start = kNil;
+    const int ds = nextDir(P, md, kind, fi, tol, nd, nv, start);
+    if (ds == 1) { o.status = kEpaFailed; return; }
+    if (ds == 2) {
+      assignPair(md, nd, raw, o);
+      o.iters = it;
+      o.status = kEpaOK;
+      return;
+    }
+    if (start == kNil) { o.status = kEpaFailed; return; }
+    if (epaTerminate(raw, nv, tol)) {
+      assignPair(md, nd, raw, o);
+      o.iters = it;
+      o.status = kEpaOK;
+      return;
+    }
+    const int ex = P.expand(nv, nd, start);
+    if (ex == 1) { o.status = kEpaFailed; return; }
+    if (ex == 2) {
+      assignPair(md, nd, raw, o);
+      o.iters = it;
+      o.status = kEpaMallocFailed;
+      return;
+    }
+    ++it;
+    if (it >= max_iter) {
+      assignPair(md, nd, raw, o);
+      o.iters = it;
+      o.status = kEpaIterLimit;
+      return;
+    }
+  }
+}
+
+}  // namespace mfcl

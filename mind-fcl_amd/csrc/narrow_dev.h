@@ -1,0 +1,705 @@
+// Device narrowphase building blocks: support mapping, GJK (boolean and
+// minimum distance), MPR, and the closed-form primitive pair tests.
+//
+// Semantics follow mind-fcl's cvx_collide and GJKSolver (file:line citations
+// on each function); the representation is the GPU's own: one lane per pair,
+// every simplex slot addressed with compile-time indices so the simplex lives
+// in VGPRs, hull vertices read as 16-byte float4 records.
+//
+// Precision: hull support mapping scans vertices in FP32 and tracks the best
+// and runner-up scores; when the FP32 gap is inside a proven rounding bound
+// the scan is redone in FP64, so the selected vertex equals the FP64 argmax.
+// Everything else (simplex sub-distance, portal updates, EPA polytope) is FP64.
+#pragma once
+
+#include "dmath.h"
+
+namespace mfcl {
+
+enum ShapeType : int { kBox = 0, kSphere = 1, kCapsule = 3, kCylinder = 5, kConvex = 100 };
+
+struct DevShape {
+  int type;
+  double p0, p1, p2;    // box: full sides; sphere: r; capsule/cylinder: r, full length
+  const float4* verts;  // convex: vertices (x, y, z, unused)
+  int nv;
+  float rbound;         // convex: max |coordinate| over the vertices
+};
+
+struct ShapeTables {
+  const int* type;
+  const float4* param;  // x, y, z as above; w = rbound for convex
+  const uint32_t* vert_off;
+  const uint32_t* vert_cnt;
+  const float4* verts;
+};
+
+MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
+  DevShape s;
+  s.type = T.type[i];
+  const float4 p = T.param[i];
+  s.p0 = p.x;
+  s.p1 = p.y;
+  s.p2 = p.z;
+  s.rbound = p.w;
+  if (s.type == kConvex) {
+    s.verts = T.verts + T.vert_off[i];
+    s.nv = static_cast<int>(T.vert_cnt[i]);
+  } else {
+    s.verts = nullptr;
+    s.nv = 0;
+  }
+  return s;
+}
+
+// FP64 first argmax of dot(d, v) (the reference's naive scan order).
+MFCL_D int hullArgmax64(const float4* v, int nv, d3 d) {
+  double bd = d.x * static_cast<double>(v[0].x) + d.y * static_cast<double>(v[0].y) + d.z * static_cast<double>(v[0].z);
+  int bi = 0;
+  for (int i = 1; i < nv; ++i) {
+    const float4 q = v[i];
+    const double val = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
+    if (val > bd) {
+      bd = val;
+      bi = i;
+    }
+  }
+  return bi;
+}
+
+// FP32 scan with an FP64 recheck near ties; returns the index of the first
+// vertex attaining the FP64 maximum of dot(d, v) whenever that maximum is
+// separated from the runner-up by more than the FP32 rounding bound, and the
+// FP64 first argmax otherwise.
+MFCL_D int hullArgmax(const DevShape& s, d3 d, int* scanned) {
+#pragma clang fp contract(fast)
+  const float fx = static_cast<float>(d.x), fy = static_cast<float>(d.y), fz = static_cast<float>(d.z);
+  const float4* v = s.verts;
+  float4 q = v[0];
+  float best = fx * q.x + fy * q.y + fz * q.z;
+  float second = -INFINITY;
+  int bi = 0;
+  for (int i = 1; i < s.nv; ++i) {
+    q = v[i];
+    const float val = fx * q.x + fy * q.y + fz * q.z;
+    if (val > best) {
+      second = best;
+      best = val;
+      bi = i;
+    } else if (val > second) {
+      second = val;
+    }
+  }
+  *scanned += s.nv;
+  // |fl(d.v) - d.v| <= (rounding of d to FP32 + 3 FMA roundings) * |d|_1 * max|v|
+  // <= 8 * 2^-24 * sqrt(3) * rbound for |d| = 1; gap test uses twice that.
+  const float bound = 2.0f * 8.0f * 5.9604645e-08f * s.rbound * (fabsf(fx) + fabsf(fy) + fabsf(fz));
+  if (best - second > bound) return bi;
+  *scanned += s.nv;
+  return hullArgmax64(v, s.nv, d);
+}
+
+// include/fcl/cvx_collide/gjk_shape.hpp:41-180; shape_gjk_interface-inl.h:87-92.
+MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int* scanned) {
+  switch (s.type) {
+    case kBox:
+      return {dir.x > 0 ? s.p0 / 2 : -s.p0 / 2, dir.y > 0 ? s.p1 / 2 : -s.p1 / 2, dir.z > 0 ? s.p2 / 2 : -s.p2 / 2};
+    case kSphere:
+      return dir * s.p0;
+    case kCapsule: {
+      const double r = s.p0, hh = s.p1 * 0.5;
+      const d3 v = dir * r;
+      const d3 a = d3{0, 0, hh} + v, b = d3{0, 0, -hh} + v;
+      return dot(dir, a) > dot(dir, b) ? a : b;
+    }
+    case kCylinder: {
+      const double r = s.p0, hh = s.p1 * 0.5;
+      const double zd = sqrt(dir.x * dir.x + dir.y * dir.y);
+      if (zd == 0.0) return {0, 0, dir.z > 0 ? hh : -hh};
+      const double k = r / zd;
+      return {k * dir.x, k * dir.y, dir.z > 0 ? hh : -hh};
+    }
+    case kConvex: {
+      const float4 q = s.verts[hullArgmax(s, dir, scanned)];
+      return {q.x, q.y, q.z};
+    }
+    default:
+      return {0, 0, 0};
+  }
+}
+
+MFCL_D d3 shapeInterior(const DevShape& s, const float4* interior_tab, uint32_t id) {
+  if (s.type == kConvex) {
+    const float4 c = interior_tab[id];
+    return {c.x, c.y, c.z};
+  }
+  return {0, 0, 0};
+}
+
+// MinkowskiDiff (include/fcl/cvx_collide/minkowski_diff.hpp:14-85).
+struct Mink {
+  DevShape s0, s1;
+  m33 toshape1;  // frame 0 -> frame 1 rotation
+  tf3 toshape0;  // frame 1 -> frame 0
+  d3 int0, int1; // interior points in each shape's own frame
+  int scanned;   // hull vertices scanned (for the VALU accounting)
+  MFCL_D d3 support0(d3 d) { return shapeSupport(s0, d, &scanned); }
+  MFCL_D d3 support1(d3 d) { return apply(toshape0, shapeSupport(s1, mul(toshape1, d), &scanned)); }
+  MFCL_D d3 support(d3 d) { return support0(d) - support1(-d); }
+  MFCL_D d3 interior() { return int0 - apply(toshape0, int1); }
+};
+
+MFCL_D void setupMink(Mink& md, const DevShape& a, const tf3& ta, const DevShape& b, const tf3& tb) {
+  md.s0 = a;
+  md.s1 = b;
+  md.toshape1 = tmulm(tb.R, ta.R);
+  md.toshape0 = relTf(ta, tb);
+  md.scanned = 0;
+  md.int0 = d3{0, 0, 0};
+  md.int1 = d3{0, 0, 0};
+}
+
+struct MV {
+  d3 v, d;
+};
+
+// Four fixed simplex slots; every access below uses a constant slot index.
+struct Smp {
+  MV a, b, c, e;  // slots 0..3
+  int rank;
+};
+
+MFCL_D void smpAdd(Smp& s, const MV& m) {
+  if (s.rank < 0) s.rank = 0;
+  if (s.rank == 0) s.a = m;
+  else if (s.rank == 1) s.b = m;
+  else if (s.rank == 2) s.c = m;
+  else s.e = m;
+  s.rank += 1;
+}
+MFCL_D bool smpDup(const Smp& s, d3 v, double tol2) {
+  if (s.rank > 0 && sqnorm(s.a.v - v) < tol2) return true;
+  if (s.rank > 1 && sqnorm(s.b.v - v) < tol2) return true;
+  if (s.rank > 2 && sqnorm(s.c.v - v) < tol2) return true;
+  if (s.rank > 3 && sqnorm(s.e.v - v) < tol2) return true;
+  return false;
+}
+
+enum GjkStatus : int { kGjkIntersect = 0, kGjkSeparated = 1, kGjkNoProgress = 2, kGjkIterLimit = 3, kGjkFailed = 4 };
+enum ProjStatus : int { kProjFailed, kProjContinue, kProjIntersect, kProjZeroVolume };
+
+// gjk.hpp:221-294 on slots (C=a, B=b, A=c).
+MFCL_D int gjkProj3(Smp& s, d3& dir, double tol) {
+  const MV C = s.a, B = s.b, A = s.c;
+  const d3 a = A.v;
+  const d3 ab = B.v - a, ac = C.v - a;
+  const bool sb = dot(a, ab) >= 0, sc = dot(a, ac) >= 0;
+  if (sb && sc) {
+    s.rank = 1;
+    s.a = A;
+    dir = normalized(-a);
+    return kProjContinue;
+  }
+  const d3 n = cross(ab, ac);
+  if (dot(a, cross(n, ac)) <= 0) {
+    s.b = A;
+    s.rank = 2;
+    dir = normalized(cross(cross(a, ac), ac));
+    return kProjContinue;
+  }
+  if (dot(a, cross(ab, n)) <= 0) {
+    s.a = B;
+    s.b = A;
+    s.rank = 2;
+    dir = normalized(cross(cross(a, ab), ab));
+    return kProjContinue;
+  }
+  const double area = norm(n);
+  if (area < tol * tol) return kProjFailed;
+  const d3 nu = n / area;
+  const double nd = dot(nu, a);
+  if (fabs(nd) < tol) return kProjIntersect;
+  dir = nd <= 0 ? nu : -nu;
+  return kProjContinue;
+}
+
+// gjk.hpp:181-363
+MFCL_D int gjkProject(Smp& s, d3& dir, double tol) {
+  if (s.rank == 2) {
+    const MV B = s.a, A = s.b;
+    const d3 a = A.v;
+    const d3 a_to_b = B.v - a;
+    const double ao_ab = -dot(a_to_b, a);
+    const d3 abx = cross(a, a_to_b);
+    if (ao_ab > 0 && sqnorm(abx) <= 0.0) return kProjIntersect;
+    if (ao_ab <= 0) {
+      s.rank = 1;
+      s.a = A;
+      dir = -normalized(a);
+      return kProjContinue;
+    }
+    dir = normalized(cross(abx, a_to_b));
+    return kProjContinue;
+  }
+  if (s.rank == 3) return gjkProj3(s, dir, tol);
+  // rank 4: D=a, C=b, B=c, A=e
+  const MV D = s.a, C = s.b, B = s.c, A = s.e;
+  const d3 a = A.v;
+  const d3 ab = B.v - a, ac = C.v - a, ad = D.v - a;
+  d3 abc = cross(ab, ac), acd = cross(ac, ad), abd = cross(ab, ad);
+  const double abc_ad = dot(abc, ad), acd_ab = dot(acd, ab), abd_ac = dot(abd, ac);
+  if (fabs(abc_ad) <= 0.0) return kProjZeroVolume;
+  if (abc_ad > 0) abc = -abc;
+  if (acd_ab > 0) acd = -acd;
+  if (abd_ac > 0) abd = -abd;
+  const bool ds = dot(a, abc) > 0, cs = dot(a, abd) > 0, bs = dot(a, acd) > 0;
+  if (ds && cs && bs) return kProjIntersect;
+  if (!bs) {
+    s.a = D; s.b = C; s.c = A;
+  } else if (!cs) {
+    s.a = D; s.b = B; s.c = A;
+  } else {
+    s.a = C; s.b = B; s.c = A;
+  }
+  s.rank = 3;
+  return gjkProj3(s, dir, tol);
+}
+
+// ---------------------------------------------------------------- distance
+// gjk_distance.hpp:128-154 on a two-slot simplex (x = old slot 0, y = new slot 1).
+// Writes the result simplex back into (x, y, rank).
+MFCL_D void dist2(MV& x, MV& y, int& rank, d3& out, double tol) {
+  const d3 s1 = y.v, s2 = x.v;
+  const d3 d = s2 - s1;
+  const double l2 = sqnorm(d);
+  const double t = -dot(s1, d);
+  if (t <= 0 || l2 <= tol * tol) {
+    out = s1;
+    x = y;
+    rank = 1;
+  } else if (t >= l2) {
+    out = s2;
+    rank = 1;
+  } else {
+    const double w2 = t / l2;
+    out = w2 * s2 + (1.0 - w2) * s1;
+    rank = 2;
+  }
+}
+
+// gjk_distance.hpp:156-289 on three slots (s3 = x, s2 = y, s1(new) = z).
+// Result simplex written to (x, y, z, rank).
+MFCL_D void dist3(MV& x, MV& y, MV& z, int& rank, d3& out, double tol) {
+  const d3 s1 = z.v, s2 = y.v, s3 = x.v;
+  const d3 s12 = s2 - s1, s13 = s3 - s1;
+  const bool sep2 = dot(s1, s12) >= 0, sep3 = dot(s1, s13) >= 0;
+  if (sep2 && sep3) {
+    out = s1;
+    x = z;
+    rank = 1;
+    return;
+  }
+  const d3 n = cross(s12, s13);
+  const double area2 = sqnorm(n);
+  const bool zero = area2 <= 0.0;
+  const bool s12sep = dot(s1, cross(n, s12)) > 0;
+  if (!sep2 && s12sep) {
+    MV p = y, q = z;
+    int r;
+    dist2(p, q, r, out, tol);
+    x = p; y = q; rank = r;
+    return;
+  }
+  const bool s13sep = dot(s1, cross(n, s13)) < 0;
+  if (!sep3 && s13sep) {
+    MV p = x, q = z;
+    int r;
+    dist2(p, q, r, out, tol);
+    x = p; y = q; rank = r;
+    return;
+  }
+  double best = -1;
+  d3 bpt = {0, 0, 0};
+  MV bx = x, by = y;
+  int br = 0;
+  if (zero || s12sep) {
+    MV p = y, q = z;
+    int r;
+    d3 pt;
+    dist2(p, q, r, pt, tol);
+    const double dd = sqnorm(pt);
+    if (best < 0 || dd < best) { best = dd; bpt = pt; bx = p; by = q; br = r; }
+  }
+  if (zero || s13sep) {
+    MV p = x, q = z;
+    int r;
+    d3 pt;
+    dist2(p, q, r, pt, tol);
+    const double dd = sqnorm(pt);
+    if (best < 0 || dd < best) { best = dd; bpt = pt; bx = p; by = q; br = r; }
+  }
+  const bool s23sep = dot(s2, cross(n, s3 - s2)) > 0;
+  if (zero || s23sep) {
+    MV p = x, q = y;
+    int r;
+    d3 pt;
+    dist2(p, q, r, pt, tol);
+    const double dd = sqnorm(pt);
+    if (best < 0 || dd < best) { best = dd; bpt = pt; bx = p; by = q; br = r; }
+  }
+  if (best < 0) {
+    out = n * (dot(s1, n) / area2);
+    rank = 3;
+  } else {
+    x = bx;
+    y = by;
+    rank = br;
+    out = bpt;
+  }
+}
+
+enum DistUpd : int { kNoImprovement, kUpdOK, kUpdFailed };
+
+MFCL_D int distUpdate(Smp& s, d3& out, double tol) {
+  if (s.rank == 1) {
+    out = s.a.v;
+    return kUpdOK;
+  }
+  if (s.rank == 2) {
+    dist2(s.a, s.b, s.rank, out, tol);
+    return kUpdOK;
+  }
+  if (s.rank == 3) {
+    dist3(s.a, s.b, s.c, s.rank, out, tol);
+    return kUpdOK;
+  }
+  if (s.rank == 4) {
+    // gjk_distance.hpp:291-373: faces (3,2,1), (3,2,0), (3,1,0)
+    double best = -1;
+    d3 bpt = {0, 0, 0};
+    MV bx = s.a, by = s.b, bz = s.c;
+    int br = 0;
+    {
+      MV x = s.b, y = s.c, z = s.e;
+      int r;
+      d3 pt;
+      dist3(x, y, z, r, pt, tol);
+      const double dd = sqnorm(pt);
+      if (best < 0 || dd < best) { best = dd; bpt = pt; bx = x; by = y; bz = z; br = r; }
+    }
+    {
+      MV x = s.a, y = s.c, z = s.e;
+      int r;
+      d3 pt;
+      dist3(x, y, z, r, pt, tol);
+      const double dd = sqnorm(pt);
+      if (best < 0 || dd < best) { best = dd; bpt = pt; bx = x; by = y; bz = z; br = r; }
+    }
+    {
+      MV x = s.a, y = s.b, z = s.e;
+      int r;
+      d3 pt;
+      dist3(x, y, z, r, pt, tol);
+      const double dd = sqnorm(pt);
+      if (best < 0 || dd < best) { best = dd; bpt = pt; bx = x; by = y; bz = z; br = r; }
+    }
+    if (best < 0) return kNoImprovement;
+    s.a = bx;
+    s.b = by;
+    s.c = bz;
+    s.rank = br;
+    out = bpt;
+    return kUpdOK;
+  }
+  return kUpdFailed;
+}
+
+constexpr double kBaryTol = 1e-3;  // gjk.h:125
+
+// gjk_distance.hpp:375-486 (witness points re-evaluated from the stored
+// directions, as the reference does).
+MFCL_D bool extractNoSub(Mink& md, const Smp& s, d3& p0, d3& p1) {
+  if (s.rank == 4 || s.rank <= 0) return false;
+  if (s.rank == 1) {
+    p0 = md.support0(s.a.d);
+    p1 = md.support1(-s.a.d);
+    return true;
+  }
+  if (s.rank == 2) {
+    const MV& v1 = s.a;
+    const MV& v2 = s.b;
+    const d3 d = v2.v - v1.v;
+    const double l2 = sqnorm(d);
+    if (l2 <= 0.0) {
+      p0 = md.support0(v1.d);
+      p1 = md.support1(-v1.d);
+      return true;
+    }
+    const double w2 = -dot(v1.v, d) / l2;
+    if (w2 > 1 + kBaryTol) {
+      p0 = md.support0(v2.d);
+      p1 = md.support1(-v2.d);
+      return false;
+    }
+    if (w2 < -kBaryTol) {
+      p0 = md.support0(v1.d);
+      p1 = md.support1(-v1.d);
+      return false;
+    }
+    const double w1 = 1.0 - w2;
+    p0 = md.support0(v1.d) * w1 + md.support0(v2.d) * w2;
+    p1 = md.support1(-v1.d) * w1 + md.support1(-v2.d) * w2;
+    return true;
+  }
+  const MV& v1 = s.a;
+  const MV& v2 = s.b;
+  const MV& v3 = s.c;
+  const d3 s12 = v2.v - v1.v, s13 = v3.v - v1.v;
+  const d3 n = cross(s12, s13);
+  const double area2 = sqnorm(n);
+  if (area2 <= 0.0) return false;
+  const d3 proj = n * (dot(v1.v, n) / area2);
+  const double area = sqrt(area2);
+  const double w2 = norm(cross(s13, v1.v - proj)) / area;
+  const double w3 = norm(cross(s12, v1.v - proj)) / area;
+  const double w1 = 1.0 - w2 - w3;
+  if (w1 < -kBaryTol || w2 < -kBaryTol || w3 < -kBaryTol) return false;
+  p0 = md.support0(v1.d) * w1 + md.support0(v2.d) * w2 + md.support0(v3.d) * w3;
+  p1 = md.support1(-v1.d) * w1 + md.support1(-v2.d) * w2 + md.support1(-v3.d) * w3;
+  return true;
+}
+
+// gjk_distance.hpp:488-559
+MFCL_D bool extractTrySub(Mink& md, const Smp& s, d3& p0, d3& p1) {
+  if (s.rank >= 4) return false;
+  if (s.rank <= 2) {
+    extractNoSub(md, s, p0, p1);
+    return true;
+  }
+  double best = -1;
+  d3 bp0 = {0, 0, 0}, bp1 = {0, 0, 0}, q0, q1;
+  if (extractNoSub(md, s, q0, q1)) {
+    const double dd = sqnorm(q0 - q1);
+    if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; }
+  }
+  Smp c = s;
+  c.rank = 2;
+  extractNoSub(md, c, q0, q1);
+  { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
+  c = s; c.b = s.c; c.rank = 2;
+  extractNoSub(md, c, q0, q1);
+  { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
+  c = s; c.a = s.b; c.b = s.c; c.rank = 2;
+  extractNoSub(md, c, q0, q1);
+  { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
+  p0 = bp0;
+  p1 = bp1;
+  return true;
+}
+
+// gjk_distance.hpp:11-105
+MFCL_D bool gjkMinDistance(Mink& md, Smp& s, double tol, int max_iter, d3& p0, d3& p1, int* iters) {
+  if (s.rank != 1) return false;
+  d3 cur;
+  if (distUpdate(s, cur, tol) != kUpdOK) return false;
+  double book = norm(cur);
+  if (book <= tol) return extractNoSub(md, s, p0, p1);
+  d3 nd = -cur / book;
+  const double tol2 = tol * tol;
+  for (int it = 0; it < max_iter; ++it) {
+    (*iters)++;
+    MV nv;
+    nv.d = nd;
+    nv.v = md.support(nd);
+    if (dot(nd, nv.v - cur) < tol) return extractNoSub(md, s, p0, p1);
+    if (smpDup(s, nv.v, tol2)) return extractNoSub(md, s, p0, p1);
+    smpAdd(s, nv);
+    const int st = distUpdate(s, cur, tol);
+    if (st == kNoImprovement) return extractNoSub(md, s, p0, p1);
+    if (st != kUpdOK) return false;
+    const double nm = norm(cur);
+    if (book - nm < tol) return extractNoSub(md, s, p0, p1);
+    if (nm < tol) return extractNoSub(md, s, p0, p1);
+    book = nm;
+    nd = -cur / book;
+  }
+  return false;
+}
+
+struct GjkOut {
+  int status;
+  bool sep_valid;
+  d3 p0, p1;
+  int iters;       // boolean iterations
+  int dist_iters;  // distance iterations
+};
+
+// gjk.hpp:11-140.  want_distance selects the MinSeparationDistanceOutput path.
+MFCL_D void gjkEvaluate(Mink& md, Smp& s, d3 guess, double tol, int max_iter, bool want_distance,
+                        GjkOut& o) {
+  o.sep_valid = false;
+  o.iters = 0;
+  o.dist_iters = 0;
+  o.p0 = d3{0, 0, 0};
+  o.p1 = d3{0, 0, 0};
+  d3 dir = guess;
+  if (sqnorm(dir) <= 0.0) dir = d3{1, 0, 0};
+  dir = normalized(dir);
+  MV v;
+  v.d = dir;
+  v.v = md.support(dir);
+  s.rank = -1;
+  smpAdd(s, v);
+  const double tol2 = tol * tol;
+  if (sqnorm(v.v) <= tol2) {
+    o.status = kGjkIntersect;
+    return;
+  }
+  bool separated = dot(v.v, dir) < 0;
+  int status = -1;
+  if (!separated) {
+    dir = -dir;
+    status = kGjkIterLimit;
+    for (int it = 0; it < max_iter; ++it) {
+      o.iters = it + 1;
+      v.d = dir;
+      v.v = md.support(dir);
+      if (dot(v.v, dir) < 0) {
+        separated = true;
+        break;
+      }
+      if (smpDup(s, v.v, tol2)) {
+        status = kGjkNoProgress;
+        break;
+      }
+      if (sqnorm(v.v) <= tol2) {
+        smpAdd(s, v);
+        o.status = kGjkIntersect;
+        return;
+      }
+      smpAdd(s, v);
+      const int p = gjkProject(s, dir, tol);
+      if (p == kProjFailed) {
+        o.status = kGjkFailed;
+        return;
+      }
+      if (p == kProjIntersect) {
+        o.status = kGjkIntersect;
+        return;
+      }
+      if (p == kProjZeroVolume) {
+        status = kGjkNoProgress;
+        break;
+      }
+    }
+  }
+  if (separated) {
+    o.status = kGjkSeparated;
+    if (want_distance) {
+      s.rank = -1;
+      smpAdd(s, v);
+      o.sep_valid = gjkMinDistance(md, s, tol, max_iter, o.p0, o.p1, &o.dist_iters);
+    }
+    return;
+  }
+  o.status = status;
+  if (want_distance) o.sep_valid = extractTrySub(md, s, o.p0, o.p1);
+}
+
+// ------------------------------------------------------------------- MPR
+// include/fcl/cvx_collide/mpr.hpp:34-185, findPortal :188-328, updatePortal :436-495.
+enum MprStatus : int { kMprIntersect = 0, kMprSeparated = 1, kMprFailed = 2 };
+
+MFCL_D d3 mprSupport(Mink& md, d3& d) {
+  d = normalized(d);
+  return md.support(d);
+}
+
+MFCL_D void swapd3(d3& a, d3& b) {
+  const d3 t = a;
+  a = b;
+  b = t;
+}
+
+MFCL_D int mprIntersect(Mink& md, int max_iter, double tol) {
+  constexpr double kDotEps = 2.220446049250313e-16;
+  const d3 v0 = md.interior();
+  if (sqnorm(v0) <= tol * tol) return kMprIntersect;
+  d3 d1 = -v0;
+  d3 v1 = mprSupport(md, d1);
+  if (dot(d1, v1) < 0) return kMprSeparated;
+  d3 d2 = cross(v0, v1);
+  if (absnorm(d2) <= absnorm(v0) * absnorm(v1) * tol) return kMprIntersect;
+  d3 v2 = mprSupport(md, d2);
+  if (dot(d2, v2) < 0) return kMprSeparated;
+  d3 d3v = cross(v1 - v0, v2 - v0);
+  if (dot(d3v, v0) > 0) {
+    swapd3(v1, v2);
+    swapd3(d1, d2);
+    d3v = -d3v;
+  }
+  d3 v3 = mprSupport(md, d3v);
+  if (dot(d3v, v3) < 0) return kMprSeparated;
+  const double v0n = absnorm(v0);
+  int it = 0;
+  while (true) {
+    if (it >= max_iter) return kMprFailed;
+    ++it;
+    d3 a1 = v1 - v0, a2 = v2 - v0, a3 = v3 - v0;
+    d3 n031 = cross(a3, a1), n012 = cross(a1, a2);
+    if (dot(a2, n031) < 0) {
+      swapd3(v2, v3);
+      swapd3(d2, d3v);
+      swapd3(a2, a3);
+      swapd3(n012, n031);
+      n031 = -n031;
+      n012 = -n012;
+    }
+    if (dot(v0, n031) > kDotEps * v0n * absnorm(n031)) {
+      d3 sd = -n031;
+      v2 = mprSupport(md, sd);
+      d2 = sd;
+      if (dot(v2, sd) < 0) return kMprSeparated;
+      continue;
+    }
+    if (dot(v0, n012) > kDotEps * v0n * absnorm(n012)) {
+      d3 sd = -n012;
+      v3 = mprSupport(md, sd);
+      d3v = sd;
+      if (dot(v3, sd) < 0) return kMprSeparated;
+      continue;
+    }
+    const d3 n023 = cross(a2, a3);
+    if (dot(v0, n023) > kDotEps * v0n * absnorm(n023)) {
+      d3 sd = -n023;
+      v1 = mprSupport(md, sd);
+      d1 = sd;
+      if (dot(v1, sd) < 0) return kMprSeparated;
+      continue;
+    }
+    break;
+  }
+  for (it = 0; it < max_iter; ++it) {
+    d3 n = cross(v2 - v1, v3 - v1);
+    if (dot(n, v0) > 0) {
+      swapd3(v2, v3);
+      swapd3(d2, d3v);
+      n = -n;
+    }
+    if (!(dot(v1, n) < 0)) return kMprIntersect;
+    const d3 v4 = mprSupport(md, n);
+    if (dot(v4, n) < 0) return kMprSeparated;
+    if (fabs(dot(v4 - v1, n)) < tol * absnorm(n)) return kMprSeparated;
+    const d3 nn = cross(v4, v0);
+    if (dot(v1, nn) > 0) {
+      if (dot(v2, nn) > 0) { v1 = v4; d1 = n; }
+      else { v3 = v4; d3v = n; }
+    } else {
+      if (dot(v3, nn) > 0) { v2 = v4; d2 = n; }
+      else { v1 = v4; d1 = n; }
+    }
+  }
+  return kMprFailed;
+}
+
+}  // namespace mfcl

@@ -1,0 +1,417 @@
+// Batched convex narrowphase for gfx950.
+//
+// Pipeline per query (one stream, no host round trip):
+//   1. costKey / bucketScan / bucketScatter: counting sort of the pairs by
+//      (closed-form vs support-mapping, log2 of hull vertex count) so the
+//      lanes of a wavefront run loops of similar trip count;
+//   2. narrowKernel: one lane per pair -- closed forms, MPR, GJK boolean or
+//      GJK distance; pairs that need EPA append their simplex to a queue;
+//   3. epaKernel: persistent grid, one lane per queued pair, polytope in a
+//      per-lane device-memory arena.
+#include <hip/hip_runtime.h>
+
+#include "ctx.h"
+#include "epa_dev.h"
+#include "prim_dev.h"
+
+namespace mfcl {
+
+constexpr int kBuckets = 128;
+constexpr int kBlock = 256;
+
+struct PairIO {
+  const uint32_t* pa;
+  const uint32_t* pb;
+  const float* posea;
+  const float* poseb;
+  const float* guess;
+  mfcl_convex_result* out;
+};
+
+struct QEntry {
+  Smp s;
+  uint32_t pair;
+  uint32_t pad;
+};
+
+struct Params {
+  int mode, max_contacts, penetration, gjk_max_it, epa_max_faces, epa_max_it, use_guess;
+  uint32_t n_shapes;
+  double gjk_tol, epa_tol;
+};
+
+MFCL_D int shapeCost(const ShapeTables& T, uint32_t i) {
+  return T.type[i] == kConvex ? static_cast<int>(T.vert_cnt[i]) : 1;
+}
+
+MFCL_D bool closedForm(int ta, int tb) {
+  if (ta == kSphere && (tb == kSphere || tb == kCapsule || tb == kBox || tb == kCylinder)) return true;
+  if (tb == kSphere && (ta == kCapsule || ta == kBox || ta == kCylinder)) return true;
+  return ta == kBox && tb == kBox;
+}
+
+__global__ void costKeyKernel(ShapeTables T, PairIO io, uint64_t n, Params P, uint8_t* key, uint32_t* hist) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t a = io.pa[i], b = io.pb[i];
+  if (a >= P.n_shapes || b >= P.n_shapes) {
+    key[i] = 0;
+    atomicAdd(&hist[0], 1u);
+    return;
+  }
+  const int ta = T.type[a], tb = T.type[b];
+  const bool cf = P.mode == MFCL_MODE_COLLIDE && closedForm(ta, tb);
+  const int cost = shapeCost(T, a) + shapeCost(T, b);
+  int lg = 0;
+  // 4 buckets per octave of the summed vertex count
+  const float l2 = log2f(static_cast<float>(cost));
+  lg = min(63, static_cast<int>(4.0f * l2));
+  const uint8_t k = static_cast<uint8_t>((cf ? 0 : 64) + lg);
+  key[i] = k;
+  atomicAdd(&hist[k], 1u);
+}
+
+__global__ void bucketScanKernel(uint32_t* hist, uint32_t* offs) {
+  // single block of kBuckets threads: exclusive scan
+  __shared__ uint32_t s[kBuckets];
+  const int t = threadIdx.x;
+  s[t] = hist[t];
+  __syncthreads();
+  if (t == 0) {
+    uint32_t acc = 0;
+    for (int k = 0; k < kBuckets; ++k) {
+      const uint32_t v = s[k];
+      s[k] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  offs[t] = s[t];
+}
+
+__global__ void bucketScatterKernel(const uint8_t* key, uint64_t n, uint32_t* offs, uint32_t* order) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t pos = atomicAdd(&offs[key[i]], 1u);
+  order[pos] = static_cast<uint32_t>(i);
+}
+
+MFCL_D void clearResult(mfcl_convex_result& r) {
+  r.hit = 0;
+  r.n_contacts = 0;
+  r.method = 0;
+  r.gjk_status = -1;
+  r.epa_status = -1;
+  r.ok = 0;
+  r.gjk_iters = 0;
+  r.epa_iters = 0;
+  r.distance = 0;
+  r.depth = 0;
+  r.normal[0] = r.normal[1] = r.normal[2] = 0;
+  r.pos[0] = r.pos[1] = r.pos[2] = 0;
+  r.pa[0] = r.pa[1] = r.pa[2] = 0;
+  r.pb[0] = r.pb[1] = r.pb[2] = 0;
+}
+
+MFCL_D void put3(double* dst, d3 v) {
+  dst[0] = v.x;
+  dst[1] = v.y;
+  dst[2] = v.z;
+}
+
+MFCL_D uint16_t sat16(int v) { return static_cast<uint16_t>(v > 65535 ? 65535 : v); }
+
+// Closed-form dispatch with the reference's operand swaps and normal flips
+// (gjk_solver-inl.h:158-217).  Returns hit; fills contacts if requested.
+MFCL_D bool runClosedForm(const DevShape& A, const tf3& ta, const DevShape& B, const tf3& tb, PContacts* pc) {
+  const int x = A.type, y = B.type;
+  bool hit = false, flip = false;
+  if (x == kSphere && y == kSphere) hit = primSphereSphere(A.p0, ta, B.p0, tb, pc);
+  else if (x == kBox && y == kBox) hit = primBoxBox(d3{A.p0, A.p1, A.p2}, ta, d3{B.p0, B.p1, B.p2}, tb, pc) != 0;
+  else if (x == kSphere && y == kCapsule) hit = primSphereCapsule(A.p0, ta, B.p0, B.p1, tb, pc);
+  else if (x == kCapsule && y == kSphere) { hit = primSphereCapsule(B.p0, tb, A.p0, A.p1, ta, pc); flip = true; }
+  else if (x == kSphere && y == kBox) hit = primSphereBox(A.p0, ta, d3{B.p0, B.p1, B.p2}, tb, pc);
+  else if (x == kBox && y == kSphere) { hit = primSphereBox(B.p0, tb, d3{A.p0, A.p1, A.p2}, ta, pc); flip = true; }
+  else if (x == kSphere && y == kCylinder) hit = primSphereCylinder(A.p0, ta, B.p0, B.p1, tb, pc);
+  else if (x == kCylinder && y == kSphere) { hit = primSphereCylinder(B.p0, tb, A.p0, A.p1, ta, pc); flip = true; }
+  if (flip && pc)
+    for (int k = 0; k < pc->n; ++k) pc->c[k].n = -pc->c[k].n;
+  return hit;
+}
+
+MFCL_D void loadPair(const ShapeTables& T, const double4* interior, const PairIO& io, uint32_t i,
+                     DevShape& A, DevShape& B, tf3& ta, tf3& tb, uint32_t& ia, uint32_t& ib) {
+  ia = io.pa[i];
+  ib = io.pb[i];
+  A = loadShape(T, ia);
+  B = loadShape(T, ib);
+  ta = poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i));
+  tb = poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i));
+}
+
+__global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const double4* interior, PairIO io,
+                                                       const uint32_t* order, uint64_t n, Params P,
+                                                       QEntry* queue, uint32_t* qcount,
+                                                       unsigned long long* scanned_total) {
+  const uint64_t slot = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int scanned = 0;
+  if (slot < n) {
+    const uint32_t i = order[slot];
+    mfcl_convex_result r;
+    clearResult(r);
+    bool done = true;
+    const bool bad_ids = io.pa[i] >= P.n_shapes || io.pb[i] >= P.n_shapes;
+    if (bad_ids) {
+      r.ok = -1;  // shape id out of range
+      io.out[i] = r;
+    }
+    if (!bad_ids) {
+    DevShape A, B;
+    tf3 ta, tb;
+    uint32_t ia, ib;
+    loadPair(T, interior, io, i, A, B, ta, tb, ia, ib);
+    if (P.mode == MFCL_MODE_COLLIDE) {
+      if (P.max_contacts <= 0) {
+        // the reference warns and adds nothing (collision-inl.h:81-84)
+      } else if (closedForm(A.type, B.type)) {
+        PContacts pc;
+        pc.n = 0;
+        const bool hit = runClosedForm(A, ta, B, tb, P.penetration ? &pc : nullptr);
+        r.hit = hit;
+        r.ok = 1;
+        if (hit) {
+          if (!P.penetration) {
+            r.n_contacts = 1;
+          } else {
+            int k0 = 0;
+            if (pc.n > P.max_contacts) {
+              for (int k = 1; k < pc.n; ++k)
+                if (pc.c[k].depth > pc.c[k0].depth) k0 = k;
+            }
+            r.n_contacts = pc.n < P.max_contacts ? pc.n : P.max_contacts;
+            if (pc.n > 0) {
+              r.depth = pc.c[k0].depth;
+              r.distance = -r.depth;
+              put3(r.normal, pc.c[k0].n);
+              put3(r.pos, pc.c[k0].p);
+            }
+          }
+        }
+      } else {
+        Mink md;
+        setupMink(md, A, ta, B, tb);
+        if (A.type == kConvex) { const double4 c = interior[ia]; md.int0 = d3{c.x, c.y, c.z}; }
+        if (B.type == kConvex) { const double4 c = interior[ib]; md.int1 = d3{c.x, c.y, c.z}; }
+        r.method = 1;
+        bool decided = false;
+        if (!P.penetration) {
+          r.method = 2;
+          const int m = mprIntersect(md, P.gjk_max_it, P.gjk_tol);
+          if (m != kMprFailed) {
+            r.hit = m == kMprIntersect;
+            r.n_contacts = r.hit;
+            r.ok = 1;
+            decided = true;
+          }
+        }
+        if (!decided) {
+          d3 guess = {1, 0, 0};
+          if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
+          Smp s;
+          s.rank = -1;
+          GjkOut g;
+          gjkEvaluate(md, s, -guess, P.gjk_tol, P.gjk_max_it, false, g);
+          r.gjk_status = static_cast<int8_t>(g.status);
+          r.gjk_iters = sat16(g.iters);
+          r.ok = 1;
+          if (!P.penetration) {
+            r.hit = g.status == kGjkIntersect;
+            r.n_contacts = r.hit;
+          } else if (g.status == kGjkIntersect) {
+            const uint32_t q = atomicAdd(qcount, 1u);
+            queue[q].s = s;
+            queue[q].pair = i;
+            done = false;
+          }
+        }
+        scanned = md.scanned;
+      }
+    } else {
+      // signed distance: no closed forms (gjk_solver-inl.h:820-874)
+      Mink md;
+      setupMink(md, A, ta, B, tb);
+      r.method = 1;
+      d3 guess = {1, 0, 0};
+      if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
+      Smp s;
+      s.rank = -1;
+      GjkOut g;
+      gjkEvaluate(md, s, -guess, P.gjk_tol, P.gjk_max_it, true, g);
+      r.gjk_status = static_cast<int8_t>(g.status);
+      r.gjk_iters = sat16(g.iters + g.dist_iters);
+      if (g.status == kGjkSeparated && g.sep_valid) {
+        const d3 pa = apply(ta, g.p0), pb = apply(ta, g.p1);
+        put3(r.pa, pa);
+        put3(r.pb, pb);
+        r.distance = norm(g.p0 - g.p1);
+        r.ok = 1;
+      } else if (g.status == kGjkIntersect) {
+        const uint32_t q = atomicAdd(qcount, 1u);
+        queue[q].s = s;
+        queue[q].pair = i;
+        done = false;
+      } else {
+        r.distance = -1;
+      }
+      scanned = md.scanned;
+    }
+    // pairs queued for EPA keep their GJK status here; the EPA stage
+    // completes the record
+    io.out[i] = r;
+    (void)done;
+    }
+  }
+  // wave-level reduction of the scan counter, one atomic per wave
+  unsigned long long v = static_cast<unsigned long long>(scanned);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(scanned_total, v);
+}
+
+__global__ void __launch_bounds__(kBlock) epaKernel(ShapeTables T, const double4* interior, PairIO io, Params P,
+                                                    const QEntry* queue, const uint32_t* qcount,
+                                                    char* arenas, size_t arena_bytes,
+                                                    unsigned long long* scanned_total) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * blockDim.x;
+  const uint32_t count = *qcount;
+  const ArenaDims dims = arenaDims(P.epa_max_faces);
+  char* base = arenas + static_cast<size_t>(tid) * arena_bytes;
+  Poly poly;
+  poly.V = reinterpret_cast<EVert*>(base);
+  poly.E = reinterpret_cast<EEdge*>(base + sizeof(EVert) * dims.nv);
+  poly.F = reinterpret_cast<EFace*>(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv);
+  poly.stk = reinterpret_cast<int32_t*>(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv +
+                                        sizeof(EFace) * dims.nf);
+  poly.nv = dims.nv;
+  poly.nf = dims.nf;
+  int scanned = 0;
+  for (uint32_t q = tid; q < count; q += nthreads) {
+    const QEntry e = queue[q];
+    const uint32_t i = e.pair;
+    DevShape A, B;
+    tf3 ta, tb;
+    uint32_t ia, ib;
+    loadPair(T, interior, io, i, A, B, ta, tb, ia, ib);
+    Mink md;
+    setupMink(md, A, ta, B, tb);
+    EpaOut eo;
+    epaEvaluate(poly, md, e.s, P.epa_tol, P.epa_max_it, eo);
+    scanned += md.scanned;
+    mfcl_convex_result& r = io.out[i];
+    r.epa_status = static_cast<int8_t>(eo.status);
+    r.epa_iters = sat16(eo.iters);
+    if (eo.status == kEpaFailed) {
+      r.hit = 0;
+      r.ok = 0;
+      continue;
+    }
+    r.hit = 1;
+    r.ok = 1;
+    if (P.mode == MFCL_MODE_COLLIDE) {
+      // gjk_solver-inl.h:117-129
+      d3 n1 = eo.p0 - eo.p1;
+      n1 = sqnorm(n1) <= 0 ? d3{0, 0, 1} : normalized(n1);
+      const d3 pt = 0.5 * (eo.p0 + eo.p1);
+      r.n_contacts = 1;
+      r.depth = eo.depth;
+      r.distance = -eo.depth;
+      put3(r.normal, mul(ta.R, n1));
+      put3(r.pos, apply(ta, pt));
+    } else {
+      put3(r.pa, apply(ta, eo.p0));
+      put3(r.pb, apply(ta, eo.p1));
+      r.distance = -eo.depth;
+      r.depth = eo.depth;
+    }
+  }
+  unsigned long long v = static_cast<unsigned long long>(scanned);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(scanned_total, v);
+}
+
+__global__ void zeroKernel(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+static int g_epa_threads = 32768;
+
+int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                      const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
+                      mfcl_convex_result* out, hipStream_t s) {
+  if (n == 0) return MFCL_OK;
+  if (ctx->n_shapes == 0) {
+    ctx->err = "no shapes uploaded";
+    return MFCL_E_STATE;
+  }
+  if (n > 0xFFFFFFFFull) {
+    ctx->err = "too many pairs in one query";
+    return MFCL_E_INVALID;
+  }
+  if (req->epa_max_faces < 4 || req->epa_max_faces > 8192) {
+    ctx->err = "epa_max_faces out of range [4, 8192]";
+    return MFCL_E_INVALID;
+  }
+  if (req->use_guess && guess == nullptr) {
+    ctx->err = "use_guess set but guess3 is NULL";
+    return MFCL_E_INVALID;
+  }
+  Params P;
+  P.mode = req->mode;
+  P.max_contacts = req->max_contacts;
+  P.penetration = req->penetration;
+  P.gjk_max_it = req->gjk_max_iterations;
+  P.gjk_tol = req->gjk_tolerance;
+  P.epa_max_faces = req->epa_max_faces;
+  P.epa_max_it = req->epa_max_iterations;
+  P.epa_tol = req->epa_tolerance;
+  P.use_guess = req->use_guess;
+  P.n_shapes = ctx->n_shapes;
+
+  // workspace: hist + offs (2 x 128 x 4 B), qcount at byte 1024, work
+  // counters at byte 1536; keys (n B) after the order array (4n B); EPA queue; EPA arenas
+  const size_t misc_bytes = 2048;
+  if (int rc = ensure(ctx, ctx->ws_misc, misc_bytes)) return rc;
+  if (int rc = ensure(ctx, ctx->ws_order, n * (sizeof(uint32_t) + 1) + 256)) return rc;
+  if (int rc = ensure(ctx, ctx->ws_queue, n * sizeof(QEntry))) return rc;
+  const size_t ab = arenaBytes(req->epa_max_faces);
+  if (int rc = ensure(ctx, ctx->ws_arena, ab * static_cast<size_t>(g_epa_threads))) return rc;
+  uint32_t* hist = static_cast<uint32_t*>(ctx->ws_misc.p);
+  uint32_t* offs = hist + kBuckets;
+  uint32_t* qcount = offs + kBuckets;
+  unsigned long long* sc = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->ws_misc.p) + 1536);
+  ctx->d_counters = reinterpret_cast<uint64_t*>(sc);
+  uint32_t* order = static_cast<uint32_t*>(ctx->ws_order.p);
+  uint8_t* key = reinterpret_cast<uint8_t*>(order + n);
+  QEntry* queue = static_cast<QEntry*>(ctx->ws_queue.p);
+
+  ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts};
+  PairIO io{pa, pb, posea, poseb, guess, out};
+  const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+  if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
+  zeroKernel<<<1, 512, 0, s>>>(hist, 512);
+  costKeyKernel<<<nb, kBlock, 0, s>>>(T, io, n, P, key, hist);
+  bucketScanKernel<<<1, kBuckets, 0, s>>>(hist, offs);
+  bucketScatterKernel<<<nb, kBlock, 0, s>>>(key, n, offs, order);
+  if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
+  narrowKernel<<<nb, kBlock, 0, s>>>(T, ctx->d_interior, io, order, n, P, queue, qcount, sc);
+  if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  const unsigned eb = static_cast<unsigned>(g_epa_threads / kBlock);
+  epaKernel<<<eb, kBlock, 0, s>>>(T, ctx->d_interior, io, P, queue, qcount,
+                                   static_cast<char*>(ctx->ws_arena.p), ab, sc + 1);
+  if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+  MFCL_HIP_TRY(ctx, hipGetLastError());
+  return MFCL_OK;
+}
+
+}  // namespace mfcl

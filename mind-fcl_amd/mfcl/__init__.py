@@ -1,0 +1,300 @@
+"""Python binding of libmfcl.so (the C ABI in include/mfcl.h) via ctypes.
+
+This is the host-side mirror used by the tests and the benchmark; the
+product path is the HIP library itself.  Loading fails loudly when the
+shared library has not been built -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libmfcl.so")
+
+MFCL_SHAPE_BOX = 0
+MFCL_SHAPE_SPHERE = 1
+MFCL_SHAPE_CAPSULE = 3
+MFCL_SHAPE_CYLINDER = 5
+MFCL_SHAPE_CONVEX = 100
+MODE_COLLIDE = 0
+MODE_SIGNED_DISTANCE = 1
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("max_contacts", C.c_int32),
+        ("penetration", C.c_int32),
+        ("gjk_max_iterations", C.c_int32),
+        ("gjk_tolerance", C.c_double),
+        ("epa_max_faces", C.c_int32),
+        ("epa_max_iterations", C.c_int32),
+        ("epa_tolerance", C.c_double),
+        ("use_guess", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+CONVEX_RESULT_DTYPE = np.dtype(
+    [
+        ("hit", np.int32),
+        ("n_contacts", np.int32),
+        ("method", np.int8),
+        ("gjk_status", np.int8),
+        ("epa_status", np.int8),
+        ("ok", np.int8),
+        ("gjk_iters", np.uint16),
+        ("epa_iters", np.uint16),
+        ("distance", np.float64),
+        ("depth", np.float64),
+        ("normal", np.float64, 3),
+        ("pos", np.float64, 3),
+        ("pa", np.float64, 3),
+        ("pb", np.float64, 3),
+    ]
+)
+assert CONVEX_RESULT_DTYPE.itemsize == 128
+
+MESH_RESULT_DTYPE = np.dtype(
+    [("hit", np.int32), ("n_contacts", np.int32), ("bv_tests", np.int32), ("leaf_tests", np.int32)]
+)
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads libmfcl.so; raises if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"mfcl: {LIB_PATH} not built; run `make -C mind-fcl_amd` (or __graft_entry__.build())"
+        )
+    lib = C.CDLL(LIB_PATH)
+    p, u64, i32, u32 = C.c_void_p, C.c_uint64, C.c_int32, C.c_uint32
+    lib.mfcl_create.argtypes = [C.POINTER(p), C.c_int]
+    lib.mfcl_destroy.argtypes = [p]
+    lib.mfcl_last_error.argtypes = [p]
+    lib.mfcl_last_error.restype = C.c_char_p
+    lib.mfcl_sync.argtypes = [p]
+    lib.mfcl_set_timing.argtypes = [p, C.c_int]
+    lib.mfcl_last_timing.argtypes = [p, p]
+    lib.mfcl_last_counters.argtypes = [p, p]
+    lib.mfcl_request_defaults.argtypes = [C.POINTER(Request), i32]
+    lib.mfcl_request_defaults.restype = None
+    lib.mfcl_dev_alloc.argtypes = [p, u64, C.POINTER(p)]
+    lib.mfcl_dev_free.argtypes = [p, p]
+    lib.mfcl_h2d.argtypes = [p, p, p, u64]
+    lib.mfcl_d2h.argtypes = [p, p, p, u64]
+    lib.mfcl_upload_shapes.argtypes = [p, p, p, p, p, u32, p, u64]
+    lib.mfcl_convex_query.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p]
+    lib.mfcl_convex_query_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
+    lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
+    lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
+    lib.mfcl_mesh_collide.argtypes = [p, p, p, p, p, u64, i32, p, p]
+    lib.mfcl_mesh_collide_dev.argtypes = [p, p, p, p, p, u64, i32, p, p, p]
+    lib.mfcl_aabbtree_build.argtypes = [p, p, p, p, u32]
+    lib.mfcl_aabbtree_query.argtypes = [p, p, p, u64, i32, p, p]
+    lib.mfcl_aabbtree_query_dev.argtypes = [p, p, p, u64, i32, p, p, p]
+    _lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def request(mode: int = MODE_COLLIDE, **kw) -> Request:
+    r = Request()
+    load_library().mfcl_request_defaults(C.byref(r), mode)
+    for k, v in kw.items():
+        setattr(r, k, v)
+    return r
+
+
+class DeviceArray:
+    """A device allocation owned by a Context (freed with it or explicitly)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        ctx._check(ctx.lib.mfcl_dev_alloc(ctx.h, max(1, self.nbytes), C.byref(self.ptr)))
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.lib.mfcl_h2d(self.ctx.h, self.ptr, _ptr(a), a.nbytes))
+
+    def download(self, a: np.ndarray):
+        assert a.flags.c_contiguous and a.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.lib.mfcl_d2h(self.ctx.h, _ptr(a), self.ptr, a.nbytes))
+        return a
+
+    def free(self):
+        if self.ptr.value:
+            self.ctx.lib.mfcl_dev_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self.h = C.c_void_p()
+        rc = self.lib.mfcl_create(C.byref(self.h), device)
+        if rc != 0:
+            raise RuntimeError(f"mfcl_create failed ({rc}); is a GPU visible?")
+
+    def close(self):
+        if self.h:
+            self.lib.mfcl_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise RuntimeError(f"mfcl error {rc}: {self.lib.mfcl_last_error(self.h).decode()}")
+
+    def sync(self):
+        self._check(self.lib.mfcl_sync(self.h))
+
+    def set_timing(self, on: bool):
+        self._check(self.lib.mfcl_set_timing(self.h, 1 if on else 0))
+
+    def last_timing(self) -> np.ndarray:
+        ms = np.zeros(6, np.float32)
+        self._check(self.lib.mfcl_last_timing(self.h, _ptr(ms)))
+        return ms
+
+    def last_counters(self) -> np.ndarray:
+        c = np.zeros(4, np.uint64)
+        self._check(self.lib.mfcl_last_counters(self.h, _ptr(c)))
+        return c
+
+    def alloc(self, nbytes: int) -> DeviceArray:
+        return DeviceArray(self, nbytes)
+
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.nbytes)
+        d.upload(a)
+        return d
+
+    # ---- shapes
+    def upload_shapes(self, types, params4, vert_off=None, vert_cnt=None, verts=None):
+        types = np.ascontiguousarray(types, np.int32)
+        params4 = np.ascontiguousarray(params4, np.float32).reshape(-1, 4)
+        n = len(types)
+        if verts is None:
+            verts = np.zeros((1, 3), np.float32)
+            vert_off = np.zeros(n, np.uint64)
+            vert_cnt = np.zeros(n, np.uint32)
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        vert_off = np.ascontiguousarray(vert_off, np.uint64)
+        vert_cnt = np.ascontiguousarray(vert_cnt, np.uint32)
+        self._check(
+            self.lib.mfcl_upload_shapes(
+                self.h, _ptr(types), _ptr(params4), _ptr(vert_off), _ptr(vert_cnt), n, _ptr(verts), len(verts)
+            )
+        )
+
+    def convex_query(self, pair_a, pair_b, pose_a, pose_b, req: Request, guess=None) -> np.ndarray:
+        pair_a = np.ascontiguousarray(pair_a, np.uint32)
+        pair_b = np.ascontiguousarray(pair_b, np.uint32)
+        pose_a = np.ascontiguousarray(pose_a, np.float32).reshape(-1, 7)
+        pose_b = np.ascontiguousarray(pose_b, np.float32).reshape(-1, 7)
+        g = None if guess is None else np.ascontiguousarray(guess, np.float32).reshape(-1, 3)
+        n = len(pair_a)
+        out = np.zeros(n, CONVEX_RESULT_DTYPE)
+        self._check(
+            self.lib.mfcl_convex_query(
+                self.h, _ptr(pair_a), _ptr(pair_b), _ptr(pose_a), _ptr(pose_b), _ptr(g), n, C.byref(req), _ptr(out)
+            )
+        )
+        return out
+
+    def convex_query_dev(self, pa: DeviceArray, pb: DeviceArray, qa: DeviceArray, qb: DeviceArray, n: int,
+                         req: Request, out: DeviceArray, guess: Optional[DeviceArray] = None):
+        self._check(
+            self.lib.mfcl_convex_query_dev(
+                self.h, pa.ptr, pb.ptr, qa.ptr, qb.ptr, None if guess is None else guess.ptr, n, C.byref(req),
+                out.ptr, None
+            )
+        )
+
+    # ---- meshes
+    def upload_meshes(self, verts, vert_off, vert_cnt, tris, tri_off, tri_cnt):
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        vert_off = np.ascontiguousarray(vert_off, np.uint64)
+        vert_cnt = np.ascontiguousarray(vert_cnt, np.uint32)
+        tri_off = np.ascontiguousarray(tri_off, np.uint64)
+        tri_cnt = np.ascontiguousarray(tri_cnt, np.uint32)
+        self._check(
+            self.lib.mfcl_upload_meshes(
+                self.h, _ptr(verts), _ptr(vert_off), _ptr(vert_cnt), _ptr(tris), _ptr(tri_off), _ptr(tri_cnt),
+                len(vert_cnt)
+            )
+        )
+
+    def mesh_collide(self, pair_a, pair_b, pose_a, pose_b, max_contacts: int = 16):
+        pair_a = np.ascontiguousarray(pair_a, np.uint32)
+        pair_b = np.ascontiguousarray(pair_b, np.uint32)
+        pose_a = np.ascontiguousarray(pose_a, np.float32).reshape(-1, 7)
+        pose_b = np.ascontiguousarray(pose_b, np.float32).reshape(-1, 7)
+        n = len(pair_a)
+        out = np.zeros(n, MESH_RESULT_DTYPE)
+        contacts = np.full((n, max_contacts, 2), -1, np.int32)
+        self._check(
+            self.lib.mfcl_mesh_collide(
+                self.h, _ptr(pair_a), _ptr(pair_b), _ptr(pose_a), _ptr(pose_b), n, max_contacts, _ptr(out),
+                _ptr(contacts)
+            )
+        )
+        return out, contacts
+
+    def mesh_collide_dev(self, pa, pb, qa, qb, n, max_contacts, out, contacts):
+        self._check(
+            self.lib.mfcl_mesh_collide_dev(self.h, pa.ptr, pb.ptr, qa.ptr, qb.ptr, n, max_contacts, out.ptr,
+                                           contacts.ptr, None)
+        )
+
+    # ---- broadphase
+    def aabbtree_build(self, mn, mx, ids):
+        mn = np.ascontiguousarray(mn, np.float32).reshape(-1, 3)
+        mx = np.ascontiguousarray(mx, np.float32).reshape(-1, 3)
+        ids = np.ascontiguousarray(ids, np.int64)
+        self._check(self.lib.mfcl_aabbtree_build(self.h, _ptr(mn), _ptr(mx), _ptr(ids), len(ids)))
+
+    def aabbtree_query(self, qmn, qmx, cap: int):
+        qmn = np.ascontiguousarray(qmn, np.float32).reshape(-1, 3)
+        qmx = np.ascontiguousarray(qmx, np.float32).reshape(-1, 3)
+        n = len(qmn)
+        counts = np.zeros(n, np.int32)
+        ids = np.full((n, max(cap, 1)), -1, np.int64)
+        self._check(self.lib.mfcl_aabbtree_query(self.h, _ptr(qmn), _ptr(qmx), n, cap, _ptr(counts), _ptr(ids)))
+        return counts, ids
+
+
+# Symbols the C header declares (checked by the CPU test suite).
+def header_symbols(header_path: Optional[str] = None):
+    import re
+
+    if header_path is None:
+        header_path = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "mfcl.h")
+    text = open(header_path).read()
+    return sorted(set(re.findall(r"\b(mfcl_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,225 @@
+"""Seeded synthetic workloads for the five benchmark configurations.
+
+All geometry and poses are generated in FP32 (the batched API's input
+format); the FP64 oracle consumes exactly the same FP32 values.
+Poses are (tx, ty, tz, qw, qx, qy, qz); rotations are uniform (Shoemake).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BOX, SPHERE, CAPSULE, CYLINDER, CONVEX = 0, 1, 3, 5, 100
+
+
+def random_quats(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Uniform unit quaternions (Shoemake 1992), returned as (w, x, y, z)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    x = a * np.sin(2 * np.pi * u2)
+    y = a * np.cos(2 * np.pi * u2)
+    z = b * np.sin(2 * np.pi * u3)
+    w = b * np.cos(2 * np.pi * u3)
+    return np.stack([w, x, y, z], axis=1)
+
+
+def poses(rng: np.random.Generator, n: int, lo: float, hi: float) -> np.ndarray:
+    p = np.empty((n, 7), np.float32)
+    p[:, 0:3] = rng.uniform(lo, hi, size=(n, 3))
+    p[:, 3:7] = random_quats(rng, n)
+    return p
+
+
+def primitive_pairs(n_pairs: int = 4096, seed: int = 1):
+    """4096 pairs over {box, sphere, capsule, cylinder}; half-extents, radii
+    and half-lengths U[0.05, 0.5] m, centres U[0, 1]^3 m, uniform rotations.
+    Shape 2i is pair i's first shape, 2i+1 its second."""
+    rng = np.random.default_rng(seed)
+    n = 2 * n_pairs
+    kinds = np.array([BOX, SPHERE, CAPSULE, CYLINDER], np.int32)
+    types = kinds[rng.integers(0, 4, size=n)]
+    half = rng.uniform(0.05, 0.5, size=(n, 3)).astype(np.float32)
+    params = np.zeros((n, 4), np.float32)
+    box = types == BOX
+    params[box, 0:3] = 2.0 * half[box]
+    sph = types == SPHERE
+    params[sph, 0] = half[sph, 0]
+    cap = (types == CAPSULE) | (types == CYLINDER)
+    params[cap, 0] = half[cap, 0]          # radius
+    params[cap, 1] = 2.0 * half[cap, 1]    # full length lz
+    pa = np.arange(0, n, 2, dtype=np.uint32)
+    pb = np.arange(1, n, 2, dtype=np.uint32)
+    posea = poses(rng, n_pairs, 0.0, 1.0)
+    poseb = poses(rng, n_pairs, 0.0, 1.0)
+    return dict(types=types, params=params, pair_a=pa, pair_b=pb, pose_a=posea, pose_b=poseb)
+
+
+def bounding_radius(types, params, vert_cnt=None, verts=None, vert_off=None):
+    """Radius of a ball around each shape's local origin containing it."""
+    t = np.asarray(types)
+    p = np.asarray(params, np.float64)
+    r = np.zeros(len(t))
+    r[t == BOX] = 0.5 * np.linalg.norm(p[t == BOX, 0:3], axis=1)
+    r[t == SPHERE] = p[t == SPHERE, 0]
+    m = t == CAPSULE
+    r[m] = p[m, 0] + 0.5 * p[m, 1]
+    m = t == CYLINDER
+    r[m] = np.hypot(p[m, 0], 0.5 * p[m, 1])
+    if verts is not None:
+        for i in np.nonzero(t == CONVEX)[0]:
+            v = verts[vert_off[i]:vert_off[i] + vert_cnt[i]]
+            r[i] = np.sqrt((v.astype(np.float64) ** 2).sum(1).max())
+    return r
+
+
+def ellipsoid_hulls(rng: np.random.Generator, n_hulls: int, nmin: int = 8, nmax: int = 512,
+                    amin: float = 0.05, amax: float = 0.4, chunk: int = 1 << 20):
+    """Vertex sets of hulls of N points on random ellipsoids: axes U[amin,
+    amax], N = round(exp(U[ln nmin, ln nmax])), x = diag(a) u with u uniform
+    on the unit sphere (Archimedes: z ~ U[-1, 1], phi ~ U[0, 2 pi)), so every
+    point lies on the ellipsoid surface."""
+    counts = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=n_hulls))).astype(np.uint32)
+    axes = rng.uniform(amin, amax, size=(n_hulls, 3)).astype(np.float32)
+    off = np.zeros(n_hulls, np.uint64)
+    off[1:] = np.cumsum(counts[:-1], dtype=np.uint64)
+    total = int(counts.sum(dtype=np.uint64))
+    verts = np.empty((total, 3), np.float32)
+    owner = np.repeat(np.arange(n_hulls, dtype=np.int32), counts.astype(np.intp))
+    # work buffers reused across chunks (fresh large temporaries are slow to fault in)
+    z = np.empty(chunk, np.float32)
+    phi = np.empty(chunk, np.float32)
+    rho = np.empty(chunk, np.float32)
+    t = np.empty(chunk, np.float32)
+    a = np.empty((chunk, 3), np.float32)
+    for s in range(0, total, chunk):
+        e = min(total, s + chunk)
+        m = e - s
+        zz, pp, rr, tt, aa = z[:m], phi[:m], rho[:m], t[:m], a[:m]
+        rng.random(out=zz, dtype=np.float32)
+        rng.random(out=pp, dtype=np.float32)
+        np.multiply(zz, np.float32(2.0), out=zz)
+        np.subtract(zz, np.float32(1.0), out=zz)
+        np.multiply(pp, np.float32(2.0 * np.pi), out=pp)
+        np.multiply(zz, zz, out=rr)
+        np.subtract(np.float32(1.0), rr, out=rr)
+        np.maximum(rr, np.float32(0.0), out=rr)
+        np.sqrt(rr, out=rr)
+        np.take(axes, owner[s:e], axis=0, out=aa)
+        np.cos(pp, out=tt)
+        np.multiply(tt, rr, out=tt)
+        np.multiply(tt, aa[:, 0], out=verts[s:e, 0])
+        np.sin(pp, out=tt)
+        np.multiply(tt, rr, out=tt)
+        np.multiply(tt, aa[:, 1], out=verts[s:e, 1])
+        np.multiply(zz, aa[:, 2], out=verts[s:e, 2])
+    return verts, off, counts, axes
+
+
+def hull_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int = 512):
+    """Batched convex-hull workload: 2 hulls per pair (shape 2i, 2i+1),
+    centres U[0, 2]^3 m, uniform rotations."""
+    rng = np.random.default_rng(seed)
+    n = 2 * n_pairs
+    verts, off, cnt, axes = ellipsoid_hulls(rng, n, nmin, nmax)
+    types = np.full(n, CONVEX, np.int32)
+    params = np.zeros((n, 4), np.float32)
+    pa = np.arange(0, n, 2, dtype=np.uint32)
+    pb = np.arange(1, n, 2, dtype=np.uint32)
+    posea = poses(rng, n_pairs, 0.0, 2.0)
+    poseb = poses(rng, n_pairs, 0.0, 2.0)
+    return dict(types=types, params=params, verts=verts, vert_off=off, vert_cnt=cnt, axes=axes,
+                pair_a=pa, pair_b=pb, pose_a=posea, pose_b=poseb)
+
+
+def hull_faces(v: np.ndarray):
+    """Convex hull of a point set: (indices of hull vertices, triangles over
+    those indices renumbered 0..k-1).  Uses qhull through scipy."""
+    from scipy.spatial import ConvexHull
+
+    h = ConvexHull(v.astype(np.float64))
+    keep = np.unique(h.simplices)
+    remap = -np.ones(len(v), np.int64)
+    remap[keep] = np.arange(len(keep))
+    return keep, remap[h.simplices].astype(np.int32)
+
+
+def icosphere(freq: int):
+    """Geodesic sphere: each icosahedron face split into freq^2 triangles,
+    vertices projected to the unit sphere; T = 20 freq^2."""
+    t = (1.0 + 5 ** 0.5) / 2
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t],
+                  [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    F = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                  [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+                  [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]])
+    n = freq
+    ij = [(i, j) for i in range(n + 1) for j in range(n + 1 - i)]
+    idx = {p: k for k, p in enumerate(ij)}
+    loc_tris = []
+    for i in range(n):
+        for j in range(n - i):
+            loc_tris.append((idx[(i, j)], idx[(i + 1, j)], idx[(i, j + 1)]))
+            if j < n - i - 1:
+                loc_tris.append((idx[(i + 1, j)], idx[(i + 1, j + 1)], idx[(i, j + 1)]))
+    loc_tris = np.array(loc_tris, np.int64)
+    bary = np.array(ij, np.float64) / n
+    pts, tris = [], []
+    for f, (a, b, c) in enumerate(F):
+        P = V[a][None] * (1 - bary[:, 0:1] - bary[:, 1:2]) + V[b][None] * bary[:, 0:1] + V[c][None] * bary[:, 1:2]
+        pts.append(P)
+        tris.append(loc_tris + f * len(ij))
+    P = np.concatenate(pts)
+    T = np.concatenate(tris)
+    P /= np.linalg.norm(P, axis=1, keepdims=True)
+    key = np.round(P * 1e9).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    return P[first], inv[T].astype(np.int32)
+
+
+def mesh_pool(n_meshes: int = 256, seed: int = 3, tmin: float = 1e3, tmax: float = 1e5):
+    """Icospheres with T* log-uniform in [tmin, tmax] (freq = round(sqrt(T*/20))),
+    radius U[0.1, 0.5] m, radial Gaussian noise with sigma = 2% of radius."""
+    rng = np.random.default_rng(seed)
+    tstar = np.exp(rng.uniform(np.log(tmin), np.log(tmax), size=n_meshes))
+    freqs = np.maximum(1, np.rint(np.sqrt(tstar / 20.0))).astype(int)
+    radii = rng.uniform(0.1, 0.5, size=n_meshes)
+    cache = {}
+    vs, ts = [], []
+    for k in range(n_meshes):
+        f = int(freqs[k])
+        if f not in cache:
+            cache[f] = icosphere(f)
+        P, T = cache[f]
+        rr = radii[k] * (1.0 + 0.02 * rng.standard_normal(len(P)))
+        vs.append((P * rr[:, None]).astype(np.float32))
+        ts.append(T)
+    vcnt = np.array([len(v) for v in vs], np.uint32)
+    tcnt = np.array([len(t) for t in ts], np.uint32)
+    voff = np.zeros(n_meshes, np.uint64)
+    toff = np.zeros(n_meshes, np.uint64)
+    voff[1:] = np.cumsum(vcnt[:-1], dtype=np.uint64)
+    toff[1:] = np.cumsum(tcnt[:-1], dtype=np.uint64)
+    return dict(verts=np.concatenate(vs), tris=np.concatenate(ts), vert_off=voff, vert_cnt=vcnt,
+                tri_off=toff, tri_cnt=tcnt, radius=radii, freq=freqs)
+
+
+def mesh_pairs(pool: dict, n_pairs: int = 4096, seed: int = 30, spread: float = 1.75):
+    """Pairs of pool meshes; centre distance d = U[0, spread] * (r1 + r2)
+    along a random direction, so that about 40% of the pairs have
+    intersecting surfaces (spheres intersect iff |r1 - r2| < d < r1 + r2)."""
+    rng = np.random.default_rng(seed)
+    m = len(pool["vert_cnt"])
+    a = rng.integers(0, m, size=n_pairs).astype(np.uint32)
+    b = rng.integers(0, m, size=n_pairs).astype(np.uint32)
+    r = pool["radius"]
+    d = rng.uniform(0.0, spread, size=n_pairs) * (r[a] + r[b])
+    u = rng.standard_normal((n_pairs, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    pa = np.empty((n_pairs, 7), np.float32)
+    pb = np.empty((n_pairs, 7), np.float32)
+    pa[:, 0:3] = rng.uniform(0.0, 1.0, size=(n_pairs, 3))
+    pa[:, 3:7] = random_quats(rng, n_pairs)
+    pb[:, 0:3] = pa[:, 0:3] + u * d[:, None]
+    pb[:, 3:7] = random_quats(rng, n_pairs)
+    return dict(pair_a=a, pair_b=b, pose_a=pa, pose_b=pb)

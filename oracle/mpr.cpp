@@ -1,0 +1,119 @@
+// CPU oracle -- TEST INFRASTRUCTURE ONLY (see omath.h).
+// MPR boolean: include/fcl/cvx_collide/mpr.hpp:34-185 (RunIntersect),
+// :188-328 (findPortal), :436-495 (updatePortal).
+#include <utility>
+
+#include "narrowphase.h"
+
+namespace oracle {
+namespace {
+
+constexpr double kDotEps = 2.220446049250313e-16;  // numeric_limits<double>::epsilon(), mpr.h:103
+
+// mpr.hpp:11-16: the direction is normalised in place.
+V3 support(const MinkDiff& md, V3& d) {
+  normalize(d);
+  return md.support(d);
+}
+
+enum class Portal { IterationLimit, Separated, Found };
+
+Portal findPortal(const MinkDiff& md, const V3& v0, V3& v1, V3& v2, V3& v3, V3& d1, V3& d2,
+                  V3& d3, int max_iter) {
+  const double v0n = absnorm(v0);
+  int it = 0;
+  while (true) {
+    if (it >= max_iter) return Portal::IterationLimit;
+    ++it;
+    V3 a1 = v1 - v0, a2 = v2 - v0, a3 = v3 - v0;
+    V3 n031 = cross(a3, a1);
+    V3 n012 = cross(a1, a2);
+    const double vol = dot(a2, n031);
+    if (vol < 0) {
+      std::swap(v2, v3);
+      std::swap(d2, d3);
+      std::swap(a2, a3);
+      std::swap(n012, n031);
+      n031 = -n031;
+      n012 = -n012;
+    }
+    if (dot(v0, n031) > kDotEps * v0n * absnorm(n031)) {
+      V3 s = -n031;
+      v2 = support(md, s);
+      d2 = s;
+      if (dot(v2, s) < 0) return Portal::Separated;
+      continue;
+    }
+    if (dot(v0, n012) > kDotEps * v0n * absnorm(n012)) {
+      V3 s = -n012;
+      v3 = support(md, s);
+      d3 = s;
+      if (dot(v3, s) < 0) return Portal::Separated;
+      continue;
+    }
+    const V3 n023 = cross(a2, a3);
+    if (dot(v0, n023) > kDotEps * v0n * absnorm(n023)) {
+      V3 s = -n023;
+      v1 = support(md, s);
+      d1 = s;
+      if (dot(v1, s) < 0) return Portal::Separated;
+      continue;
+    }
+    return Portal::Found;
+  }
+}
+
+void updatePortal(const V3& v0, const V3& v4, const V3& n, V3& v1, V3& v2, V3& v3, V3& d1,
+                  V3& d2, V3& d3) {
+  const V3 nn = cross(v4, v0);
+  if (dot(v1, nn) > 0) {
+    if (dot(v2, nn) > 0) { v1 = v4; d1 = n; }
+    else { v3 = v4; d3 = n; }
+  } else {
+    if (dot(v3, nn) > 0) { v2 = v4; d2 = n; }
+    else { v1 = v4; d1 = n; }
+  }
+}
+
+}  // namespace
+
+MPRStatus mprIntersect(const MinkDiff& md, int max_iter, double tol) {
+  const V3 v0 = md.interior();
+  if (sqnorm(v0) <= tol * tol) return MPRStatus::Intersect;
+  V3 d1 = -v0;
+  V3 v1 = support(md, d1);
+  if (dot(d1, v1) < 0) return MPRStatus::Separated;
+  V3 d2 = cross(v0, v1);
+  if (absnorm(d2) <= absnorm(v0) * absnorm(v1) * tol) return MPRStatus::Intersect;
+  V3 v2 = support(md, d2);
+  if (dot(d2, v2) < 0) return MPRStatus::Separated;
+  V3 d3 = cross(v1 - v0, v2 - v0);
+  if (dot(d3, v0) > 0) {
+    std::swap(v1, v2);
+    std::swap(d1, d2);
+    d3 = -d3;
+  }
+  V3 v3 = support(md, d3);
+  if (dot(d3, v3) < 0) return MPRStatus::Separated;
+  const Portal ps = findPortal(md, v0, v1, v2, v3, d1, d2, d3, max_iter);
+  if (ps == Portal::IterationLimit) return MPRStatus::Failed;
+  if (ps == Portal::Separated) return MPRStatus::Separated;
+  int it = 0;
+  while (it < max_iter) {
+    ++it;
+    V3 n = cross(v2 - v1, v3 - v1);
+    if (dot(n, v0) > 0) {
+      std::swap(v2, v3);
+      std::swap(d2, d3);
+      n = -n;
+    }
+    if (!(dot(v1, n) < 0)) return MPRStatus::Intersect;
+    const V3 v4 = support(md, n);  // n is normalised in place
+    if (dot(v4, n) < 0) return MPRStatus::Separated;
+    if (std::fabs(dot(v4 - v1, n)) < tol * absnorm(n)) return MPRStatus::Separated;
+    updatePortal(v0, v4, n, v1, v2, v3, d1, d2, d3);
+  }
+  return MPRStatus::Failed;
+}
+
+}  // namespace oracle
