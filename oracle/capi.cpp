@@ -289,6 +289,37 @@ int64_t orc_mesh_collide_all(void* m, int ia, int ib, const double* tfa12, const
   return static_cast<int64_t>(o.contacts.size());
 }
 
+// Every colliding triangle pair by exhaustive search (no BVH), with the same
+// relative transform and triangle test the traversal uses; pins the BVH
+// build and traversal (test/test_fcl_collision.cpp:915-1059 compares the
+// BVH result against the same brute force).
+int64_t orc_mesh_collide_brute(void* m, int ia, int ib, const double* tfa12, const double* tfb12,
+                               int32_t* out_pairs, int64_t cap) {
+  const auto* ms = static_cast<MeshSet*>(m);
+  const Mesh& m1 = ms->meshes[ia];
+  const Mesh& m2 = ms->meshes[ib];
+  const Tf tf1 = tfFrom12(tfa12), tf2 = tfFrom12(tfb12);
+  const M3 R = transpose(tf1.R) * tf2.R;
+  const V3 T = tmul(tf1.R, tf2.t - tf1.t);
+  const int n1 = static_cast<int>(m1.tris.size() / 3), n2 = static_cast<int>(m2.tris.size() / 3);
+  int64_t k = 0;
+  for (int p1 = 0; p1 < n1; ++p1) {
+    const int* ta = &m1.tris[3 * p1];
+    for (int p2 = 0; p2 < n2; ++p2) {
+      const int* tb = &m2.tris[3 * p2];
+      if (!triTriIntersect(m1.verts[ta[0]], m1.verts[ta[1]], m1.verts[ta[2]], m2.verts[tb[0]],
+                           m2.verts[tb[1]], m2.verts[tb[2]], R, T))
+        continue;
+      if (k < cap) {
+        out_pairs[2 * k] = p1;
+        out_pairs[2 * k + 1] = p2;
+      }
+      ++k;
+    }
+  }
+  return k;
+}
+
 // Brute-force triangle-triangle test of two triangles given in their own
 // frames with the relative transform of the second into the first.
 int orc_tri_tri(const double* P9, const double* Q9, const double* R9, const double* T3) {

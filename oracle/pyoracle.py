@@ -64,6 +64,8 @@ def lib() -> C.CDLL:
         L.orc_mesh_collide_batch.restype = d
         L.orc_mesh_collide_all.argtypes = [p, i32, i32, p, p, p, i64]
         L.orc_mesh_collide_all.restype = i64
+        L.orc_mesh_collide_brute.argtypes = [p, i32, i32, p, p, p, i64]
+        L.orc_mesh_collide_brute.restype = i64
         L.orc_tri_tri.argtypes = [p, p, p, p]
         L.orc_obb_disjoint.argtypes = [p, p, p, p]
         L.orc_aabbtree_create.argtypes = [p, p, p, i32]
@@ -219,6 +221,13 @@ class MeshSet:
         out = np.zeros((cap, 2), np.int32)
         n = lib().orc_mesh_collide_all(self.h, ia, ib, _p(np.ascontiguousarray(tfa12, np.float64)),
                                        _p(np.ascontiguousarray(tfb12, np.float64)), _p(out), cap)
+        return out[:min(n, cap)], int(n)
+
+    def collide_brute(self, ia, ib, tfa12, tfb12, cap=1 << 20):
+        """All colliding triangle pairs by exhaustive search (no BVH)."""
+        out = np.zeros((cap, 2), np.int32)
+        n = lib().orc_mesh_collide_brute(self.h, ia, ib, _p(np.ascontiguousarray(tfa12, np.float64)),
+                                         _p(np.ascontiguousarray(tfb12, np.float64)), _p(out), cap)
         return out[:min(n, cap)], int(n)
 
 
