@@ -100,7 +100,21 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES bytes 3 times (the known-size "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        x = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (pairs would shard by "
               "contiguous chunks with replicated geometry); run with --gpus 1", file=sys.stderr)
@@ -122,6 +136,16 @@ def main():
     ctx.sync()
     log(f"[upload] {time.time() - t0:.1f}s")
     req = mfcl.request(mfcl.MODE_SIGNED_DISTANCE)
+    tuned = {}
+    if not args.no_tune:
+        from mfcl import autotune
+
+        def run():
+            ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, req, dout)
+
+        t_group = autotune.tune(ctx, autotune.GJK_GROUP, [1, 8, 16], run, reps=2, log=log)
+        t_epa = autotune.tune(ctx, autotune.EPA_THREADS, [8192, 32768, 131072], run, reps=2, log=log)
+        tuned = {"gjk_group": min(t_group, key=t_group.get), "epa_threads": min(t_epa, key=t_epa.get)}
     ctx.set_timing(True)
     for _ in range(args.warmup):
         ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, req, dout)
@@ -144,7 +168,16 @@ def main():
     t_gjk = float(kern[:, 1].mean()) * 1e-3
     achieved = alg_bytes / t_gjk / 1e9
     log(f"[gpu] {ms_step:.3f} ms/step; kernels ms (order, gjk, epa) = {kern.mean(0)}; "
-        f"epa pairs {int(counters[2])}; vertices scanned {int(counters[1])}")
+        f"epa pairs {int(counters[2])} (rerun at full capacity {int(counters[3])}); "
+        f"vertices scanned {int(counters[1])}")
+    ep = out["epa_status"] >= 0
+    if ep.any():
+        it = out["epa_iters"][ep].astype(np.int64)
+        log(f"[gpu] EPA status counts {dict(zip(*np.unique(out['epa_status'][ep], return_counts=True)))}; "
+            f"iterations mean {it.mean():.2f} p50 {np.percentile(it, 50):.0f} p99 {np.percentile(it, 99):.0f} "
+            f"max {it.max()}")
+    gi = out["gjk_iters"].astype(np.int64)
+    log(f"[gpu] GJK iterations mean {gi.mean():.2f} p99 {np.percentile(gi, 99):.0f} max {gi.max()}")
     result = {
         "metric": "pair queries/s, 1 MI355X: GJK/EPA signed distance over convex-hull pairs",
         "value": value,
@@ -164,6 +197,7 @@ def main():
             "pairs": n,
             "mean_vertices_per_hull": float(nv.mean()),
             "intersecting_fraction": float((out["distance"] < 0).mean()),
+            "launch": tuned,
         },
         "roofline": {
             "kernel": "narrowKernel (GJK stage)",

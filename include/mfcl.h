@@ -112,8 +112,18 @@ int mfcl_set_timing(mfcl_ctx* ctx, int enabled);
  * stage, ms[3] mesh traversal, ms[4] broadphase, ms[5] MLP. */
 int mfcl_last_timing(mfcl_ctx* ctx, float* ms6);
 /* Work counters of the last convex query: [0] pairs, [1] hull vertices
- * scanned by support mapping, [2] EPA pairs. */
+ * scanned by support mapping, [2] EPA pairs, [3] EPA pairs rerun at full
+ * polytope capacity after exhausting the on-chip polytope. */
 int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4);
+
+/* Launch-shape knobs (no counterpart in the reference: GPU scheduling
+ * parameters, chosen by the autotuner in mind-fcl_amd/mfcl/autotune.py). */
+#define MFCL_TUNE_GJK_GROUP 1         /* lanes per pair in the GJK stage: 1, 8, 16 */
+#define MFCL_TUNE_EPA_THREADS 2       /* persistent EPA lanes (64 per pair), multiple of 256 */
+#define MFCL_TUNE_MESH_BLOCKS_PER_CU 3 /* persistent traversal blocks (4 wavefronts) per CU, 1..16 */
+#define MFCL_TUNE_MESH_STACK 4        /* node-pair stack entries per wavefront, 1024..262144 */
+#define MFCL_TUNE_EPA_LDS_FACES 5     /* face capacity of the LDS polytope, 16..128 */
+int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value);
 
 /* ---- device memory helpers (so callers need no HIP runtime) ------------- */
 int mfcl_dev_alloc(mfcl_ctx* ctx, uint64_t bytes, void** dptr);

@@ -545,9 +545,9 @@ int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   }
   hipDeviceProp_t prop;
   MFCL_HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
-  const int blocks = prop.multiProcessorCount * 4;
+  const int blocks = prop.multiProcessorCount * ctx->tune.mesh_blocks_per_cu;
   const int waves = blocks * 4;
-  const uint32_t cap = 16384;
+  const uint32_t cap = static_cast<uint32_t>(ctx->tune.mesh_stack);
   if (int rc = ensure(ctx, ctx->ws_arena, static_cast<size_t>(waves) * cap * sizeof(uint64_t))) return rc;
   if (int rc = ensure(ctx, ctx->ws_misc, 2048)) return rc;
   uint32_t* next = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->ws_misc.p) + 1024);

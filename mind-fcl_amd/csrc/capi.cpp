@@ -68,6 +68,7 @@ void freeAll(mfcl_ctx* c) {
   freePtr(c->ws_arena.p);
   freePtr(c->ws_misc.p);
   freePtr(c->ws_io.p);
+  freePtr(c->ws_oflow.p);
 }
 
 // Host builders (bvh_build.cpp, broadphase.hip).
@@ -161,15 +162,45 @@ int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4) {
   if (ctx->d_counters) {
     MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     uint64_t dev[2] = {0, 0};
-    uint32_t q = 0;
+    uint32_t q[2] = {0, 0};  // EPA queue count, overflow count
     MFCL_HIP_TRY(ctx, hipMemcpy(dev, ctx->d_counters, sizeof(dev), hipMemcpyDeviceToHost));
-    MFCL_HIP_TRY(ctx, hipMemcpy(&q, static_cast<char*>(ctx->ws_misc.p) + 1024, sizeof(q), hipMemcpyDeviceToHost));
+    MFCL_HIP_TRY(ctx, hipMemcpy(q, static_cast<char*>(ctx->ws_misc.p) + 1024, sizeof(q), hipMemcpyDeviceToHost));
     ctx->counters[1] = dev[0] + dev[1];
-    ctx->counters[2] = q;
-    ctx->counters[3] = dev[1];
+    ctx->counters[2] = q[0];
+    ctx->counters[3] = q[1];
   }
   std::memcpy(c4, ctx->counters, sizeof(ctx->counters));
   return MFCL_OK;
+}
+
+int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
+  if (!ctx) return MFCL_E_INVALID;
+  switch (knob) {
+    case MFCL_TUNE_GJK_GROUP:
+      if (value != 1 && value != 8 && value != 16) break;
+      ctx->tune.gjk_group = value;
+      return MFCL_OK;
+    case MFCL_TUNE_EPA_THREADS:
+      if (value < 256 || value % 256 != 0 || value > (1 << 20)) break;
+      ctx->tune.epa_threads = value;
+      return MFCL_OK;
+    case MFCL_TUNE_MESH_BLOCKS_PER_CU:
+      if (value < 1 || value > 16) break;
+      ctx->tune.mesh_blocks_per_cu = value;
+      return MFCL_OK;
+    case MFCL_TUNE_EPA_LDS_FACES:
+      if (value < 16 || value > 128) break;
+      ctx->tune.epa_lds_faces = value;
+      return MFCL_OK;
+    case MFCL_TUNE_MESH_STACK:
+      if (value < 1024 || value > (1 << 18)) break;
+      ctx->tune.mesh_stack = value;
+      return MFCL_OK;
+    default:
+      break;
+  }
+  ctx->err = "invalid tuning knob/value";
+  return MFCL_E_INVALID;
 }
 
 int mfcl_dev_alloc(mfcl_ctx* ctx, uint64_t bytes, void** dptr) {

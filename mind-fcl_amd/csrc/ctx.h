@@ -34,6 +34,15 @@ struct MeshHost {
 
 struct mfcl_ctx {
   int device{0};
+  // launch-shape knobs (set by mfcl_tune / the autotuner)
+  struct {
+    int gjk_group{16};          // lanes per pair in the GJK stage (1, 8, 16)
+    int epa_threads{32768};     // persistent EPA lanes (multiple of 256): 64 per wavefront
+    int epa_lds_faces{64};      // face capacity of the LDS polytope (overflow reruns at full size)
+    int epa_overflow_waves{4096};  // persistent wavefronts for overflow pairs
+    int mesh_blocks_per_cu{4};  // persistent mesh-traversal blocks (4 waves) per CU
+    int mesh_stack{16384};      // node-pair stack entries per wavefront
+  } tune;
   std::string err;
   hipStream_t stream{nullptr};
 
@@ -62,7 +71,7 @@ struct mfcl_ctx {
   int64_t* d_bp_ids{nullptr};
 
   // growable device workspaces, keyed by purpose
-  mfcl::DevBuf ws_order, ws_queue, ws_arena, ws_misc, ws_io;
+  mfcl::DevBuf ws_order, ws_queue, ws_arena, ws_misc, ws_io, ws_oflow;
 
   // timing
   bool timing{false};

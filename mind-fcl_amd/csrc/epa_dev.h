@@ -259,7 +259,7 @@ struct Poly {
   }
 
   // epa_polytope_expand.hpp:32-281. 0 ok, 1 failed, 2 malloc failed
-  MFCL_D int expand(d3 nv, d3 nd, int16_t start) {
+  __device__ __attribute__((noinline)) int expand(d3 nv, d3 nd, int16_t start) {
     for (int16_t i = vl; i != kNil; i = V[i].next) { V[i].remove = 1; V[i].new_edge = kNil; }
     for (int16_t i = el; i != kNil; i = E[i].next) E[i].vis = kUnknown;
     for (int16_t i = fl; i != kNil; i = F[i].next) F[i].vis = kFUnknown;
@@ -369,14 +369,15 @@ struct EpaOut {
   int iters;
 };
 
-MFCL_D void touchingPoint(Mink& md, d3 d, EpaOut& o) {
+template <class MK>
+MFCL_D void touchingPoint(MK& md, d3 d, EpaOut& o) {
   const d3 mid = (md.support0(d) + md.support1(-d)) / 2;
   o.p0 = mid;
   o.p1 = mid;
 }
 
 // epa_simplex2polytope.hpp:184-217; returns 0 ok, 2 failed
-MFCL_D int tetra(Poly& P, const MV& a, const MV& b, const MV& c, const MV& d) {
+__device__ __attribute__((noinline)) int tetra(Poly& P, const MV& a, const MV& b, const MV& c, const MV& d) {
   P.reset();
   int16_t v[4], e[6], f[4];
   v[0] = P.addVertex(a.v, a.d);
@@ -399,7 +400,8 @@ MFCL_D int tetra(Poly& P, const MV& a, const MV& b, const MV& c, const MV& d) {
 }
 
 // s2p status: 0 OK, 1 touching, 2 failed
-MFCL_D int s2p3(Poly& P, Mink& md, const MV& a, const MV& b, const MV& c, double thr, EpaOut& o) {
+template <class MK>
+__device__ __attribute__((noinline)) int s2p3(Poly& P, MK& md, const MV& a, const MV& b, const MV& c, double thr, EpaOut& o) {
   d3 n = cross(b.v - a.v, c.v - a.v);
   if (sqnorm(n) <= 0.0) return 2;
   n = normalized(n);
@@ -414,7 +416,8 @@ MFCL_D int s2p3(Poly& P, Mink& md, const MV& a, const MV& b, const MV& c, double
   return tetra(P, a, b, c, MV{d1, dir1});
 }
 
-MFCL_D int s2p2(Poly& P, Mink& md, const MV& a, const MV& b, double thr, EpaOut& o) {
+template <class MK>
+__device__ __attribute__((noinline)) int s2p2(Poly& P, MK& md, const MV& a, const MV& b, double thr, EpaOut& o) {
   const double thr2 = thr * thr;
   const d3 ab = b.v - a.v;
   if (sqnorm(ab) < thr2) return 2;
@@ -488,7 +491,8 @@ MFCL_D int s2p2(Poly& P, Mink& md, const MV& a, const MV& b, double thr, EpaOut&
 }
 
 // epa_simplex2polytope.hpp:46-110
-MFCL_D int simplexToPolytope(Poly& P, Mink& md, const Smp& s, double thr, EpaOut& o) {
+template <class MK>
+MFCL_D int simplexToPolytope(Poly& P, MK& md, const Smp& s, double thr, EpaOut& o) {
   const double thr2 = thr * thr;
   if (s.rank > 0 && sqnorm(s.a.v) <= thr2) { touchingPoint(md, s.a.d, o); return 1; }
   if (s.rank > 1 && sqnorm(s.b.v) <= thr2) { touchingPoint(md, s.b.d, o); return 1; }
@@ -538,7 +542,8 @@ MFCL_D void toRaw(const Poly& P, int kind, int16_t i, RawF& r) {
 }
 
 // epa.hpp:11-113; returns 0 ok, 1 failed, 2 converge
-MFCL_D int nextDir(const Poly& P, Mink& md, int kind, int16_t fi, double tol, d3& nd, d3& nv, int16_t& start) {
+template <class MK>
+MFCL_D int nextDir(const Poly& P, MK& md, int kind, int16_t fi, double tol, d3& nd, d3& nv, int16_t& start) {
   constexpr double kOuter = 1e-3 * 1e-3;
   auto face_cand = [&](int16_t f, bool try_w) -> int {
     d3 fn;
@@ -588,7 +593,8 @@ MFCL_D bool epaTerminate(const RawF& f, d3 nv, double tol) {
 }
 
 // epa.hpp:341-498
-MFCL_D void assignPair(Mink& md, d3 cand_d, const RawF& f, EpaOut& o) {
+template <class MK>
+__device__ __attribute__((noinline)) void assignPair(MK& md, d3 cand_d, const RawF& f, EpaOut& o) {
   if (f.kind == 1) {
     o.p0 = md.support0(f.a.d);
     o.p1 = md.support1(-f.a.d);
@@ -638,7 +644,8 @@ MFCL_D void assignPair(Mink& md, d3 cand_d, const RawF& f, EpaOut& o) {
 }
 
 // epa.hpp:137-257
-MFCL_D void epaEvaluate(Poly& P, Mink& md, const Smp& s, double tol, int max_iter, EpaOut& o) {
+template <class MK>
+MFCL_D void epaEvaluate(Poly& P, MK& md, const Smp& s, double tol, int max_iter, EpaOut& o) {
   o.iters = 0;
   o.depth = 0;
   o.p0 = d3{0, 0, 0};

@@ -52,55 +52,78 @@ MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
   return s;
 }
 
-// FP64 first argmax of dot(d, v) (the reference's naive scan order).
-MFCL_D int hullArgmax64(const float4* v, int nv, d3 d) {
-  double bd = d.x * static_cast<double>(v[0].x) + d.y * static_cast<double>(v[0].y) + d.z * static_cast<double>(v[0].z);
-  int bi = 0;
-  for (int i = 1; i < nv; ++i) {
-    const float4 q = v[i];
+// Support-point search over a hull, shared by the G lanes that own a pair
+// (G = 1: one lane per pair).  Lane lg scans vertices lg, lg + G, ... in
+// FP32 keeping the best and runner-up scores; a butterfly over the group
+// merges them (ties resolve to the lower index).  When the FP32 gap is
+// inside the rounding bound the group rescans in FP64, so the result is the
+// first vertex attaining the FP64 maximum of dot(d, v) -- the reference's
+// naive scan result (convex-inl.h:133-149), which the reference's adjacency
+// walk also returns whenever that maximum is unique.
+template <int G>
+MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
+  int bi = 0x7fffffff;
+  float best = -INFINITY, second = -INFINITY;
+  {
+#pragma clang fp contract(fast)
+    const float fx = static_cast<float>(d.x), fy = static_cast<float>(d.y), fz = static_cast<float>(d.z);
+    for (int i = lg; i < s.nv; i += G) {
+      const float4 q = s.verts[i];
+      const float val = fx * q.x + fy * q.y + fz * q.z;
+      if (val > best) {
+        second = best;
+        best = val;
+        bi = i;
+      } else if (val > second) {
+        second = val;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    const float ob = __shfl_xor(best, off, G);
+    const float os = __shfl_xor(second, off, G);
+    const int oi = __shfl_xor(bi, off, G);
+    if (ob > best || (ob == best && oi < bi)) {
+      second = fmaxf(fmaxf(second, os), best);
+      best = ob;
+      bi = oi;
+    } else {
+      second = fmaxf(fmaxf(second, os), ob);
+    }
+  }
+  *scanned += s.nv;
+  // |fl32(d.v) - d.v| <= (rounding of d + 3 roundings of the dot) * |d|_1 max|v|
+  // <= 8 * 2^-24 * |d|_1 * rbound; the gap test uses twice that.
+  const float bound = 2.0f * 8.0f * 5.9604645e-08f * s.rbound *
+                      (fabsf(static_cast<float>(d.x)) + fabsf(static_cast<float>(d.y)) + fabsf(static_cast<float>(d.z)));
+  if (best - second > bound) return bi;
+  *scanned += s.nv;
+  double bd = -INFINITY;
+  bi = 0x7fffffff;
+  for (int i = lg; i < s.nv; i += G) {
+    const float4 q = s.verts[i];
     const double val = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
     if (val > bd) {
       bd = val;
       bi = i;
     }
   }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    const double ob = __shfl_xor(bd, off, G);
+    const int oi = __shfl_xor(bi, off, G);
+    if (ob > bd || (ob == bd && oi < bi)) {
+      bd = ob;
+      bi = oi;
+    }
+  }
   return bi;
 }
 
-// FP32 scan with an FP64 recheck near ties; returns the index of the first
-// vertex attaining the FP64 maximum of dot(d, v) whenever that maximum is
-// separated from the runner-up by more than the FP32 rounding bound, and the
-// FP64 first argmax otherwise.
-MFCL_D int hullArgmax(const DevShape& s, d3 d, int* scanned) {
-#pragma clang fp contract(fast)
-  const float fx = static_cast<float>(d.x), fy = static_cast<float>(d.y), fz = static_cast<float>(d.z);
-  const float4* v = s.verts;
-  float4 q = v[0];
-  float best = fx * q.x + fy * q.y + fz * q.z;
-  float second = -INFINITY;
-  int bi = 0;
-  for (int i = 1; i < s.nv; ++i) {
-    q = v[i];
-    const float val = fx * q.x + fy * q.y + fz * q.z;
-    if (val > best) {
-      second = best;
-      best = val;
-      bi = i;
-    } else if (val > second) {
-      second = val;
-    }
-  }
-  *scanned += s.nv;
-  // |fl(d.v) - d.v| <= (rounding of d to FP32 + 3 FMA roundings) * |d|_1 * max|v|
-  // <= 8 * 2^-24 * sqrt(3) * rbound for |d| = 1; gap test uses twice that.
-  const float bound = 2.0f * 8.0f * 5.9604645e-08f * s.rbound * (fabsf(fx) + fabsf(fy) + fabsf(fz));
-  if (best - second > bound) return bi;
-  *scanned += s.nv;
-  return hullArgmax64(v, s.nv, d);
-}
-
 // include/fcl/cvx_collide/gjk_shape.hpp:41-180; shape_gjk_interface-inl.h:87-92.
-MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int* scanned) {
+template <int G>
+MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int lg, int* scanned) {
   switch (s.type) {
     case kBox:
       return {dir.x > 0 ? s.p0 / 2 : -s.p0 / 2, dir.y > 0 ? s.p1 / 2 : -s.p1 / 2, dir.z > 0 ? s.p2 / 2 : -s.p2 / 2};
@@ -120,7 +143,7 @@ MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int* scanned) {
       return {k * dir.x, k * dir.y, dir.z > 0 ? hh : -hh};
     }
     case kConvex: {
-      const float4 q = s.verts[hullArgmax(s, dir, scanned)];
+      const float4 q = s.verts[hullArgmax<G>(s, dir, lg, scanned)];
       return {q.x, q.y, q.z};
     }
     default:
@@ -136,25 +159,31 @@ MFCL_D d3 shapeInterior(const DevShape& s, const float4* interior_tab, uint32_t 
   return {0, 0, 0};
 }
 
-// MinkowskiDiff (include/fcl/cvx_collide/minkowski_diff.hpp:14-85).
-struct Mink {
+// MinkowskiDiff (include/fcl/cvx_collide/minkowski_diff.hpp:14-85), evaluated
+// by a group of G lanes (lane lg of the group).
+template <int G>
+struct MinkT {
   DevShape s0, s1;
   m33 toshape1;  // frame 0 -> frame 1 rotation
   tf3 toshape0;  // frame 1 -> frame 0
   d3 int0, int1; // interior points in each shape's own frame
   int scanned;   // hull vertices scanned (for the VALU accounting)
-  MFCL_D d3 support0(d3 d) { return shapeSupport(s0, d, &scanned); }
-  MFCL_D d3 support1(d3 d) { return apply(toshape0, shapeSupport(s1, mul(toshape1, d), &scanned)); }
+  int lg;
+  MFCL_D d3 support0(d3 d) { return shapeSupport<G>(s0, d, lg, &scanned); }
+  MFCL_D d3 support1(d3 d) { return apply(toshape0, shapeSupport<G>(s1, mul(toshape1, d), lg, &scanned)); }
   MFCL_D d3 support(d3 d) { return support0(d) - support1(-d); }
   MFCL_D d3 interior() { return int0 - apply(toshape0, int1); }
 };
+using Mink = MinkT<1>;
 
-MFCL_D void setupMink(Mink& md, const DevShape& a, const tf3& ta, const DevShape& b, const tf3& tb) {
+template <class MK>
+MFCL_D void setupMink(MK& md, const DevShape& a, const tf3& ta, const DevShape& b, const tf3& tb, int lg) {
   md.s0 = a;
   md.s1 = b;
   md.toshape1 = tmulm(tb.R, ta.R);
   md.toshape0 = relTf(ta, tb);
   md.scanned = 0;
+  md.lg = lg;
   md.int0 = d3{0, 0, 0};
   md.int1 = d3{0, 0, 0};
 }
@@ -266,96 +295,108 @@ MFCL_D int gjkProject(Smp& s, d3& dir, double tol) {
 }
 
 // ---------------------------------------------------------------- distance
-// gjk_distance.hpp:128-154 on a two-slot simplex (x = old slot 0, y = new slot 1).
-// Writes the result simplex back into (x, y, rank).
-MFCL_D void dist2(MV& x, MV& y, int& rank, d3& out, double tol) {
-  const d3 s1 = y.v, s2 = x.v;
+// Sub-simplex outcomes are encoded as the ordered list of surviving input
+// slots: bits [0,2) rank, then 2 bits per kept slot index, oldest first.
+MFCL_D int code1(int a) { return 1 | (a << 2); }
+MFCL_D int code2(int a, int b) { return 2 | (a << 2) | (b << 4); }
+MFCL_D int code3(int a, int b, int c) { return 3 | (a << 2) | (b << 4) | (c << 6); }
+MFCL_D int codeRank(int c) { return c & 3; }
+MFCL_D int codeSlot(int c, int k) { return (c >> (2 + 2 * k)) & 3; }
+// Re-express a code over local slots through a map local -> outer slot.
+MFCL_D int codeMap(int c, int m0, int m1, int m2) {
+  const int r = codeRank(c);
+  int o = r;
+  for (int k = 0; k < r; ++k) {
+    const int l = codeSlot(c, k);
+    const int g = l == 0 ? m0 : (l == 1 ? m1 : m2);
+    o |= g << (2 + 2 * k);
+  }
+  return o;
+}
+
+// gjk_distance.hpp:128-154: segment (old x = slot 0, new y = slot 1).
+MFCL_D int dist2(d3 x, d3 y, d3& out, double tol) {
+  const d3 s1 = y, s2 = x;
   const d3 d = s2 - s1;
   const double l2 = sqnorm(d);
   const double t = -dot(s1, d);
   if (t <= 0 || l2 <= tol * tol) {
     out = s1;
-    x = y;
-    rank = 1;
-  } else if (t >= l2) {
-    out = s2;
-    rank = 1;
-  } else {
-    const double w2 = t / l2;
-    out = w2 * s2 + (1.0 - w2) * s1;
-    rank = 2;
+    return code1(1);
   }
+  if (t >= l2) {
+    out = s2;
+    return code1(0);
+  }
+  const double w2 = t / l2;
+  out = w2 * s2 + (1.0 - w2) * s1;
+  return code2(0, 1);
 }
 
-// gjk_distance.hpp:156-289 on three slots (s3 = x, s2 = y, s1(new) = z).
-// Result simplex written to (x, y, z, rank).
-MFCL_D void dist3(MV& x, MV& y, MV& z, int& rank, d3& out, double tol) {
-  const d3 s1 = z.v, s2 = y.v, s3 = x.v;
+// gjk_distance.hpp:156-289: triangle (s3 = slot 0, s2 = slot 1, s1 new = slot 2).
+MFCL_D int dist3(d3 x, d3 y, d3 z, d3& out, double tol) {
+  const d3 s1 = z, s2 = y, s3 = x;
   const d3 s12 = s2 - s1, s13 = s3 - s1;
   const bool sep2 = dot(s1, s12) >= 0, sep3 = dot(s1, s13) >= 0;
   if (sep2 && sep3) {
     out = s1;
-    x = z;
-    rank = 1;
-    return;
+    return code1(2);
   }
   const d3 n = cross(s12, s13);
   const double area2 = sqnorm(n);
   const bool zero = area2 <= 0.0;
   const bool s12sep = dot(s1, cross(n, s12)) > 0;
-  if (!sep2 && s12sep) {
-    MV p = y, q = z;
-    int r;
-    dist2(p, q, r, out, tol);
-    x = p; y = q; rank = r;
-    return;
-  }
+  if (!sep2 && s12sep) return codeMap(dist2(y, z, out, tol), 1, 2, 0);
   const bool s13sep = dot(s1, cross(n, s13)) < 0;
-  if (!sep3 && s13sep) {
-    MV p = x, q = z;
-    int r;
-    dist2(p, q, r, out, tol);
-    x = p; y = q; rank = r;
-    return;
-  }
+  if (!sep3 && s13sep) return codeMap(dist2(x, z, out, tol), 0, 2, 0);
   double best = -1;
   d3 bpt = {0, 0, 0};
-  MV bx = x, by = y;
-  int br = 0;
+  int bc = 0;
   if (zero || s12sep) {
-    MV p = y, q = z;
-    int r;
     d3 pt;
-    dist2(p, q, r, pt, tol);
+    const int c = codeMap(dist2(y, z, pt, tol), 1, 2, 0);
     const double dd = sqnorm(pt);
-    if (best < 0 || dd < best) { best = dd; bpt = pt; bx = p; by = q; br = r; }
+    if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
   }
   if (zero || s13sep) {
-    MV p = x, q = z;
-    int r;
     d3 pt;
-    dist2(p, q, r, pt, tol);
+    const int c = codeMap(dist2(x, z, pt, tol), 0, 2, 0);
     const double dd = sqnorm(pt);
-    if (best < 0 || dd < best) { best = dd; bpt = pt; bx = p; by = q; br = r; }
+    if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
   }
   const bool s23sep = dot(s2, cross(n, s3 - s2)) > 0;
   if (zero || s23sep) {
-    MV p = x, q = y;
-    int r;
     d3 pt;
-    dist2(p, q, r, pt, tol);
+    const int c = dist2(x, y, pt, tol);
     const double dd = sqnorm(pt);
-    if (best < 0 || dd < best) { best = dd; bpt = pt; bx = p; by = q; br = r; }
+    if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
   }
   if (best < 0) {
     out = n * (dot(s1, n) / area2);
-    rank = 3;
-  } else {
-    x = bx;
-    y = by;
-    rank = br;
-    out = bpt;
+    return code3(0, 1, 2);
   }
+  out = bpt;
+  return bc;
+}
+
+// Slot k of the simplex by value through selects (no indexed private array).
+MFCL_D MV smpSlot(const Smp& s, int k) {
+  MV r = s.a;
+  if (k == 1) r = s.b;
+  if (k == 2) r = s.c;
+  if (k == 3) r = s.e;
+  return r;
+}
+
+MFCL_D void smpApply(Smp& s, int code) {
+  const int r = codeRank(code);
+  const MV n0 = smpSlot(s, codeSlot(code, 0));
+  const MV n1 = smpSlot(s, codeSlot(code, 1));
+  const MV n2 = smpSlot(s, codeSlot(code, 2));
+  s.a = n0;
+  if (r > 1) s.b = n1;
+  if (r > 2) s.c = n2;
+  s.rank = r;
 }
 
 enum DistUpd : int { kNoImprovement, kUpdOK, kUpdFailed };
@@ -366,48 +407,38 @@ MFCL_D int distUpdate(Smp& s, d3& out, double tol) {
     return kUpdOK;
   }
   if (s.rank == 2) {
-    dist2(s.a, s.b, s.rank, out, tol);
+    smpApply(s, dist2(s.a.v, s.b.v, out, tol));
     return kUpdOK;
   }
   if (s.rank == 3) {
-    dist3(s.a, s.b, s.c, s.rank, out, tol);
+    smpApply(s, dist3(s.a.v, s.b.v, s.c.v, out, tol));
     return kUpdOK;
   }
   if (s.rank == 4) {
     // gjk_distance.hpp:291-373: faces (3,2,1), (3,2,0), (3,1,0)
     double best = -1;
     d3 bpt = {0, 0, 0};
-    MV bx = s.a, by = s.b, bz = s.c;
-    int br = 0;
+    int bc = 0;
     {
-      MV x = s.b, y = s.c, z = s.e;
-      int r;
       d3 pt;
-      dist3(x, y, z, r, pt, tol);
+      const int c = codeMap(dist3(s.b.v, s.c.v, s.e.v, pt, tol), 1, 2, 3);
       const double dd = sqnorm(pt);
-      if (best < 0 || dd < best) { best = dd; bpt = pt; bx = x; by = y; bz = z; br = r; }
+      if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
     }
     {
-      MV x = s.a, y = s.c, z = s.e;
-      int r;
       d3 pt;
-      dist3(x, y, z, r, pt, tol);
+      const int c = codeMap(dist3(s.a.v, s.c.v, s.e.v, pt, tol), 0, 2, 3);
       const double dd = sqnorm(pt);
-      if (best < 0 || dd < best) { best = dd; bpt = pt; bx = x; by = y; bz = z; br = r; }
+      if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
     }
     {
-      MV x = s.a, y = s.b, z = s.e;
-      int r;
       d3 pt;
-      dist3(x, y, z, r, pt, tol);
+      const int c = codeMap(dist3(s.a.v, s.b.v, s.e.v, pt, tol), 0, 1, 3);
       const double dd = sqnorm(pt);
-      if (best < 0 || dd < best) { best = dd; bpt = pt; bx = x; by = y; bz = z; br = r; }
+      if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
     }
     if (best < 0) return kNoImprovement;
-    s.a = bx;
-    s.b = by;
-    s.c = bz;
-    s.rank = br;
+    smpApply(s, bc);
     out = bpt;
     return kUpdOK;
   }
@@ -417,17 +448,16 @@ MFCL_D int distUpdate(Smp& s, d3& out, double tol) {
 constexpr double kBaryTol = 1e-3;  // gjk.h:125
 
 // gjk_distance.hpp:375-486 (witness points re-evaluated from the stored
-// directions, as the reference does).
-MFCL_D bool extractNoSub(Mink& md, const Smp& s, d3& p0, d3& p1) {
-  if (s.rank == 4 || s.rank <= 0) return false;
-  if (s.rank == 1) {
-    p0 = md.support0(s.a.d);
-    p1 = md.support1(-s.a.d);
+// directions, as the reference does), on the simplex vertices v1..v_rank.
+template <class MK>
+MFCL_D bool extractNoSub(MK& md, int rank, const MV& v1, const MV& v2, const MV& v3, d3& p0, d3& p1) {
+  if (rank == 4 || rank <= 0) return false;
+  if (rank == 1) {
+    p0 = md.support0(v1.d);
+    p1 = md.support1(-v1.d);
     return true;
   }
-  if (s.rank == 2) {
-    const MV& v1 = s.a;
-    const MV& v2 = s.b;
+  if (rank == 2) {
     const d3 d = v2.v - v1.v;
     const double l2 = sqnorm(d);
     if (l2 <= 0.0) {
@@ -451,9 +481,6 @@ MFCL_D bool extractNoSub(Mink& md, const Smp& s, d3& p0, d3& p1) {
     p1 = md.support1(-v1.d) * w1 + md.support1(-v2.d) * w2;
     return true;
   }
-  const MV& v1 = s.a;
-  const MV& v2 = s.b;
-  const MV& v3 = s.c;
   const d3 s12 = v2.v - v1.v, s13 = v3.v - v1.v;
   const d3 n = cross(s12, s13);
   const double area2 = sqnorm(n);
@@ -468,9 +495,14 @@ MFCL_D bool extractNoSub(Mink& md, const Smp& s, d3& p0, d3& p1) {
   p1 = md.support1(-v1.d) * w1 + md.support1(-v2.d) * w2 + md.support1(-v3.d) * w3;
   return true;
 }
+template <class MK>
+MFCL_D bool extractNoSub(MK& md, const Smp& s, d3& p0, d3& p1) {
+  return extractNoSub(md, s.rank, s.a, s.b, s.c, p0, p1);
+}
 
 // gjk_distance.hpp:488-559
-MFCL_D bool extractTrySub(Mink& md, const Smp& s, d3& p0, d3& p1) {
+template <class MK>
+MFCL_D bool extractTrySub(MK& md, const Smp& s, d3& p0, d3& p1) {
   if (s.rank >= 4) return false;
   if (s.rank <= 2) {
     extractNoSub(md, s, p0, p1);
@@ -478,19 +510,15 @@ MFCL_D bool extractTrySub(Mink& md, const Smp& s, d3& p0, d3& p1) {
   }
   double best = -1;
   d3 bp0 = {0, 0, 0}, bp1 = {0, 0, 0}, q0, q1;
-  if (extractNoSub(md, s, q0, q1)) {
+  if (extractNoSub(md, 3, s.a, s.b, s.c, q0, q1)) {
     const double dd = sqnorm(q0 - q1);
     if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; }
   }
-  Smp c = s;
-  c.rank = 2;
-  extractNoSub(md, c, q0, q1);
+  extractNoSub(md, 2, s.a, s.b, s.b, q0, q1);
   { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
-  c = s; c.b = s.c; c.rank = 2;
-  extractNoSub(md, c, q0, q1);
+  extractNoSub(md, 2, s.a, s.c, s.c, q0, q1);
   { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
-  c = s; c.a = s.b; c.b = s.c; c.rank = 2;
-  extractNoSub(md, c, q0, q1);
+  extractNoSub(md, 2, s.b, s.c, s.c, q0, q1);
   { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
   p0 = bp0;
   p1 = bp1;
@@ -498,7 +526,8 @@ MFCL_D bool extractTrySub(Mink& md, const Smp& s, d3& p0, d3& p1) {
 }
 
 // gjk_distance.hpp:11-105
-MFCL_D bool gjkMinDistance(Mink& md, Smp& s, double tol, int max_iter, d3& p0, d3& p1, int* iters) {
+template <class MK>
+MFCL_D bool gjkMinDistance(MK& md, Smp& s, double tol, int max_iter, d3& p0, d3& p1, int* iters) {
   if (s.rank != 1) return false;
   d3 cur;
   if (distUpdate(s, cur, tol) != kUpdOK) return false;
@@ -535,7 +564,8 @@ struct GjkOut {
 };
 
 // gjk.hpp:11-140.  want_distance selects the MinSeparationDistanceOutput path.
-MFCL_D void gjkEvaluate(Mink& md, Smp& s, d3 guess, double tol, int max_iter, bool want_distance,
+template <class MK>
+MFCL_D void gjkEvaluate(MK& md, Smp& s, d3 guess, double tol, int max_iter, bool want_distance,
                         GjkOut& o) {
   o.sep_valid = false;
   o.iters = 0;
@@ -610,7 +640,8 @@ MFCL_D void gjkEvaluate(Mink& md, Smp& s, d3 guess, double tol, int max_iter, bo
 // include/fcl/cvx_collide/mpr.hpp:34-185, findPortal :188-328, updatePortal :436-495.
 enum MprStatus : int { kMprIntersect = 0, kMprSeparated = 1, kMprFailed = 2 };
 
-MFCL_D d3 mprSupport(Mink& md, d3& d) {
+template <class MK>
+MFCL_D d3 mprSupport(MK& md, d3& d) {
   d = normalized(d);
   return md.support(d);
 }
@@ -621,7 +652,8 @@ MFCL_D void swapd3(d3& a, d3& b) {
   b = t;
 }
 
-MFCL_D int mprIntersect(Mink& md, int max_iter, double tol) {
+template <class MK>
+MFCL_D int mprIntersect(MK& md, int max_iter, double tol) {
   constexpr double kDotEps = 2.220446049250313e-16;
   const d3 v0 = md.interior();
   if (sqnorm(v0) <= tol * tol) return kMprIntersect;

@@ -6,8 +6,9 @@
 //      lanes of a wavefront run loops of similar trip count;
 //   2. narrowKernel: one lane per pair -- closed forms, MPR, GJK boolean or
 //      GJK distance; pairs that need EPA append their simplex to a queue;
-//   3. epaKernel: persistent grid, one lane per queued pair, polytope in a
-//      per-lane device-memory arena.
+//   3. epaWaveKernel: persistent grid, one wavefront per queued pair, polytope
+//      in LDS; pairs that outgrow the LDS arena rerun with the full capacity
+//      in device memory.
 #include <hip/hip_runtime.h>
 
 #include "ctx.h"
@@ -50,25 +51,30 @@ MFCL_D bool closedForm(int ta, int tb) {
   return ta == kBox && tb == kBox;
 }
 
-__global__ void costKeyKernel(ShapeTables T, PairIO io, uint64_t n, Params P, uint8_t* key, uint32_t* hist) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// 4 buckets per octave of the summed vertex count; closed-form pairs first.
+MFCL_D uint8_t pairKey(const ShapeTables& T, const PairIO& io, uint64_t i, const Params& P) {
   const uint32_t a = io.pa[i], b = io.pb[i];
-  if (a >= P.n_shapes || b >= P.n_shapes) {
-    key[i] = 0;
-    atomicAdd(&hist[0], 1u);
-    return;
-  }
-  const int ta = T.type[a], tb = T.type[b];
-  const bool cf = P.mode == MFCL_MODE_COLLIDE && closedForm(ta, tb);
+  if (a >= P.n_shapes || b >= P.n_shapes) return 0;
+  const bool cf = P.mode == MFCL_MODE_COLLIDE && closedForm(T.type[a], T.type[b]);
   const int cost = shapeCost(T, a) + shapeCost(T, b);
-  int lg = 0;
-  // 4 buckets per octave of the summed vertex count
-  const float l2 = log2f(static_cast<float>(cost));
-  lg = min(63, static_cast<int>(4.0f * l2));
-  const uint8_t k = static_cast<uint8_t>((cf ? 0 : 64) + lg);
-  key[i] = k;
-  atomicAdd(&hist[k], 1u);
+  const int lg = min(63, static_cast<int>(4.0f * log2f(static_cast<float>(cost))));
+  return static_cast<uint8_t>((cf ? 0 : 64) + lg);
+}
+
+// Block-local histogram in LDS, one global atomic per bin per block.
+__global__ void __launch_bounds__(kBlock) costKeyKernel(ShapeTables T, PairIO io, uint64_t n, Params P,
+                                                        uint8_t* key, uint32_t* hist) {
+  __shared__ uint32_t h[kBuckets];
+  if (threadIdx.x < kBuckets) h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint8_t k = pairKey(T, io, i, P);
+    key[i] = k;
+    atomicAdd(&h[k], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBuckets && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void bucketScanKernel(uint32_t* hist, uint32_t* offs) {
@@ -89,11 +95,24 @@ __global__ void bucketScanKernel(uint32_t* hist, uint32_t* offs) {
   offs[t] = s[t];
 }
 
-__global__ void bucketScatterKernel(const uint8_t* key, uint64_t n, uint32_t* offs, uint32_t* order) {
+// Each block reserves its bins' ranges with one atomic per bin, then places
+// its pairs with LDS atomics (order inside a bin is irrelevant).
+__global__ void __launch_bounds__(kBlock) bucketScatterKernel(const uint8_t* key, uint64_t n, uint32_t* offs,
+                                                              uint32_t* order) {
+  __shared__ uint32_t cnt[kBuckets], base[kBuckets];
+  if (threadIdx.x < kBuckets) cnt[threadIdx.x] = 0;
+  __syncthreads();
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t pos = atomicAdd(&offs[key[i]], 1u);
-  order[pos] = static_cast<uint32_t>(i);
+  uint8_t k = 0;
+  uint32_t local = 0;
+  if (i < n) {
+    k = key[i];
+    local = atomicAdd(&cnt[k], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBuckets && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&offs[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (i < n) order[base[k] + local] = static_cast<uint32_t>(i);
 }
 
 MFCL_D void clearResult(mfcl_convex_result& r) {
@@ -113,10 +132,17 @@ MFCL_D void clearResult(mfcl_convex_result& r) {
   r.pb[0] = r.pb[1] = r.pb[2] = 0;
 }
 
-MFCL_D void put3(double* dst, d3 v) {
-  dst[0] = v.x;
-  dst[1] = v.y;
-  dst[2] = v.z;
+// Result fields gathered in registers and stored once.
+struct ResAcc {
+  d3 normal, pos, pa, pb;
+};
+MFCL_D void storeResult(mfcl_convex_result* dst, const mfcl_convex_result& r, const ResAcc& x) {
+  mfcl_convex_result o = r;
+  o.normal[0] = x.normal.x; o.normal[1] = x.normal.y; o.normal[2] = x.normal.z;
+  o.pos[0] = x.pos.x; o.pos[1] = x.pos.y; o.pos[2] = x.pos.z;
+  o.pa[0] = x.pa.x; o.pa[1] = x.pa.y; o.pa[2] = x.pa.z;
+  o.pb[0] = x.pb.x; o.pb[1] = x.pb.y; o.pb[2] = x.pb.z;
+  *dst = o;
 }
 
 MFCL_D uint16_t sat16(int v) { return static_cast<uint16_t>(v > 65535 ? 65535 : v); }
@@ -149,143 +175,157 @@ MFCL_D void loadPair(const ShapeTables& T, const double4* interior, const PairIO
   tb = poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i));
 }
 
+// One group of G lanes per pair (G divides 64); all lanes of a group run
+// the same FP64 logic on identical values, share the support scans, and
+// lane 0 of the group writes the result / queues the EPA work.
+constexpr int kModeBool = 0, kModePen = 1, kModeSd = 2;
+
+template <int G, int MODE>
 __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const double4* interior, PairIO io,
                                                        const uint32_t* order, uint64_t n, Params P,
                                                        QEntry* queue, uint32_t* qcount,
                                                        unsigned long long* scanned_total) {
-  const uint64_t slot = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t slot = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+  const int lg = static_cast<int>(threadIdx.x % G);
   int scanned = 0;
   if (slot < n) {
     const uint32_t i = order[slot];
     mfcl_convex_result r;
     clearResult(r);
-    bool done = true;
+    ResAcc acc;
+    acc.normal = acc.pos = acc.pa = acc.pb = d3{0, 0, 0};
     const bool bad_ids = io.pa[i] >= P.n_shapes || io.pb[i] >= P.n_shapes;
+    bool queued = false;
+    Smp s;
+    s.rank = -1;
     if (bad_ids) {
       r.ok = -1;  // shape id out of range
-      io.out[i] = r;
-    }
-    if (!bad_ids) {
-    DevShape A, B;
-    tf3 ta, tb;
-    uint32_t ia, ib;
-    loadPair(T, interior, io, i, A, B, ta, tb, ia, ib);
-    if (P.mode == MFCL_MODE_COLLIDE) {
-      if (P.max_contacts <= 0) {
-        // the reference warns and adds nothing (collision-inl.h:81-84)
-      } else if (closedForm(A.type, B.type)) {
-        PContacts pc;
-        pc.n = 0;
-        const bool hit = runClosedForm(A, ta, B, tb, P.penetration ? &pc : nullptr);
-        r.hit = hit;
-        r.ok = 1;
-        if (hit) {
-          if (!P.penetration) {
-            r.n_contacts = 1;
-          } else {
-            int k0 = 0;
-            if (pc.n > P.max_contacts) {
-              for (int k = 1; k < pc.n; ++k)
-                if (pc.c[k].depth > pc.c[k0].depth) k0 = k;
-            }
-            r.n_contacts = pc.n < P.max_contacts ? pc.n : P.max_contacts;
-            if (pc.n > 0) {
-              r.depth = pc.c[k0].depth;
-              r.distance = -r.depth;
-              put3(r.normal, pc.c[k0].n);
-              put3(r.pos, pc.c[k0].p);
+    } else {
+      DevShape A, B;
+      tf3 ta, tb;
+      uint32_t ia, ib;
+      loadPair(T, interior, io, i, A, B, ta, tb, ia, ib);
+      if constexpr (MODE != kModeSd) {
+        if (P.max_contacts <= 0) {
+          // the reference warns and adds nothing (collision-inl.h:81-84)
+        } else if (closedForm(A.type, B.type)) {
+          PContacts pc;
+          pc.n = 0;
+          const bool hit = runClosedForm(A, ta, B, tb, MODE == kModePen ? &pc : nullptr);
+          r.hit = hit;
+          r.ok = 1;
+          if (hit) {
+            if constexpr (MODE == kModeBool) {
+              r.n_contacts = 1;
+            } else {
+              // shape_pair_intersect-inl.h:91-113: deepest contacts first when truncated
+              int k0 = 0;
+              if (pc.n > P.max_contacts) {
+                for (int k = 1; k < pc.n; ++k)
+                  if (pc.c[k].depth > pc.c[k0].depth) k0 = k;
+              }
+              r.n_contacts = pc.n < P.max_contacts ? pc.n : P.max_contacts;
+              if (pc.n > 0) {
+                r.depth = pc.c[k0].depth;
+                r.distance = -r.depth;
+                acc.normal = pc.c[k0].n;
+                acc.pos = pc.c[k0].p;
+              }
             }
           }
+        } else {
+          MinkT<G> md;
+          setupMink(md, A, ta, B, tb, lg);
+          if (A.type == kConvex) { const double4 c = interior[ia]; md.int0 = d3{c.x, c.y, c.z}; }
+          if (B.type == kConvex) { const double4 c = interior[ib]; md.int1 = d3{c.x, c.y, c.z}; }
+          r.method = 1;
+          bool decided = false;
+          if constexpr (MODE == kModeBool) {
+            r.method = 2;
+            const int m = mprIntersect(md, P.gjk_max_it, P.gjk_tol);
+            if (m != kMprFailed) {
+              r.hit = m == kMprIntersect;
+              r.n_contacts = r.hit;
+              r.ok = 1;
+              decided = true;
+            }
+          }
+          if (!decided) {
+            d3 guess = {1, 0, 0};
+            if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
+            GjkOut g;
+            gjkEvaluate(md, s, -guess, P.gjk_tol, P.gjk_max_it, false, g);
+            r.gjk_status = static_cast<int8_t>(g.status);
+            r.gjk_iters = sat16(g.iters);
+            r.ok = 1;
+            if constexpr (MODE == kModeBool) {
+              r.hit = g.status == kGjkIntersect;
+              r.n_contacts = r.hit;
+            } else if (g.status == kGjkIntersect) {
+              queued = true;
+            }
+          }
+          scanned = md.scanned;
         }
       } else {
-        Mink md;
-        setupMink(md, A, ta, B, tb);
-        if (A.type == kConvex) { const double4 c = interior[ia]; md.int0 = d3{c.x, c.y, c.z}; }
-        if (B.type == kConvex) { const double4 c = interior[ib]; md.int1 = d3{c.x, c.y, c.z}; }
+        // signed distance: no closed forms (gjk_solver-inl.h:820-874)
+        MinkT<G> md;
+        setupMink(md, A, ta, B, tb, lg);
         r.method = 1;
-        bool decided = false;
-        if (!P.penetration) {
-          r.method = 2;
-          const int m = mprIntersect(md, P.gjk_max_it, P.gjk_tol);
-          if (m != kMprFailed) {
-            r.hit = m == kMprIntersect;
-            r.n_contacts = r.hit;
-            r.ok = 1;
-            decided = true;
-          }
-        }
-        if (!decided) {
-          d3 guess = {1, 0, 0};
-          if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
-          Smp s;
-          s.rank = -1;
-          GjkOut g;
-          gjkEvaluate(md, s, -guess, P.gjk_tol, P.gjk_max_it, false, g);
-          r.gjk_status = static_cast<int8_t>(g.status);
-          r.gjk_iters = sat16(g.iters);
+        d3 guess = {1, 0, 0};
+        if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
+        GjkOut g;
+        gjkEvaluate(md, s, -guess, P.gjk_tol, P.gjk_max_it, true, g);
+        r.gjk_status = static_cast<int8_t>(g.status);
+        r.gjk_iters = sat16(g.iters + g.dist_iters);
+        if (g.status == kGjkSeparated && g.sep_valid) {
+          acc.pa = apply(ta, g.p0);
+          acc.pb = apply(ta, g.p1);
+          r.distance = norm(g.p0 - g.p1);
           r.ok = 1;
-          if (!P.penetration) {
-            r.hit = g.status == kGjkIntersect;
-            r.n_contacts = r.hit;
-          } else if (g.status == kGjkIntersect) {
-            const uint32_t q = atomicAdd(qcount, 1u);
-            queue[q].s = s;
-            queue[q].pair = i;
-            done = false;
-          }
+        } else if (g.status == kGjkIntersect) {
+          queued = true;
+        } else {
+          r.distance = -1;
         }
         scanned = md.scanned;
       }
-    } else {
-      // signed distance: no closed forms (gjk_solver-inl.h:820-874)
-      Mink md;
-      setupMink(md, A, ta, B, tb);
-      r.method = 1;
-      d3 guess = {1, 0, 0};
-      if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
-      Smp s;
-      s.rank = -1;
-      GjkOut g;
-      gjkEvaluate(md, s, -guess, P.gjk_tol, P.gjk_max_it, true, g);
-      r.gjk_status = static_cast<int8_t>(g.status);
-      r.gjk_iters = sat16(g.iters + g.dist_iters);
-      if (g.status == kGjkSeparated && g.sep_valid) {
-        const d3 pa = apply(ta, g.p0), pb = apply(ta, g.p1);
-        put3(r.pa, pa);
-        put3(r.pb, pb);
-        r.distance = norm(g.p0 - g.p1);
-        r.ok = 1;
-      } else if (g.status == kGjkIntersect) {
+    }
+    if (lg == 0) {
+      if (queued) {
         const uint32_t q = atomicAdd(qcount, 1u);
         queue[q].s = s;
         queue[q].pair = i;
-        done = false;
-      } else {
-        r.distance = -1;
       }
-      scanned = md.scanned;
-    }
-    // pairs queued for EPA keep their GJK status here; the EPA stage
-    // completes the record
-    io.out[i] = r;
-    (void)done;
+      // queued pairs keep their GJK status here; the EPA stage completes them
+      storeResult(&io.out[i], r, acc);
     }
   }
+  if (lg != 0) scanned = 0;
   // wave-level reduction of the scan counter, one atomic per wave
   unsigned long long v = static_cast<unsigned long long>(scanned);
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(scanned_total, v);
 }
 
-__global__ void __launch_bounds__(kBlock) epaKernel(ShapeTables T, const double4* interior, PairIO io, Params P,
-                                                    const QEntry* queue, const uint32_t* qcount,
-                                                    char* arenas, size_t arena_bytes,
-                                                    unsigned long long* scanned_total) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nthreads = gridDim.x * blockDim.x;
-  const uint32_t count = *qcount;
-  const ArenaDims dims = arenaDims(P.epa_max_faces);
-  char* base = arenas + static_cast<size_t>(tid) * arena_bytes;
+// EPA stage: one wavefront per queued pair; the 64 lanes share every
+// support scan and run the polytope bookkeeping in lockstep on identical
+// values.  kLds: the polytope lives in LDS with capacity cap_faces; a pair
+// that exhausts it (status MallocFailed while cap_faces < epa_max_faces) is
+// re-queued to `overflow` and rerun by the device-memory instance with the
+// requested capacity, so every result equals a run at full capacity.
+template <bool kLds>
+__global__ void __launch_bounds__(kBlock) epaWaveKernel(ShapeTables T, PairIO io, Params P, const QEntry* queue,
+                                                        const uint32_t* qcount, int cap_faces, char* garena,
+                                                        size_t arena_bytes, QEntry* overflow, uint32_t* ocount,
+                                                        unsigned long long* scanned_total) {
+  extern __shared__ __align__(16) char lds_arena[];
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const ArenaDims dims = arenaDims(cap_faces);
+  char* base = kLds ? lds_arena + static_cast<size_t>(threadIdx.x >> 6) * arena_bytes
+                    : garena + static_cast<size_t>(wave) * arena_bytes;
   Poly poly;
   poly.V = reinterpret_cast<EVert*>(base);
   poly.E = reinterpret_cast<EEdge*>(base + sizeof(EVert) * dims.nv);
@@ -294,19 +334,28 @@ __global__ void __launch_bounds__(kBlock) epaKernel(ShapeTables T, const double4
                                         sizeof(EFace) * dims.nf);
   poly.nv = dims.nv;
   poly.nf = dims.nf;
+  const uint32_t count = *qcount;
   int scanned = 0;
-  for (uint32_t q = tid; q < count; q += nthreads) {
+  for (uint32_t q = wave; q < count; q += nwaves) {
     const QEntry e = queue[q];
     const uint32_t i = e.pair;
     DevShape A, B;
     tf3 ta, tb;
     uint32_t ia, ib;
-    loadPair(T, interior, io, i, A, B, ta, tb, ia, ib);
-    Mink md;
-    setupMink(md, A, ta, B, tb);
+    loadPair(T, nullptr, io, i, A, B, ta, tb, ia, ib);
+    MinkT<64> md;
+    setupMink(md, A, ta, B, tb, lane);
     EpaOut eo;
     epaEvaluate(poly, md, e.s, P.epa_tol, P.epa_max_it, eo);
     scanned += md.scanned;
+    if (eo.status == kEpaMallocFailed && cap_faces < P.epa_max_faces) {
+      if (lane == 0) {
+        const uint32_t o = atomicAdd(ocount, 1u);
+        overflow[o] = e;
+      }
+      continue;
+    }
+    if (lane != 0) continue;
     mfcl_convex_result& r = io.out[i];
     r.epa_status = static_cast<int8_t>(eo.status);
     r.epa_iters = sat16(eo.iters);
@@ -325,18 +374,21 @@ __global__ void __launch_bounds__(kBlock) epaKernel(ShapeTables T, const double4
       r.n_contacts = 1;
       r.depth = eo.depth;
       r.distance = -eo.depth;
-      put3(r.normal, mul(ta.R, n1));
-      put3(r.pos, apply(ta, pt));
+      const d3 nw = mul(ta.R, n1), pw = apply(ta, pt);
+      r.normal[0] = nw.x; r.normal[1] = nw.y; r.normal[2] = nw.z;
+      r.pos[0] = pw.x; r.pos[1] = pw.y; r.pos[2] = pw.z;
     } else {
-      put3(r.pa, apply(ta, eo.p0));
-      put3(r.pb, apply(ta, eo.p1));
+      const d3 wa = apply(ta, eo.p0), wb = apply(ta, eo.p1);
+      r.pa[0] = wa.x; r.pa[1] = wa.y; r.pa[2] = wa.z;
+      r.pb[0] = wb.x; r.pb[1] = wb.y; r.pb[2] = wb.z;
       r.distance = -eo.depth;
       r.depth = eo.depth;
     }
   }
+  if (lane != 0) scanned = 0;
   unsigned long long v = static_cast<unsigned long long>(scanned);
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(scanned_total, v);
+  if (lane == 0 && v) atomicAdd(scanned_total, v);
 }
 
 __global__ void zeroKernel(uint32_t* p, int n) {
@@ -344,7 +396,7 @@ __global__ void zeroKernel(uint32_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-static int g_epa_threads = 32768;
+
 
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
@@ -385,10 +437,17 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   if (int rc = ensure(ctx, ctx->ws_order, n * (sizeof(uint32_t) + 1) + 256)) return rc;
   if (int rc = ensure(ctx, ctx->ws_queue, n * sizeof(QEntry))) return rc;
   const size_t ab = arenaBytes(req->epa_max_faces);
-  if (int rc = ensure(ctx, ctx->ws_arena, ab * static_cast<size_t>(g_epa_threads))) return rc;
+  if (int rc = ensure(ctx, ctx->ws_arena, ab * static_cast<size_t>(ctx->tune.epa_overflow_waves))) return rc;
+  if (int rc = ensure(ctx, ctx->ws_oflow, n * sizeof(QEntry))) return rc;
+  if (arenaBytes(ctx->tune.epa_lds_faces) * 4 > 160 * 1024) {
+    ctx->err = "EPA LDS arena too large for a 4-wave block";
+    return MFCL_E_INVALID;
+  }
   uint32_t* hist = static_cast<uint32_t*>(ctx->ws_misc.p);
   uint32_t* offs = hist + kBuckets;
   uint32_t* qcount = offs + kBuckets;
+  uint32_t* ocount = qcount + 1;
+  QEntry* oflow = static_cast<QEntry*>(ctx->ws_oflow.p);
   unsigned long long* sc = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->ws_misc.p) + 1536);
   ctx->d_counters = reinterpret_cast<uint64_t*>(sc);
   uint32_t* order = static_cast<uint32_t*>(ctx->ws_order.p);
@@ -404,11 +463,36 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   bucketScanKernel<<<1, kBuckets, 0, s>>>(hist, offs);
   bucketScatterKernel<<<nb, kBlock, 0, s>>>(key, n, offs, order);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
-  narrowKernel<<<nb, kBlock, 0, s>>>(T, ctx->d_interior, io, order, n, P, queue, qcount, sc);
+  {
+    const int G = ctx->tune.gjk_group;
+    const unsigned gb = static_cast<unsigned>((n * G + kBlock - 1) / kBlock);
+    const int mode = P.mode == MFCL_MODE_SIGNED_DISTANCE ? kModeSd : (P.penetration ? kModePen : kModeBool);
+#define MFCL_NARROW(GG, MM) \
+  narrowKernel<GG, MM><<<gb, kBlock, 0, s>>>(T, ctx->d_interior, io, order, n, P, queue, qcount, sc)
+#define MFCL_NARROW_MODES(GG)                 \
+  if (mode == kModeSd) MFCL_NARROW(GG, kModeSd); \
+  else if (mode == kModePen) MFCL_NARROW(GG, kModePen); \
+  else MFCL_NARROW(GG, kModeBool)
+    switch (G) {
+      case 1: MFCL_NARROW_MODES(1); break;
+      case 8: MFCL_NARROW_MODES(8); break;
+      default: MFCL_NARROW_MODES(16); break;
+    }
+#undef MFCL_NARROW_MODES
+#undef MFCL_NARROW
+  }
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
-  const unsigned eb = static_cast<unsigned>(g_epa_threads / kBlock);
-  epaKernel<<<eb, kBlock, 0, s>>>(T, ctx->d_interior, io, P, queue, qcount,
-                                   static_cast<char*>(ctx->ws_arena.p), ab, sc + 1);
+  {
+    const int cap_small = P.epa_max_faces < ctx->tune.epa_lds_faces ? P.epa_max_faces : ctx->tune.epa_lds_faces;
+    const size_t lds_bytes = arenaBytes(cap_small);
+    const unsigned eb = static_cast<unsigned>(ctx->tune.epa_threads / kBlock);
+    epaWaveKernel<true><<<eb, kBlock, 4 * lds_bytes, s>>>(T, io, P, queue, qcount, cap_small, nullptr, lds_bytes,
+                                                        oflow, ocount, sc + 1);
+    // overflow pairs: full capacity in device memory, one wave per pair
+    const unsigned ob = static_cast<unsigned>(ctx->tune.epa_overflow_waves / 4);
+    epaWaveKernel<false><<<ob, kBlock, 0, s>>>(T, io, P, oflow, ocount, P.epa_max_faces,
+                                               static_cast<char*>(ctx->ws_arena.p), ab, oflow, ocount, sc + 1);
+  }
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
   MFCL_HIP_TRY(ctx, hipGetLastError());
   return MFCL_OK;

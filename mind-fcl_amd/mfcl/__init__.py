@@ -101,6 +101,7 @@ def load_library() -> C.CDLL:
     lib.mfcl_aabbtree_build.argtypes = [p, p, p, p, u32]
     lib.mfcl_aabbtree_query.argtypes = [p, p, p, u64, i32, p, p]
     lib.mfcl_aabbtree_query_dev.argtypes = [p, p, p, u64, i32, p, p, p]
+    lib.mfcl_tune.argtypes = [p, i32, i32]
     _lib = lib
     return lib
 
