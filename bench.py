@@ -101,6 +101,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start measurement")
+    ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-mesh measurement")
+    ap.add_argument("--mesh-pool", type=int, default=256)
+    ap.add_argument("--mesh-cpu-pairs", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes 3 times (the known-size "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -143,7 +147,7 @@ def main():
         def run():
             ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, req, dout)
 
-        t_group = autotune.tune(ctx, autotune.GJK_GROUP, [1, 8, 16], run, reps=2, log=log)
+        t_group = autotune.tune(ctx, autotune.GJK_GROUP, [16, 32, 64], run, reps=2, log=log)
         t_epa = autotune.tune(ctx, autotune.EPA_THREADS, [8192, 32768, 131072], run, reps=2, log=log)
         tuned = {"gjk_group": min(t_group, key=t_group.get), "epa_threads": min(t_epa, key=t_epa.get)}
     ctx.set_timing(True)
@@ -212,6 +216,9 @@ def main():
                           "epa": float(kern[:, 2].mean())},
         },
     }
+    params_path = os.path.join(ROOT, "mind-fcl_amd", "mfcl", "data", "warmstart_mlp.npz")
+    if not args.no_warm and os.path.exists(params_path):
+        result["warm_start"] = bench_warm(ctx, args, h, dpa, dpb, dqa, dqb, dout, n, out, value, params_path)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         rate, ref, info = cpu_baseline(h, min(args.cpu_pairs, n), threads)
@@ -220,8 +227,127 @@ def main():
         result["cpu_baseline"] = {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
                                   "sample": info["sample"], "sample_sign_mismatches_vs_gpu": sign_mism}
         result["speedup_vs_cpu"] = value / rate
+    del h
+    if not args.no_mesh:
+        result["mesh"] = bench_mesh(ctx, args, threads=args.cpu_threads or min(16, os.cpu_count() or 1))
     ctx.close()
     print(json.dumps(result), flush=True)
+
+
+def bench_warm(ctx, args, h, dpa, dpb, dqa, dqb, dout, n, out_plain, value_plain, params_path):
+    """The same workload with the learned warm start: MLP guess (features +
+    BF16 MFMA forward) then GJK/EPA from that guess; compared with the plain
+    results and with a tighter solve."""
+    import mfcl
+    from mfcl import warmstart as WS
+
+    params, meta = WS.load_params(params_path)
+    ctx.mlp_set_params(params)
+    dguess = ctx.alloc(n * 12)
+    req_w = mfcl.request(mfcl.MODE_SIGNED_DISTANCE, use_guess=1)
+
+    def run():
+        ctx.mlp_guess_dev(dpa, dpb, dqa, dqb, n, dguess)
+        ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, req_w, dout, guess=dguess)
+
+    for _ in range(args.warmup):
+        run()
+    ctx.sync()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+        kern.append(ctx.last_timing().copy())
+    ctx.sync()
+    el = time.perf_counter() - t0
+    kern = np.array(kern).mean(0)
+    out_w = np.zeros(n, mfcl.CONVEX_RESULT_DTYPE)
+    dout.download(out_w)
+    ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, mfcl.request(mfcl.MODE_SIGNED_DISTANCE, gjk_tolerance=1e-10,
+                                                             epa_tolerance=1e-10), dout)
+    tight = np.zeros(n, mfcl.CONVEX_RESULT_DTYPE)
+    dout.download(tight)
+    from mfcl import workloads as W
+
+    r = W.bounding_radius(h["types"], h["params"], h["vert_cnt"], h["verts"], h["vert_off"])
+    agree = WS.compare(out_plain, out_w, tight, r[h["pair_a"]] + r[h["pair_b"]])
+    v = n * args.steps / el
+    # MLP roofline: 2 * (39*128 + 128*128*2 + 128*4) = 76,544 FLOP per pair on the BF16 MFMA peak
+    mlp_flop = 76544.0 * n
+    res = {"value": v, "unit": "pairs/s", "ms_per_step": 1e3 * el / args.steps, "speedup_vs_plain": v / value_plain,
+           "kernel_ms": {"order": float(kern[0]), "gjk": float(kern[1]), "epa": float(kern[2]), "mlp": float(kern[5])},
+           "gjk_iters_mean": {"plain": float(out_plain["gjk_iters"].mean()), "warm": float(out_w["gjk_iters"].mean())},
+           "agreement": agree,
+           "mlp_roofline": {"bound": "mfma", "achieved": mlp_flop / (kern[5] * 1e-3) / 1e12, "peak": 2500.0,
+                            "unit": "TFLOP/s", "frac": mlp_flop / (kern[5] * 1e-3) / 1e12 / 2500.0},
+           "trained_on": {k: meta[k] for k in ("pairs", "seed", "epochs") if k in meta}}
+    log(f"[warm] {json.dumps(res)}")
+    dguess.free()
+    return res
+
+
+def bench_mesh(ctx, args, threads):
+    """Mesh-mesh OBB-tree collision: 4096 pairs per launch over a pool of
+    icosphere meshes (1e3..1e5 triangles, log-uniform), CollisionRequest(16)."""
+    import mfcl
+    from mfcl import workloads as W
+
+    t0 = time.time()
+    pool = W.mesh_pool(args.mesh_pool, 3)
+    pairs = W.mesh_pairs(pool, 4096, 30)
+    ctx.upload_meshes(pool["verts"], pool["vert_off"], pool["vert_cnt"], pool["tris"], pool["tri_off"],
+                      pool["tri_cnt"])
+    log(f"[mesh] pool {args.mesh_pool} meshes, {int(pool['tri_cnt'].sum())} triangles, built in "
+        f"{time.time() - t0:.1f}s")
+    n = len(pairs["pair_a"])
+    dpa, dpb = ctx.to_device(pairs["pair_a"]), ctx.to_device(pairs["pair_b"])
+    dqa, dqb = ctx.to_device(pairs["pose_a"]), ctx.to_device(pairs["pose_b"])
+    dres = ctx.alloc(n * mfcl.MESH_RESULT_DTYPE.itemsize)
+    dcon = ctx.alloc(n * 16 * 8)
+    for _ in range(2):
+        ctx.mesh_collide_dev(dpa, dpb, dqa, dqb, n, 16, dres, dcon)
+    ctx.sync()
+    km = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.mesh_collide_dev(dpa, dpb, dqa, dqb, n, 16, dres, dcon)
+        km.append(float(ctx.last_timing()[3]))
+    ctx.sync()
+    el = time.perf_counter() - t0
+    res = np.zeros(n, mfcl.MESH_RESULT_DTYPE)
+    dres.download(res)
+    bv, lf = res["bv_tests"].astype(np.float64).sum(), res["leaf_tests"].astype(np.float64).sum()
+    alg = 2 * 64 * bv + 2 * 36 * lf + 64 * n
+    kms = float(np.mean(km))
+    out = {"value": n * args.steps / el, "unit": "pairs/s", "pairs_per_launch": n, "kernel_ms": kms,
+           "hit_fraction": float(res["hit"].mean()), "bv_tests_mean": bv / n, "leaf_tests_mean": lf / n,
+           "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": alg},
+           "workload": f"{n} mesh pairs over a pool of {args.mesh_pool} noisy icospheres "
+                       "(1e3..1e5 triangles, radius U[0.1,0.5]), ~40% overlapping, first 16 contacts"}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+
+        M = pyoracle.MeshSet(pool["verts"], pool["vert_off"], pool["vert_cnt"], pool["tris"], pool["tri_off"],
+                             pool["tri_cnt"], threads=threads)
+        m = min(args.mesh_cpu_pairs, n)
+        ta, tb = pyoracle.pose12(pairs["pose_a"][:m]), pyoracle.pose12(pairs["pose_b"][:m])
+        ref, secs = M.collide(pairs["pair_a"][:m], pairs["pair_b"][:m], ta, tb, 16, threads=threads)
+        reps = max(1, int(np.ceil(10.0 / max(1e-6, secs * threads))))
+        tot = 0.0
+        for _ in range(reps):
+            _, s = M.collide(pairs["pair_a"][:m], pairs["pair_b"][:m], ta, tb, 16, threads=threads)
+            tot += s
+        rate = reps * m / tot
+        out["cpu_baseline"] = {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} pairs, {reps} timed passes ({tot:.2f}s wall)",
+                               "sample_bool_mismatches_vs_gpu": int(np.sum(ref["hit"] != res["hit"][:m]))}
+        out["speedup_vs_cpu"] = out["value"] / rate
+    log(f"[mesh] {json.dumps(out)}")
+    for a in (dpa, dpb, dqa, dqb, dres, dcon):
+        a.free()
+    return out
 
 
 if __name__ == "__main__":

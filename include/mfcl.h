@@ -118,7 +118,7 @@ int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4);
 
 /* Launch-shape knobs (no counterpart in the reference: GPU scheduling
  * parameters, chosen by the autotuner in mind-fcl_amd/mfcl/autotune.py). */
-#define MFCL_TUNE_GJK_GROUP 1         /* lanes per pair in the GJK stage: 1, 8, 16 */
+#define MFCL_TUNE_GJK_GROUP 1         /* lanes per pair in the GJK stage: 16, 32, 64 */
 #define MFCL_TUNE_EPA_THREADS 2       /* persistent EPA lanes (64 per pair), multiple of 256 */
 #define MFCL_TUNE_MESH_BLOCKS_PER_CU 3 /* persistent traversal blocks (4 wavefronts) per CU, 1..16 */
 #define MFCL_TUNE_MESH_STACK 4        /* node-pair stack entries per wavefront, 1024..262144 */
@@ -191,6 +191,48 @@ int mfcl_aabbtree_query(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uin
                         int32_t cap, int32_t* counts, int64_t* ids);
 int mfcl_aabbtree_query_dev(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uint64_t nq,
                             int32_t cap, int32_t* counts, int64_t* ids, void* stream);
+
+/* ---- learned GJK warm start --------------------------------------------- */
+/* No counterpart in the reference: a 39-128-128-128-4 GELU MLP whose output
+ * direction is passed as the per-pair GJK guess (the reference's hook
+ * GJKSolver::gjk_guess / enable_cached_guess, include/fcl/narrowphase/
+ * detail/gjk_solver.h:140-163, consumed at gjk_solver-inl.h:77-78,103).
+ * Inputs per pair: relative pose of b in a's frame (translation 3,
+ * quaternion 4) and a 16-float descriptor of each shape (computed at
+ * mfcl_upload_shapes); feature rows are 64 floats (39 used).  Outputs per
+ * pair: 4 floats = unnormalised direction in a's frame (3) + signed
+ * distance estimate (1).  BF16 MFMA products with FP32 accumulation; FP32
+ * master parameters trained with Adam. */
+int mfcl_mlp_num_params(void);
+/* Xavier-uniform initialisation (std::mt19937_64 seeded) and Adam reset. */
+int mfcl_mlp_init(mfcl_ctx* ctx, uint64_t seed);
+/* Host FP32 parameter vector of mfcl_mlp_num_params() floats (layout:
+ * W1[128][64], b1[128], W2[128][128], b2, W3[128][128], b3, W4[32][128],
+ * b4[32]; padding entries zero).  set_params also resets Adam. */
+int mfcl_mlp_set_params(mfcl_ctx* ctx, const float* params, int32_t n);
+/* Copies parameters and/or the gradients of the last training step to host
+ * (either pointer may be NULL). */
+int mfcl_mlp_get_params(mfcl_ctx* ctx, float* params, float* grads, int32_t n);
+/* Feature rows (n x 64 FP32, device) for pairs of uploaded shapes. */
+int mfcl_mlp_features_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                          const float* pose_a7, const float* pose_b7, uint64_t n, float* feat64,
+                          void* stream);
+/* Features + forward pass + normalisation: unit guess directions (n x 3,
+ * a's frame) ready for mfcl_convex_query_dev(..., guess3, ...) with
+ * req->use_guess = 1, and optional distance estimates (n, may be NULL). */
+int mfcl_mlp_guess_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                       const float* pose_a7, const float* pose_b7, uint64_t n, float* guess3,
+                       float* dist_estimate, void* stream);
+/* Forward pass on given feature rows (device): out4 is n x 4. */
+int mfcl_mlp_forward_dev(mfcl_ctx* ctx, const float* feat64, uint64_t n, float* out4, void* stream);
+/* One training step on a batch (device feature rows, targets n x 4 =
+ * unit direction + signed distance; rows with a zero direction are
+ * ignored).  hyper6 = {lr, beta1, beta2, eps, unused, distance weight} or
+ * NULL for {1e-3, 0.9, 0.999, 1e-8, 0, 1}.  apply_update = 0 computes the
+ * loss and gradients only (read them with mfcl_mlp_get_params).
+ * loss_host (may be NULL) receives the mean loss after synchronising. */
+int mfcl_mlp_train_step_dev(mfcl_ctx* ctx, const float* feat64, const float* target4, uint64_t n,
+                            const float* hyper6, int32_t apply_update, float* loss_host, void* stream);
 
 #ifdef __cplusplus
 }

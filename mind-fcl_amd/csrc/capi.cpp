@@ -45,6 +45,7 @@ void freeShapes(mfcl_ctx* c) {
   freePtr(reinterpret_cast<void*&>(c->d_vcnt));
   freePtr(reinterpret_cast<void*&>(c->d_verts));
   freePtr(reinterpret_cast<void*&>(c->d_interior));
+  freePtr(reinterpret_cast<void*&>(c->d_desc));
   c->n_shapes = 0;
 }
 
@@ -69,6 +70,7 @@ void freeAll(mfcl_ctx* c) {
   freePtr(c->ws_misc.p);
   freePtr(c->ws_io.p);
   freePtr(c->ws_oflow.p);
+  mlpFree(c);
 }
 
 // Host builders (bvh_build.cpp, broadphase.hip).
@@ -121,6 +123,9 @@ int mfcl_create(mfcl_ctx** out, int device) {
       delete c;
       return MFCL_E_HIP;
     }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->max_lds_per_block = lds;
   *out = c;
   return MFCL_OK;
 }
@@ -177,7 +182,7 @@ int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
   if (!ctx) return MFCL_E_INVALID;
   switch (knob) {
     case MFCL_TUNE_GJK_GROUP:
-      if (value != 1 && value != 8 && value != 16) break;
+      if (value != 16 && value != 32 && value != 64) break;
       ctx->tune.gjk_group = value;
       return MFCL_OK;
     case MFCL_TUNE_EPA_THREADS:
@@ -226,6 +231,47 @@ int mfcl_d2h(mfcl_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   return MFCL_OK;
 }
 
+// 16-float shape descriptor for the warm-start MLP: local half-extents (3),
+// vertex mean (3), vertex second moments about the mean xx yy zz xy xz yz
+// (6), log2(vertex count) / 9, max |v|, mean |v|, and 1 for a hull (0 for a
+// primitive, whose extents and moments are those of its solid box).
+static void shapeDescriptor(int type, const float* p, const float* xyz, uint32_t n, float* d) {
+  for (int k = 0; k < 16; ++k) d[k] = 0.f;
+  if (type == MFCL_SHAPE_CONVEX) {
+    double mx[3] = {0, 0, 0}, mean[3] = {0, 0, 0}, rmax = 0, rsum = 0;
+    for (uint32_t k = 0; k < n; ++k)
+      for (int a = 0; a < 3; ++a) {
+        mx[a] = std::max(mx[a], static_cast<double>(std::fabs(xyz[3 * k + a])));
+        mean[a] += xyz[3 * k + a];
+      }
+    for (int a = 0; a < 3; ++a) mean[a] /= n;
+    double m2[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t k = 0; k < n; ++k) {
+      const double px = xyz[3 * k], py = xyz[3 * k + 1], pz = xyz[3 * k + 2];
+      const double x = px - mean[0], y = py - mean[1], z = pz - mean[2];
+      m2[0] += x * x; m2[1] += y * y; m2[2] += z * z; m2[3] += x * y; m2[4] += x * z; m2[5] += y * z;
+      const double r = std::sqrt(px * px + py * py + pz * pz);
+      rmax = std::max(rmax, r);
+      rsum += r;
+    }
+    for (int a = 0; a < 3; ++a) { d[a] = static_cast<float>(mx[a]); d[3 + a] = static_cast<float>(mean[a]); }
+    for (int a = 0; a < 6; ++a) d[6 + a] = static_cast<float>(m2[a] / n);
+    d[12] = static_cast<float>(std::log2(static_cast<double>(n)) / 9.0);
+    d[13] = static_cast<float>(rmax);
+    d[14] = static_cast<float>(rsum / n);
+    d[15] = 1.f;
+    return;
+  }
+  double h[3] = {0, 0, 0};
+  if (type == MFCL_SHAPE_BOX) { h[0] = p[0] / 2; h[1] = p[1] / 2; h[2] = p[2] / 2; }
+  else if (type == MFCL_SHAPE_SPHERE) { h[0] = h[1] = h[2] = p[0]; }
+  else if (type == MFCL_SHAPE_CAPSULE) { h[0] = h[1] = p[0]; h[2] = p[0] + p[1] / 2; }
+  else { h[0] = h[1] = p[0]; h[2] = p[1] / 2; }
+  for (int a = 0; a < 3; ++a) { d[a] = static_cast<float>(h[a]); d[6 + a] = static_cast<float>(h[a] * h[a] / 3); }
+  d[13] = static_cast<float>(std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]));
+  d[14] = d[13];
+}
+
 int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4, const uint64_t* vert_off,
                        const uint32_t* vert_cnt, uint32_t n_shapes, const float* verts_xyz,
                        uint64_t n_verts) {
@@ -236,6 +282,7 @@ int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
   std::vector<float4> param(n_shapes);
   std::vector<uint32_t> voff(n_shapes, 0), vcnt(n_shapes, 0);
   std::vector<double4> interior(n_shapes, double4{0, 0, 0, 0});
+  std::vector<float> desc(16 * static_cast<size_t>(n_shapes), 0.f);
   std::vector<float4> verts(std::max<uint64_t>(n_verts, 1));
   for (uint64_t k = 0; k < n_verts; ++k)
     verts[k] = float4{verts_xyz[3 * k], verts_xyz[3 * k + 1], verts_xyz[3 * k + 2], 0.f};
@@ -264,10 +311,13 @@ int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
       const double inv = 1.0 / vcnt[i];
       interior[i] = double4{sx * inv, sy * inv, sz * inv, 0};
       param[i].w = rb;
+      shapeDescriptor(t, params4 + 4 * i, verts_xyz + 3 * vert_off[i], vcnt[i], &desc[16 * static_cast<size_t>(i)]);
     } else if (t != MFCL_SHAPE_BOX && t != MFCL_SHAPE_SPHERE && t != MFCL_SHAPE_CAPSULE &&
                t != MFCL_SHAPE_CYLINDER) {
       ctx->err = "unsupported shape type " + std::to_string(t) + " for shape " + std::to_string(i);
       return MFCL_E_UNSUPPORTED;
+    } else {
+      shapeDescriptor(t, params4 + 4 * i, nullptr, 0, &desc[16 * static_cast<size_t>(i)]);
     }
   }
   int rc;
@@ -277,6 +327,7 @@ int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
   if ((rc = upload(ctx, &ctx->d_vcnt, vcnt.data(), n_shapes))) return rc;
   if ((rc = upload(ctx, &ctx->d_verts, verts.data(), verts.size()))) return rc;
   if ((rc = upload(ctx, &ctx->d_interior, interior.data(), n_shapes))) return rc;
+  if ((rc = upload(ctx, &ctx->d_desc, desc.data(), desc.size()))) return rc;
   ctx->h_vcnt = vcnt;
   ctx->n_shapes = n_shapes;
   return MFCL_OK;
@@ -476,4 +527,60 @@ int mfcl_aabbtree_query(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uin
   return MFCL_OK;
 }
 
+// ---- warm-start MLP ---------------------------------------------------------
+int mfcl_mlp_init(mfcl_ctx* ctx, uint64_t seed) {
+  if (!ctx) return MFCL_E_INVALID;
+  return mlpInit(ctx, seed);
+}
+
+int mfcl_mlp_num_params(void) { return kMlpParams; }
+
+int mfcl_mlp_set_params(mfcl_ctx* ctx, const float* params, int32_t n) {
+  if (!ctx || !params) return MFCL_E_INVALID;
+  return mlpSetParams(ctx, params, n, true);
+}
+
+int mfcl_mlp_get_params(mfcl_ctx* ctx, float* params, float* grads, int32_t n) {
+  if (!ctx) return MFCL_E_INVALID;
+  return mlpGetParams(ctx, params, n, grads);
+}
+
+int mfcl_mlp_features_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b, const float* pose_a7,
+                          const float* pose_b7, uint64_t n, float* feat64, void* stream) {
+  if (!ctx || (n && (!pair_a || !pair_b || !pose_a7 || !pose_b7 || !feat64))) return MFCL_E_INVALID;
+  return mlpFeatures(ctx, pair_a, pair_b, pose_a7, pose_b7, n, feat64,
+                     stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+int mfcl_mlp_guess_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b, const float* pose_a7,
+                       const float* pose_b7, uint64_t n, float* guess3, float* dist_estimate, void* stream) {
+  if (!ctx || (n && (!pair_a || !pair_b || !pose_a7 || !pose_b7 || !guess3))) return MFCL_E_INVALID;
+  const int rc = mlpGuess(ctx, pair_a, pair_b, pose_a7, pose_b7, n, guess3, dist_estimate,
+                          stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+  if (rc) return rc;
+  if (ctx->timing && n) {
+    MFCL_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[9]));
+    float ms = 0;
+    MFCL_HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
+    ctx->last_ms[5] = ms;
+  }
+  return MFCL_OK;
+}
+
+int mfcl_mlp_forward_dev(mfcl_ctx* ctx, const float* feat64, uint64_t n, float* out4, void* stream) {
+  if (!ctx || (n && (!feat64 || !out4))) return MFCL_E_INVALID;
+  return mlpForwardFeat(ctx, feat64, n, out4, stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+int mfcl_mlp_train_step_dev(mfcl_ctx* ctx, const float* feat64, const float* target4, uint64_t n,
+                            const float* hyper6, int32_t apply_update, float* loss_host, void* stream) {
+  if (!ctx || !feat64 || !target4) return MFCL_E_INVALID;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  const int rc = mlpTrainStep(ctx, feat64, target4, n, hyper6, apply_update, loss_host, s);
+  if (rc) return rc;
+  if (loss_host) MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return MFCL_OK;
+}
+
 }  // extern "C"
+

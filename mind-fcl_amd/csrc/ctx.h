@@ -25,6 +25,32 @@ struct DevBuf {
   size_t bytes{0};
 };
 
+// Warm-start MLP: 39 -> 128 -> 128 -> 128 -> 4 (inputs padded to 64,
+// outputs to 32).  FP32 master parameters, packed in this order.
+constexpr int kMlpIn = 39, kMlpInPad = 64, kHid = 128, kMlpOut = 4, kMlpOutPad = 32;
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kHid * kMlpInPad;
+constexpr int kOffW2 = kOffB1 + kHid;
+constexpr int kOffB2 = kOffW2 + kHid * kHid;
+constexpr int kOffW3 = kOffB2 + kHid;
+constexpr int kOffB3 = kOffW3 + kHid * kHid;
+constexpr int kOffW4 = kOffB3 + kHid;
+constexpr int kOffB4 = kOffW4 + kMlpOutPad * kHid;
+constexpr int kMlpParams = kOffB4 + kMlpOutPad;
+
+struct MlpState {
+  float* params{nullptr};  // FP32 master weights
+  float* grad{nullptr};
+  float* m{nullptr};
+  float* v{nullptr};
+  float* loss{nullptr};
+  void* wbuf{nullptr};     // BF16 copies of W (forward) and W^T (backward)
+  __bf16 *w1{nullptr}, *w2{nullptr}, *w3{nullptr}, *w4{nullptr};
+  __bf16 *w1t{nullptr}, *w2t{nullptr}, *w3t{nullptr}, *w4t{nullptr};
+  DevBuf ws;               // activations
+  int step{0};
+};
+
 struct MeshHost {
   std::vector<uint32_t> node_base;  // first node of each mesh
   std::vector<uint32_t> node_count;
@@ -34,12 +60,13 @@ struct MeshHost {
 
 struct mfcl_ctx {
   int device{0};
+  int max_lds_per_block{65536};  // device limit on dynamic + static LDS per workgroup
   // launch-shape knobs (set by mfcl_tune / the autotuner)
   struct {
-    int gjk_group{16};          // lanes per pair in the GJK stage (1, 8, 16)
+    int gjk_group{32};          // lanes per pair in the GJK stage (16, 32, 64)
     int epa_threads{32768};     // persistent EPA lanes (multiple of 256): 64 per wavefront
     int epa_lds_faces{64};      // face capacity of the LDS polytope (overflow reruns at full size)
-    int epa_overflow_waves{4096};  // persistent wavefronts for overflow pairs
+    int epa_overflow_waves{512};   // persistent wavefronts (one per block) for overflow pairs
     int mesh_blocks_per_cu{4};  // persistent mesh-traversal blocks (4 waves) per CU
     int mesh_stack{16384};      // node-pair stack entries per wavefront
   } tune;
@@ -54,6 +81,7 @@ struct mfcl_ctx {
   uint32_t* d_vcnt{nullptr};
   float4* d_verts{nullptr};
   double4* d_interior{nullptr};
+  float* d_desc{nullptr};  // 16-float shape descriptors (warm-start MLP inputs)
   std::vector<uint32_t> h_vcnt;
 
   // mesh set
@@ -72,6 +100,8 @@ struct mfcl_ctx {
 
   // growable device workspaces, keyed by purpose
   mfcl::DevBuf ws_order, ws_queue, ws_arena, ws_misc, ws_io, ws_oflow;
+
+  mfcl::MlpState mlp;
 
   // timing
   bool timing{false};
@@ -95,5 +125,18 @@ int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
                       int32_t* contacts, hipStream_t s);
 int aabbQueryLaunch(mfcl_ctx* ctx, const float* qmn, const float* qmx, uint64_t nq, int32_t cap,
                     int32_t* counts, int64_t* ids, hipStream_t s);
+
+// Warm-start MLP (mlp.hip).
+int mlpInit(mfcl_ctx* ctx, uint64_t seed);
+int mlpSetParams(mfcl_ctx* ctx, const float* host, int n, bool reset_opt);
+int mlpGetParams(mfcl_ctx* ctx, float* host, int n, float* grad_host);
+int mlpFeatures(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
+                uint64_t n, float* xf_out, hipStream_t s);
+int mlpGuess(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
+             uint64_t n, float* guess3, float* dist, hipStream_t s);
+int mlpForwardFeat(mfcl_ctx* ctx, const float* xf, uint64_t n, float* y4_out, hipStream_t s);
+int mlpTrainStep(mfcl_ctx* ctx, const float* xf, const float* target4, uint64_t n, const float* hyper,
+                 int apply_update, float* loss_out, hipStream_t s);
+void mlpFree(mfcl_ctx* ctx);
 
 }  // namespace mfcl

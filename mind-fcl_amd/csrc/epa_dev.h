@@ -1,11 +1,10 @@
 // Device EPA (include/fcl/cvx_collide/epa.hpp, epa_polytope*.hpp,
 // epa_simplex2polytope.hpp, epa_polytope_utils.h).
 //
-// GPU representation: each worker lane owns an arena in device memory that
-// holds the polytope as three index-linked element lists (newest element
-// first) plus free lists, 16-bit indices, FP64 geometry.  Scans visit the
-// elements in the reference's list order, so ties resolve identically.
-// The arena also carries the LIFO stack of the visibility flood fill.
+// GPU representation: one wavefront per pair; the polytope lives in that
+// wavefront's LDS slice as slot arrays of vertices, edges and faces (16-bit
+// indices, FP64 geometry) that the 64 lanes scan in parallel; creation
+// stamps stand in for the reference's newest-first element lists.
 #pragma once
 
 #include "narrow_dev.h"
@@ -17,37 +16,38 @@ constexpr int16_t kNil = -1;
 struct EVert {
   double vx, vy, vz;  // vertex on the Minkowski difference
   double dx, dy, dz;  // direction that produced it
-  int16_t next, mnext, new_edge;
+  int32_t stamp;      // creation order; < 0: free slot
+  int16_t new_edge;
   uint8_t remove, pad;
 };
 struct EEdge {
   double d2, wx, wy, wz;  // squared distance to origin, witness
-  int16_t v0, v1, f0, f1, next, mnext;
+  int32_t stamp;
+  int16_t v0, v1, f0, f1;
   uint8_t in, vis;
   uint16_t pad;
 };
 struct EFace {
   double d2, wx, wy, wz;
-  int16_t e0, e1, e2, next, mnext;
+  int32_t stamp;
+  int16_t e0, e1, e2;
   uint8_t in, vis;
-  uint32_t pad;
 };
 
 // Arena sizing for a given face capacity: vertices = edges = floor(1.51 F)
-// (epa_polytope.hpp:90-95), stack = 3F + 3 entries.
+// (epa_polytope.hpp:90-95).
 struct ArenaDims {
-  int nv, nf, nstack;
+  int nv, nf;
 };
 __host__ __device__ inline ArenaDims arenaDims(int max_faces) {
   ArenaDims d;
   d.nv = static_cast<int>(1.51 * static_cast<double>(max_faces));
   d.nf = max_faces;
-  d.nstack = 3 * max_faces + 3;
   return d;
 }
 __host__ __device__ inline size_t arenaBytes(int max_faces) {
   const ArenaDims d = arenaDims(max_faces);
-  size_t b = sizeof(EVert) * d.nv + sizeof(EEdge) * d.nv + sizeof(EFace) * d.nf + sizeof(int32_t) * d.nstack;
+  size_t b = sizeof(EVert) * d.nv + sizeof(EEdge) * d.nv + sizeof(EFace) * d.nf;
   return (b + 255) & ~static_cast<size_t>(255);
 }
 
@@ -129,100 +129,156 @@ MFCL_D void triDist0(d3 a, d3 b, d3 c, MinD& o) {
   o.w = best.w;
 }
 
+// The polytope, processed by the 64 lanes of one wavefront together.
+// Elements live in fixed slot arrays (slot s is visited by lane s % 64);
+// a slot is free when its stamp is negative.  Every element gets a creation
+// stamp from one counter, and the reference's element lists are
+// newest-first (epa_polytope.hpp:127-290), so "earlier in the list" is
+// "larger stamp": scans that the reference runs sequentially over its lists
+// run here as parallel passes plus wavefront reductions ordered by stamp,
+// which reproduces its tie-breaking exactly.  Slot numbers themselves never
+// influence a result.  Every member function must be called by all 64 lanes
+// with identical arguments.
 struct Poly {
-  EVert* V;
-  EEdge* E;
-  EFace* F;
-  int32_t* stk;
-  int nv, nf;
-  int16_t vl, el, fl, vfree, efree, ffree;
+  MFCL_LDS EVert* V;
+  MFCL_LDS EEdge* E;
+  MFCL_LDS EFace* F;
+  int nv, nf;   // slot capacities: vertices and edges nv, faces nf
+  int stamp;    // creation counter
 
   MFCL_D d3 vloc(int16_t i) const { return {V[i].vx, V[i].vy, V[i].vz}; }
   MFCL_D d3 vdir(int16_t i) const { return {V[i].dx, V[i].dy, V[i].dz}; }
 
-  // epa_polytope.hpp:77-125
+  // epa_polytope.hpp:77-125: everything free
   MFCL_D void reset() {
-    vl = el = fl = kNil;
-    vfree = efree = ffree = kNil;
-    for (int i = nv - 1; i >= 0; --i) { V[i].mnext = vfree; vfree = static_cast<int16_t>(i); }
-    for (int i = nv - 1; i >= 0; --i) { E[i].mnext = efree; efree = static_cast<int16_t>(i); }
-    for (int i = nf - 1; i >= 0; --i) { F[i].mnext = ffree; ffree = static_cast<int16_t>(i); }
+    const int l = __lane_id();
+    for (int s = l; s < nv; s += 64) {
+      V[s].stamp = -1;
+      E[s].stamp = -1;
+    }
+    for (int s = l; s < nf; s += 64) F[s].stamp = -1;
+    stamp = 0;
+    __builtin_amdgcn_wave_barrier();
+  }
+  // lowest free slot (the reference pops its free list; which slot is used
+  // does not matter, only the capacity does)
+  template <class R>
+  MFCL_D int16_t allocSlot(MFCL_LDS R* arr, int cap) const {
+    const int l = __lane_id();
+    for (int base = 0; base < cap; base += 64) {
+      const int s = base + l;
+      const unsigned long long m = __ballot(s < cap && arr[s].stamp < 0);
+      if (m) return static_cast<int16_t>(base + __ffsll(m) - 1);
+    }
+    return kNil;
   }
   MFCL_D int16_t addVertex(d3 v, d3 d) {
-    if (vfree == kNil) return kNil;
-    const int16_t i = vfree;
-    EVert& x = V[i];
-    vfree = x.mnext;
+    const int16_t i = allocSlot(V, nv);
+    if (i == kNil) return kNil;
+    MFCL_LDS EVert& x = V[i];
     x.vx = v.x; x.vy = v.y; x.vz = v.z;
     x.dx = d.x; x.dy = d.y; x.dz = d.z;
-    x.next = vl;
-    vl = i;
+    x.stamp = stamp++;
+    __builtin_amdgcn_wave_barrier();
     return i;
   }
   MFCL_D int16_t addEdge(int16_t a, int16_t b) {
-    if (a == kNil || b == kNil || efree == kNil) return kNil;
-    const int16_t i = efree;
-    EEdge& e = E[i];
-    efree = e.mnext;
+    if (a == kNil || b == kNil) return kNil;
+    const int16_t i = allocSlot(E, nv);
+    if (i == kNil) return kNil;
+    MFCL_LDS EEdge& e = E[i];
     e.v0 = a;
     e.v1 = b;
     e.f0 = e.f1 = kNil;
+    e.vis = kUnknown;
     MinD m;
     segDist(d3{0, 0, 0}, vloc(a), vloc(b), m);
     e.in = m.in; e.d2 = m.d2; e.wx = m.w.x; e.wy = m.w.y; e.wz = m.w.z;
-    e.next = el;
-    el = i;
+    e.stamp = stamp++;
+    __builtin_amdgcn_wave_barrier();
     return i;
   }
   MFCL_D int16_t addFace(int16_t e1, int16_t e2, int16_t e3) {
-    if (e1 == kNil || e2 == kNil || e3 == kNil || ffree == kNil) return kNil;
-    const int16_t i = ffree;
-    EFace& f = F[i];
-    ffree = f.mnext;
+    if (e1 == kNil || e2 == kNil || e3 == kNil) return kNil;
+    const int16_t i = allocSlot(F, nf);
+    if (i == kNil) return kNil;
+    MFCL_LDS EFace& f = F[i];
     f.e0 = e1; f.e1 = e2; f.e2 = e3;
+    f.vis = kFUnknown;
     const int16_t a = E[e1].v0, b = E[e1].v1;
     const int16_t c = (E[e2].v0 != a && E[e2].v0 != b) ? E[e2].v0 : E[e2].v1;
     MinD m;
     triDist0(vloc(a), vloc(b), vloc(c), m);
     f.in = m.in; f.d2 = m.d2; f.wx = m.w.x; f.wy = m.w.y; f.wz = m.w.z;
-    f.next = fl;
-    fl = i;
+    f.stamp = stamp++;
+    __builtin_amdgcn_wave_barrier();
+    // the face stays in the polytope even when an edge already has two faces
     const int16_t es[3] = {e1, e2, e3};
     for (int k = 0; k < 3; ++k) {
-      EEdge& ek = E[es[k]];
+      MFCL_LDS EEdge& ek = E[es[k]];
       if (ek.f0 == kNil) {
         ek.f0 = i;
       } else {
         if (ek.f1 != kNil) return kNil;
         ek.f1 = i;
       }
+      __builtin_amdgcn_wave_barrier();
     }
     return i;
   }
   MFCL_D void faceVerts(int16_t fi, int16_t& a, int16_t& b, int16_t& c) const {
-    const EFace& f = F[fi];
+    const MFCL_LDS EFace& f = F[fi];
     a = E[f.e0].v0;
     b = E[f.e0].v1;
     c = (E[f.e1].v0 != a && E[f.e1].v0 != b) ? E[f.e1].v0 : E[f.e1].v1;
   }
+
+  // Wavefront argmin of (d2, kind, newest first) -- the first element of
+  // the reference's vertex, edge, face scans with strict '<'.
+  MFCL_D static bool better(double d2, int kind, int st, double bd, int bk, int bs) {
+    if (d2 < bd) return true;
+    if (!(d2 == bd)) return false;
+    if (kind != bk) return kind < bk;
+    return st > bs;
+  }
   // epa_polytope.hpp:292-345; kind: 0 none, 1 vertex, 2 edge, 3 face
   MFCL_D void minToOrigin(bool excl_vertex, int& kind, int16_t& idx) const {
-    kind = 0;
-    idx = kNil;
-    if (vl == kNil) return;
-    double best = INFINITY;
+    const int l = __lane_id();
+    double bd = INFINITY;
+    int bk = 4, bs = -1, bi = kNil;
     if (!excl_vertex)
-      for (int16_t i = vl; i != kNil; i = V[i].next) {
-        const double d2 = sqnorm(vloc(i));
-        if (d2 < best) { best = d2; kind = 1; idx = i; }
+      for (int s = l; s < nv; s += 64) {
+        const int st = V[s].stamp;
+        if (st < 0) continue;
+        const double d2 = sqnorm(vloc(static_cast<int16_t>(s)));
+        if (better(d2, 1, st, bd, bk, bs)) { bd = d2; bk = 1; bs = st; bi = s; }
       }
-    for (int16_t i = el; i != kNil; i = E[i].next) {
-      if (!excl_vertex && !E[i].in) continue;
-      if (E[i].d2 < best) { best = E[i].d2; kind = 2; idx = i; }
+    for (int s = l; s < nv; s += 64) {
+      const MFCL_LDS EEdge& e = E[s];
+      const int st = e.stamp;
+      if (st < 0) continue;
+      if (!excl_vertex && !e.in) continue;
+      if (better(e.d2, 2, st, bd, bk, bs)) { bd = e.d2; bk = 2; bs = st; bi = s; }
     }
-    for (int16_t i = fl; i != kNil; i = F[i].next) {
-      if (!F[i].in) continue;
-      if (F[i].d2 < best) { best = F[i].d2; kind = 3; idx = i; }
+    for (int s = l; s < nf; s += 64) {
+      const MFCL_LDS EFace& f = F[s];
+      const int st = f.stamp;
+      if (st < 0 || !f.in) continue;
+      if (better(f.d2, 3, st, bd, bk, bs)) { bd = f.d2; bk = 3; bs = st; bi = s; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double od = __shfl_xor(bd, off);
+      const int ok = __shfl_xor(bk, off), os = __shfl_xor(bs, off), oi = __shfl_xor(bi, off);
+      if (better(od, ok, os, bd, bk, bs)) { bd = od; bk = ok; bs = os; bi = oi; }
+    }
+    // only finite candidates beat the reference's initial INFINITY
+    if (bk == 4 || !(bd < INFINITY)) {
+      kind = 0;
+      idx = kNil;
+    } else {
+      kind = bk;
+      idx = static_cast<int16_t>(bi);
     }
   }
   // epa_polytope.hpp:347-405
@@ -240,11 +296,19 @@ struct Poly {
       normal = od > 0 ? dir : -dir;
       return true;
     }
+    // max / min over the vertices: order-free, so a parallel pass
     double maxp = 0.0, minn = 0.0;
-    for (int16_t i = vl; i != kNil; i = V[i].next) {
-      const double dv = dot(vloc(i), dir);
+    for (int s = __lane_id(); s < nv; s += 64) {
+      if (V[s].stamp < 0) continue;
+      const double dv = dot(vloc(static_cast<int16_t>(s)), dir);
       if (dv > maxp) maxp = dv;
       if (dv < minn) minn = dv;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double om = __shfl_xor(maxp, off), on = __shfl_xor(minn, off);
+      if (om > maxp) maxp = om;
+      if (on < minn) minn = on;
     }
     normal = fabs(maxp) > fabs(minn) ? -dir : dir;
     return true;
@@ -257,107 +321,163 @@ struct Poly {
     if (!faceNormalOut(fi, n, &area)) return area <= 0.0;
     return dot(n, p - faceAnchor(fi)) >= 0.0;
   }
+  MFCL_D bool anyLane(bool b) const { return __ballot(b) != 0ull; }
 
-  // epa_polytope_expand.hpp:32-281. 0 ok, 1 failed, 2 malloc failed
-  __device__ __attribute__((noinline)) int expand(d3 nv, d3 nd, int16_t start) {
-    for (int16_t i = vl; i != kNil; i = V[i].next) { V[i].remove = 1; V[i].new_edge = kNil; }
-    for (int16_t i = el; i != kNil; i = E[i].next) E[i].vis = kUnknown;
-    for (int16_t i = fl; i != kNil; i = F[i].next) F[i].vis = kFUnknown;
-    int sp = 0;
+  // epa_polytope_expand.hpp:32-281. 0 ok, 1 failed, 2 malloc failed.
+  // The reference's depth-first flood fill marks the faces reachable from
+  // `start` through faces that see nv; that set, and the border / internal
+  // edge marks, do not depend on the visiting order, so it runs here as
+  // rounds of a parallel breadth-first fill.
+  MFCL_D int expand(d3 nv_, d3 nd, int16_t start) {
+    const int l = __lane_id();
+    for (int s = l; s < nv; s += 64) {
+      V[s].remove = 1;
+      V[s].new_edge = kNil;
+      E[s].vis = kUnknown;
+    }
+    for (int s = l; s < nf; s += 64) F[s].vis = kFUnknown;
+    __builtin_amdgcn_wave_barrier();
     F[start].vis = kVisible;
-    for (int k = 0; k < 3; ++k) stk[sp++] = (static_cast<int32_t>(start) << 2) | k;
-    while (sp > 0) {
-      const int32_t t = stk[--sp];
-      const int16_t fi = static_cast<int16_t>(t >> 2);
-      const int k = t & 3;
-      const int16_t ei = k == 0 ? F[fi].e0 : (k == 1 ? F[fi].e1 : F[fi].e2);
-      if (ei == kNil) return 1;
-      EEdge& e = E[ei];
-      const int16_t g = (e.f0 == fi) ? e.f1 : e.f0;
-      if (g == kNil) return 1;
-      if (F[g].vis == kVisible) { e.vis = kInternal; continue; }
-      if (F[g].vis == kHidden) { e.vis = kBorder; continue; }
-      if (outside(g, nv)) {
-        F[g].vis = kVisible;
-        e.vis = kInternal;
-        if (sp + 3 > 3 * nf + 3) return 1;  // cannot happen: each face is pushed once
-        for (int q = 0; q < 3; ++q) stk[sp++] = (static_cast<int32_t>(g) << 2) | q;
-      } else {
-        F[g].vis = kHidden;
-        e.vis = kBorder;
-      }
-    }
-    for (int16_t i = el; i != kNil; i = E[i].next)
-      if (E[i].vis != kInternal) { V[E[i].v0].remove = 0; V[E[i].v1].remove = 0; }
-    {
-      int16_t prev = kNil, cur = fl;
-      while (cur != kNil) {
-        EFace& f = F[cur];
-        const int16_t nx = f.next;
-        if (f.vis == kVisible) {
-          const int16_t es[3] = {f.e0, f.e1, f.e2};
-          for (int q = 0; q < 3; ++q) {
-            EEdge& ek = E[es[q]];
-            if (ek.f0 == cur) ek.f0 = ek.f1;
-            ek.f1 = kNil;
-          }
-          if (prev == kNil) fl = nx; else F[prev].next = nx;
-          f.next = kNil;
-          f.mnext = ffree;
-          ffree = cur;
-        } else {
-          prev = cur;
+    __builtin_amdgcn_wave_barrier();
+    while (true) {
+      bool changed = false;
+      for (int s = l; s < nf; s += 64) {
+        const MFCL_LDS EFace& f = F[s];
+        if (f.stamp < 0 || f.vis != kFUnknown) continue;
+        const int16_t es[3] = {f.e0, f.e1, f.e2};
+        bool near_visible = false;
+        for (int k = 0; k < 3; ++k) {
+          if (es[k] == kNil) continue;
+          const MFCL_LDS EEdge& e = E[es[k]];
+          const int16_t g = e.f0 == s ? e.f1 : e.f0;
+          if (g != kNil && F[g].vis == kVisible) near_visible = true;
         }
-        cur = nx;
+        if (!near_visible) continue;
+        const bool out = outsideLane(static_cast<int16_t>(s), nv_);
+        F[s].vis = out ? kVisible : kHidden;
+        changed = true;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (!anyLane(changed)) break;
+    }
+    // an open edge on a visible face is a malformed polytope
+    bool bad = false;
+    for (int s = l; s < nf; s += 64) {
+      const MFCL_LDS EFace& f = F[s];
+      if (f.stamp < 0 || f.vis != kVisible) continue;
+      const int16_t es[3] = {f.e0, f.e1, f.e2};
+      for (int k = 0; k < 3; ++k) {
+        if (es[k] == kNil) { bad = true; continue; }
+        const MFCL_LDS EEdge& e = E[es[k]];
+        const int16_t g = e.f0 == s ? e.f1 : e.f0;
+        if (g == kNil) bad = true;
       }
     }
-    {
-      int16_t prev = kNil, cur = el;
-      while (cur != kNil) {
-        EEdge& e = E[cur];
-        const int16_t nx = e.next;
-        if (e.vis == kInternal) {
-          if (prev == kNil) el = nx; else E[prev].next = nx;
-          e.next = kNil;
-          e.mnext = efree;
-          efree = cur;
-        } else {
-          prev = cur;
-        }
-        cur = nx;
-      }
+    if (anyLane(bad)) return 1;
+    // edge marks from the final face classes
+    for (int s = l; s < nv; s += 64) {
+      MFCL_LDS EEdge& e = E[s];
+      if (e.stamp < 0) continue;
+      const bool v0 = e.f0 != kNil && F[e.f0].vis == kVisible;
+      const bool v1 = e.f1 != kNil && F[e.f1].vis == kVisible;
+      if (v0 && v1) e.vis = kInternal;
+      else if (v0 || v1) e.vis = kBorder;
     }
-    {
-      int16_t prev = kNil, cur = vl;
-      while (cur != kNil) {
-        EVert& v = V[cur];
-        const int16_t nx = v.next;
-        if (v.remove) {
-          if (prev == kNil) vl = nx; else V[prev].next = nx;
-          v.next = kNil;
-          v.mnext = vfree;
-          vfree = cur;
-        } else {
-          prev = cur;
-        }
-        cur = nx;
-      }
+    __builtin_amdgcn_wave_barrier();
+    for (int s = l; s < nv; s += 64) {
+      const MFCL_LDS EEdge& e = E[s];
+      if (e.stamp < 0 || e.vis == kInternal) continue;
+      V[e.v0].remove = 0;
+      V[e.v1].remove = 0;
     }
-    const int16_t nvi = addVertex(nv, nd);
+    __builtin_amdgcn_wave_barrier();
+    // drop the visible faces (detaching them from their border edges), the
+    // internal edges and the vertices left without a non-internal edge
+    for (int s = l; s < nf; s += 64) {
+      MFCL_LDS EFace& f = F[s];
+      if (f.stamp < 0 || f.vis != kVisible) continue;
+      const int16_t es[3] = {f.e0, f.e1, f.e2};
+      for (int k = 0; k < 3; ++k) {
+        if (es[k] == kNil) continue;
+        MFCL_LDS EEdge& ek = E[es[k]];
+        if (ek.vis == kInternal) continue;  // removed below
+        if (ek.f0 == s) ek.f0 = ek.f1;
+        ek.f1 = kNil;
+      }
+      f.stamp = -1;
+    }
+    for (int s = l; s < nv; s += 64) {
+      MFCL_LDS EEdge& e = E[s];
+      if (e.stamp >= 0 && e.vis == kInternal) e.stamp = -1;
+    }
+    for (int s = l; s < nv; s += 64) {
+      MFCL_LDS EVert& v = V[s];
+      if (v.stamp >= 0 && v.remove) v.stamp = -1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int16_t nvi = addVertex(nv_, nd);
     if (nvi == kNil) return 2;
+    // new faces over the border edges, newest border edge first
     bool ok = true;
-    for (int16_t i = el; i != kNil; i = E[i].next) {
-      if (E[i].vis != kBorder) continue;
+    int last = 0x7fffffff;
+    while (true) {
+      int bs = -1, bi = kNil;
+      for (int s = l; s < nv; s += 64) {
+        const MFCL_LDS EEdge& e = E[s];
+        const int st = e.stamp;
+        if (st < 0 || e.vis != kBorder || st >= last) continue;
+        if (st > bs) { bs = st; bi = s; }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const int os = __shfl_xor(bs, off), oi = __shfl_xor(bi, off);
+        if (os > bs) { bs = os; bi = oi; }
+      }
+      if (bs < 0) break;
+      last = bs;
+      const int16_t i = static_cast<int16_t>(bi);
       int16_t ne[2];
       const int16_t vs[2] = {E[i].v0, E[i].v1};
       for (int q = 0; q < 2; ++q) {
-        EVert& vq = V[vs[q]];
-        if (vq.new_edge == kNil) vq.new_edge = addEdge(nvi, vs[q]);
+        MFCL_LDS EVert& vq = V[vs[q]];
+        if (vq.new_edge == kNil) {
+          const int16_t x = addEdge(nvi, vs[q]);
+          vq.new_edge = x;
+          __builtin_amdgcn_wave_barrier();
+        }
         ne[q] = vq.new_edge;
       }
       if (addFace(i, ne[0], ne[1]) == kNil) ok = false;
     }
     return ok ? 0 : 2;
+  }
+
+ private:
+  // outside() evaluated by one lane for its own face (no cross-lane work:
+  // the vertex fallback of faceNormalOut is scanned by that lane alone)
+  MFCL_D bool outsideLane(int16_t fi, d3 p) const {
+    int16_t ia, ib, ic;
+    faceVerts(fi, ia, ib, ic);
+    const d3 a = vloc(ia), b = vloc(ib), c = vloc(ic);
+    const d3 n = cross(a - b, b - c);
+    const double nn = norm(n);
+    if (nn <= 0.0) return 0.5 * nn <= 0.0;
+    const d3 dir = n / nn;
+    const double od = dot(a, dir);
+    d3 normal;
+    if (fabs(od) >= 1e-4) {
+      normal = od > 0 ? dir : -dir;
+    } else {
+      double maxp = 0.0, minn = 0.0;
+      for (int s = 0; s < nv; ++s) {
+        if (V[s].stamp < 0) continue;
+        const double dv = dot(vloc(static_cast<int16_t>(s)), dir);
+        if (dv > maxp) maxp = dv;
+        if (dv < minn) minn = dv;
+      }
+      normal = fabs(maxp) > fabs(minn) ? -dir : dir;
+    }
+    return dot(normal, p - faceAnchor(fi)) >= 0.0;
   }
 };
 
@@ -367,6 +487,19 @@ struct EpaOut {
   double depth;
   d3 p0, p1;
   int iters;
+  int scanned_extra;  // hull vertices scanned by the out-of-line helpers
+};
+
+// The out-of-line helpers take the Minkowski difference by value (a
+// reference would keep the caller's copy in private memory for the whole
+// EPA loop); this adds the vertices they scan to the pair's tally on exit.
+template <class MK>
+struct ScanTally {
+  MK& md;
+  EpaOut& o;
+  int start;
+  MFCL_D ScanTally(MK& m, EpaOut& out) : md(m), o(out), start(m.scanned) {}
+  MFCL_D ~ScanTally() { o.scanned_extra += md.scanned - start; }
 };
 
 template <class MK>
@@ -377,7 +510,7 @@ MFCL_D void touchingPoint(MK& md, d3 d, EpaOut& o) {
 }
 
 // epa_simplex2polytope.hpp:184-217; returns 0 ok, 2 failed
-__device__ __attribute__((noinline)) int tetra(Poly& P, const MV& a, const MV& b, const MV& c, const MV& d) {
+__device__ __attribute__((noinline)) int tetra(Poly& P, const MVD& a, const MVD& b, const MVD& c, const MVD& d) {
   P.reset();
   int16_t v[4], e[6], f[4];
   v[0] = P.addVertex(a.v, a.d);
@@ -401,7 +534,8 @@ __device__ __attribute__((noinline)) int tetra(Poly& P, const MV& a, const MV& b
 
 // s2p status: 0 OK, 1 touching, 2 failed
 template <class MK>
-__device__ __attribute__((noinline)) int s2p3(Poly& P, MK& md, const MV& a, const MV& b, const MV& c, double thr, EpaOut& o) {
+__device__ __attribute__((noinline)) int s2p3(Poly& P, MK md, const MVD& a, const MVD& b, const MVD& c, double thr, EpaOut& o) {
+  ScanTally tally(md, o);
   d3 n = cross(b.v - a.v, c.v - a.v);
   if (sqnorm(n) <= 0.0) return 2;
   n = normalized(n);
@@ -412,12 +546,13 @@ __device__ __attribute__((noinline)) int s2p3(Poly& P, MK& md, const MV& a, cons
   const double t1 = planeDist(d1, a.v, b.v, c.v);
   if (t0 <= thr) { touchingPoint(md, n, o); return 1; }
   if (t1 <= thr) { touchingPoint(md, dir1, o); return 1; }
-  if (t0 > t1) return tetra(P, a, b, c, MV{d0, n});
-  return tetra(P, a, b, c, MV{d1, dir1});
+  if (t0 > t1) return tetra(P, a, b, c, MVD{d0, n});
+  return tetra(P, a, b, c, MVD{d1, dir1});
 }
 
 template <class MK>
-__device__ __attribute__((noinline)) int s2p2(Poly& P, MK& md, const MV& a, const MV& b, double thr, EpaOut& o) {
+__device__ __attribute__((noinline)) int s2p2(Poly& P, MK md, const MVD& a, const MVD& b, double thr, EpaOut& o) {
+  ScanTally tally(md, o);
   const double thr2 = thr * thr;
   const d3 ab = b.v - a.v;
   if (sqnorm(ab) < thr2) return 2;
@@ -442,18 +577,18 @@ __device__ __attribute__((noinline)) int s2p2(Poly& P, MK& md, const MV& a, cons
     dcur = normalized(dcur + step * cross(abu, vcur));
   }
   if (!found) { touchingPoint(md, dinit, o); return 1; }
-  const MV v0 = {vfound, dfound};
-  MV v1;
+  const MVD v0 = {vfound, dfound};
+  MVD v1;
   v1.d = -v0.d;
   v1.v = md.support(v1.d);
   if (lineDist(v1.v, a.v, b.v) < thr) { touchingPoint(md, v1.d, o); return 1; }
   const d3 n = cross(v0.v - a.v, v1.v - a.v);
   if (sqnorm(n) < thr2) return 2;
-  MV v2;
+  MVD v2;
   v2.d = normalized(n);
   v2.v = md.support(v2.d);
   if (lineDist(v2.v, a.v, b.v) < thr) { touchingPoint(md, v2.d, o); return 1; }
-  MV v3;
+  MVD v3;
   v3.d = -v2.d;
   v3.v = md.support(v3.d);
   if (lineDist(v3.v, a.v, b.v) < thr) { touchingPoint(md, v3.d, o); return 1; }
@@ -492,7 +627,7 @@ __device__ __attribute__((noinline)) int s2p2(Poly& P, MK& md, const MV& a, cons
 
 // epa_simplex2polytope.hpp:46-110
 template <class MK>
-MFCL_D int simplexToPolytope(Poly& P, MK& md, const Smp& s, double thr, EpaOut& o) {
+MFCL_D int simplexToPolytope(Poly& P, MK& md, const SmpD& s, double thr, EpaOut& o) {
   const double thr2 = thr * thr;
   if (s.rank > 0 && sqnorm(s.a.v) <= thr2) { touchingPoint(md, s.a.d, o); return 1; }
   if (s.rank > 1 && sqnorm(s.b.v) <= thr2) { touchingPoint(md, s.b.d, o); return 1; }
@@ -516,28 +651,35 @@ MFCL_D int simplexToPolytope(Poly& P, MK& md, const Smp& s, double thr, EpaOut& 
 struct RawF {
   int kind;
   MinD m;
-  MV a, b, c;
+  MVD a, b, c;
 };
 
-MFCL_D void toRaw(const Poly& P, int kind, int16_t i, RawF& r) {
+MFCL_D void ldsSet(MFCL_LDS d3& o, d3 v) {
+  o.x = v.x;
+  o.y = v.y;
+  o.z = v.z;
+}
+MFCL_D d3 ldsGet(const MFCL_LDS d3& o) { return d3{o.x, o.y, o.z}; }
+
+MFCL_D void toRaw(const Poly& P, int kind, int16_t i, MFCL_LDS RawF& r) {
   r.kind = kind;
   if (kind == 1) {
     const d3 v = P.vloc(i);
-    r.m.in = true; r.m.d2 = sqnorm(v); r.m.w = v;
-    r.a = MV{v, P.vdir(i)};
+    r.m.in = true; r.m.d2 = sqnorm(v); ldsSet(r.m.w, v);
+    ldsSet(r.a.v, v); ldsSet(r.a.d, P.vdir(i));
   } else if (kind == 2) {
-    const EEdge& e = P.E[i];
-    r.m.in = e.in; r.m.d2 = e.d2; r.m.w = d3{e.wx, e.wy, e.wz};
-    r.a = MV{P.vloc(e.v0), P.vdir(e.v0)};
-    r.b = MV{P.vloc(e.v1), P.vdir(e.v1)};
+    const MFCL_LDS EEdge& e = P.E[i];
+    r.m.in = e.in; r.m.d2 = e.d2; ldsSet(r.m.w, d3{e.wx, e.wy, e.wz});
+    ldsSet(r.a.v, P.vloc(e.v0)); ldsSet(r.a.d, P.vdir(e.v0));
+    ldsSet(r.b.v, P.vloc(e.v1)); ldsSet(r.b.d, P.vdir(e.v1));
   } else {
-    const EFace& f = P.F[i];
-    r.m.in = f.in; r.m.d2 = f.d2; r.m.w = d3{f.wx, f.wy, f.wz};
+    const MFCL_LDS EFace& f = P.F[i];
+    r.m.in = f.in; r.m.d2 = f.d2; ldsSet(r.m.w, d3{f.wx, f.wy, f.wz});
     int16_t ia, ib, ic;
     P.faceVerts(i, ia, ib, ic);
-    r.a = MV{P.vloc(ia), P.vdir(ia)};
-    r.b = MV{P.vloc(ib), P.vdir(ib)};
-    r.c = MV{P.vloc(ic), P.vdir(ic)};
+    ldsSet(r.a.v, P.vloc(ia)); ldsSet(r.a.d, P.vdir(ia));
+    ldsSet(r.b.v, P.vloc(ib)); ldsSet(r.b.d, P.vdir(ib));
+    ldsSet(r.c.v, P.vloc(ic)); ldsSet(r.c.d, P.vdir(ic));
   }
 }
 
@@ -549,7 +691,7 @@ MFCL_D int nextDir(const Poly& P, MK& md, int kind, int16_t fi, double tol, d3& 
     d3 fn;
     if (!P.faceNormalOut(f, fn, nullptr)) return 1;
     const d3 on = P.faceAnchor(f);
-    const EFace& F = P.F[f];
+    const MFCL_LDS EFace& F = P.F[f];
     if (try_w && F.d2 > kOuter) {
       nd = normalized(d3{F.wx, F.wy, F.wz});
       nv = md.support(nd);
@@ -562,7 +704,7 @@ MFCL_D int nextDir(const Poly& P, MK& md, int kind, int16_t fi, double tol, d3& 
   };
   if (kind == 3) return face_cand(fi, true);
   if (kind == 2) {
-    const EEdge& e = P.E[fi];
+    const MFCL_LDS EEdge& e = P.E[fi];
     const int16_t f0 = e.f0, f1 = e.f1;
     if (f0 == kNil || f1 == kNil) return 1;  // open edge: malformed polytope
     if (e.d2 > kOuter) {
@@ -579,14 +721,14 @@ MFCL_D int nextDir(const Poly& P, MK& md, int kind, int16_t fi, double tol, d3& 
 }
 
 // epa.hpp:269-299 (edge case: squared norm of the segment projection, as in the reference)
-MFCL_D bool epaTerminate(const RawF& f, d3 nv, double tol) {
+MFCL_D bool epaTerminate(const MFCL_LDS RawF& f, d3 nv, double tol) {
   double d2;
   if (f.kind == 2) {
     MinD m;
-    segDist(nv, f.a.v, f.b.v, m);
+    segDist(nv, ldsGet(f.a.v), ldsGet(f.b.v), m);
     d2 = m.d2;
   } else {
-    const double t = planeDist(nv, f.a.v, f.b.v, f.c.v);
+    const double t = planeDist(nv, ldsGet(f.a.v), ldsGet(f.b.v), ldsGet(f.c.v));
     d2 = t * t;
   }
   return d2 < tol * tol;
@@ -594,36 +736,37 @@ MFCL_D bool epaTerminate(const RawF& f, d3 nv, double tol) {
 
 // epa.hpp:341-498
 template <class MK>
-__device__ __attribute__((noinline)) void assignPair(MK& md, d3 cand_d, const RawF& f, EpaOut& o) {
+__device__ __attribute__((noinline)) void assignPair(MK md, d3 cand_d, const MFCL_LDS RawF& f, EpaOut& o) {
+  ScanTally tally(md, o);
   if (f.kind == 1) {
-    o.p0 = md.support0(f.a.d);
-    o.p1 = md.support1(-f.a.d);
-    o.depth = norm(f.a.v);
+    o.p0 = md.support0(ldsGet(f.a.d));
+    o.p1 = md.support1(-ldsGet(f.a.d));
+    o.depth = norm(ldsGet(f.a.v));
     return;
   }
   if (f.kind == 2) {
     o.depth = sqrt(f.m.d2);
-    const d3 p = f.m.w, a = f.a.v, b = f.b.v;
+    const d3 p = ldsGet(f.m.w), a = ldsGet(f.a.v), b = ldsGet(f.b.v);
     const d3 ab = b - a;
     int ax = 0;
     double mx = fabs(ab.x);
     if (fabs(ab.y) > mx) { ax = 1; mx = fabs(ab.y); }
     if (fabs(ab.z) > mx) { ax = 2; mx = fabs(ab.z); }
     if (mx <= 1e-10) {
-      o.p0 = md.support0(f.a.d);
-      o.p1 = md.support1(-f.a.d);
+      o.p0 = md.support0(ldsGet(f.a.d));
+      o.p1 = md.support1(-ldsGet(f.a.d));
       return;
     }
     const double s = (comp(p, ax) - comp(a, ax)) / (comp(b, ax) - comp(a, ax));
     if (!(s < 0 || s > 1.0)) {
       const double wa = 1.0 - s, wb = s;
-      o.p0 = wa * md.support0(f.a.d) + wb * md.support0(f.b.d);
-      o.p1 = wa * md.support1(-f.a.d) + wb * md.support1(-f.b.d);
+      o.p0 = wa * md.support0(ldsGet(f.a.d)) + wb * md.support0(ldsGet(f.b.d));
+      o.p1 = wa * md.support1(-ldsGet(f.a.d)) + wb * md.support1(-ldsGet(f.b.d));
       return;
     }
   } else if (f.kind == 3) {
     o.depth = sqrt(f.m.d2);
-    const d3 a = f.a.v, b = f.b.v, c = f.c.v, p = f.m.w;
+    const d3 a = ldsGet(f.a.v), b = ldsGet(f.b.v), c = ldsGet(f.c.v), p = ldsGet(f.m.w);
     const d3 dl0 = a - b, dl1 = b - c, dl2 = c - a;
     const double area = sqrt(sqnorm(cross(dl0, dl1)));
     const double w0 = norm(cross(dl1, b - p)) / area;
@@ -631,8 +774,8 @@ __device__ __attribute__((noinline)) void assignPair(MK& md, d3 cand_d, const Ra
     const double chk = norm(cross(dl0, a - p)) / area;
     const double w2 = 1.0 - w0 - w1;
     if (!(fabs(chk - w2) > 0.01)) {
-      o.p0 = w0 * md.support0(f.a.d) + w1 * md.support0(f.b.d) + w2 * md.support0(f.c.d);
-      o.p1 = w0 * md.support1(-f.a.d) + w1 * md.support1(-f.b.d) + w2 * md.support1(-f.c.d);
+      o.p0 = w0 * md.support0(ldsGet(f.a.d)) + w1 * md.support0(ldsGet(f.b.d)) + w2 * md.support0(ldsGet(f.c.d));
+      o.p1 = w0 * md.support1(-ldsGet(f.a.d)) + w1 * md.support1(-ldsGet(f.b.d)) + w2 * md.support1(-ldsGet(f.c.d));
       return;
     }
   }
@@ -645,8 +788,10 @@ __device__ __attribute__((noinline)) void assignPair(MK& md, d3 cand_d, const Ra
 
 // epa.hpp:137-257
 template <class MK>
-MFCL_D void epaEvaluate(Poly& P, MK& md, const Smp& s, double tol, int max_iter, EpaOut& o) {
+MFCL_D void epaEvaluate(Poly& P, MK& md, const SmpD& s, double tol, int max_iter, EpaOut& o,
+                        MFCL_LDS RawF& raw) {
   o.iters = 0;
+  o.scanned_extra = 0;
   o.depth = 0;
   o.p0 = d3{0, 0, 0};
   o.p1 = d3{0, 0, 0};
@@ -654,7 +799,6 @@ MFCL_D void epaEvaluate(Poly& P, MK& md, const Smp& s, double tol, int max_iter,
   if (s2p == 2) { o.status = kEpaFailed; return; }
   if (s2p == 1) { o.depth = 0.0; o.status = kEpaTouching; return; }
   int it = 0;
-  RawF raw;
   while (true) {
     int kind;
     int16_t fi;

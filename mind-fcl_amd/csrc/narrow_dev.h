@@ -18,13 +18,39 @@ namespace mfcl {
 
 enum ShapeType : int { kBox = 0, kSphere = 1, kCapsule = 3, kCylinder = 5, kConvex = 100 };
 
+#define MFCL_LDS __attribute__((address_space(3)))
+
 struct DevShape {
   int type;
   double p0, p1, p2;    // box: full sides; sphere: r; capsule/cylinder: r, full length
   const float4* verts;  // convex: vertices (x, y, z, unused)
   int nv;
   float rbound;         // convex: max |coordinate| over the vertices
+  // convex: optional copy of the vertices in LDS (x at lv[i], y at lv[i + ls],
+  // z at lv[i + 2 ls]); nullptr when the hull is read from device memory
+  const MFCL_LDS float* lv;
+  int ls;
 };
+
+// Vertex i of a hull, from the LDS copy when there is one (same values).
+MFCL_D float3 hullVert(const DevShape& s, int i) {
+  if (s.lv) return make_float3(s.lv[i], s.lv[i + s.ls], s.lv[i + 2 * s.ls]);
+  const float4 q = s.verts[i];
+  return make_float3(q.x, q.y, q.z);
+}
+
+// Copies a hull's vertices into LDS (SoA rows of stride ls), lanes lg, lg+G, ...
+template <int G>
+MFCL_D void stageHull(DevShape& s, MFCL_LDS float* dst, int ls, int lg) {
+  for (int i = lg; i < s.nv; i += G) {
+    const float4 q = s.verts[i];
+    dst[i] = q.x;
+    dst[i + ls] = q.y;
+    dst[i + 2 * ls] = q.z;
+  }
+  s.lv = dst;
+  s.ls = ls;
+}
 
 struct ShapeTables {
   const int* type;
@@ -49,6 +75,8 @@ MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
     s.verts = nullptr;
     s.nv = 0;
   }
+  s.lv = nullptr;
+  s.ls = 0;
   return s;
 }
 
@@ -67,15 +95,22 @@ MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
   {
 #pragma clang fp contract(fast)
     const float fx = static_cast<float>(d.x), fy = static_cast<float>(d.y), fz = static_cast<float>(d.z);
-    for (int i = lg; i < s.nv; i += G) {
-      const float4 q = s.verts[i];
-      const float val = fx * q.x + fy * q.y + fz * q.z;
+    auto visit = [&](int i, float x, float y, float z) {
+      const float val = fx * x + fy * y + fz * z;
       if (val > best) {
         second = best;
         best = val;
         bi = i;
       } else if (val > second) {
         second = val;
+      }
+    };
+    if (s.lv) {
+      for (int i = lg; i < s.nv; i += G) visit(i, s.lv[i], s.lv[i + s.ls], s.lv[i + 2 * s.ls]);
+    } else {
+      for (int i = lg; i < s.nv; i += G) {
+        const float4 q = s.verts[i];
+        visit(i, q.x, q.y, q.z);
       }
     }
   }
@@ -102,7 +137,7 @@ MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
   double bd = -INFINITY;
   bi = 0x7fffffff;
   for (int i = lg; i < s.nv; i += G) {
-    const float4 q = s.verts[i];
+    const float3 q = hullVert(s, i);
     const double val = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
     if (val > bd) {
       bd = val;
@@ -143,12 +178,24 @@ MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int lg, int* scanned) {
       return {k * dir.x, k * dir.y, dir.z > 0 ? hh : -hh};
     }
     case kConvex: {
-      const float4 q = s.verts[hullArgmax<G>(s, dir, lg, scanned)];
+      const float3 q = hullVert(s, hullArgmax<G>(s, dir, lg, scanned));
       return {q.x, q.y, q.z};
     }
     default:
       return {0, 0, 0};
   }
+}
+
+// shapeSupport plus the index of the hull vertex returned (-1 for primitives).
+template <int G>
+MFCL_D d3 shapeSupportI(const DevShape& s, d3 dir, int lg, int* scanned, int& idx) {
+  if (s.type == kConvex) {
+    idx = hullArgmax<G>(s, dir, lg, scanned);
+    const float3 q = hullVert(s, idx);
+    return {q.x, q.y, q.z};
+  }
+  idx = -1;
+  return shapeSupport<G>(s, dir, lg, scanned);
 }
 
 MFCL_D d3 shapeInterior(const DevShape& s, const float4* interior_tab, uint32_t id) {
@@ -159,37 +206,172 @@ MFCL_D d3 shapeInterior(const DevShape& s, const float4* interior_tab, uint32_t 
   return {0, 0, 0};
 }
 
-// MinkowskiDiff (include/fcl/cvx_collide/minkowski_diff.hpp:14-85), evaluated
-// by a group of G lanes (lane lg of the group).
-template <int G>
-struct MinkT {
-  DevShape s0, s1;
-  m33 toshape1;  // frame 0 -> frame 1 rotation
-  tf3 toshape0;  // frame 1 -> frame 0
-  d3 int0, int1; // interior points in each shape's own frame
-  int scanned;   // hull vertices scanned (for the VALU accounting)
-  int lg;
-  MFCL_D d3 support0(d3 d) { return shapeSupport<G>(s0, d, lg, &scanned); }
-  MFCL_D d3 support1(d3 d) { return apply(toshape0, shapeSupport<G>(s1, mul(toshape1, d), lg, &scanned)); }
-  MFCL_D d3 support(d3 d) { return support0(d) - support1(-d); }
-  MFCL_D d3 interior() { return int0 - apply(toshape0, int1); }
-};
-using Mink = MinkT<1>;
+// ---------------------------------------------------------------------------
+// The same Minkowski difference with its configuration (both shapes, the two
+// relative transforms, interior points) held in LDS instead of VGPRs.  The
+// lanes that own a pair all write the same values to their slot and re-read
+// them at every support call (a compiler-only memory fence stops the loads
+// being hoisted into registers), which keeps ~80 FP64 registers out of the
+// long-lived state of the GJK and EPA loops.
+#define MFCL_LDS __attribute__((address_space(3)))
 
-template <class MK>
-MFCL_D void setupMink(MK& md, const DevShape& a, const tf3& ta, const DevShape& b, const tf3& tb, int lg) {
-  md.s0 = a;
-  md.s1 = b;
-  md.toshape1 = tmulm(tb.R, ta.R);
-  md.toshape0 = relTf(ta, tb);
-  md.scanned = 0;
-  md.lg = lg;
-  md.int0 = d3{0, 0, 0};
-  md.int1 = d3{0, 0, 0};
+// Direction that produced a simplex vertex, and the hull vertex indices its
+// support points came from (-1 for a primitive shape: recomputed from the
+// direction when needed).
+struct DirSlot {
+  double d[3];
+  int i0, i1;
+};
+constexpr int kDirSlots = 8;
+
+struct MinkCfg {
+  DirSlot slot[kDirSlots];
+  double r1[9];  // frame 0 -> frame 1 rotation (toshape1)
+  double r0[9];  // frame 1 -> frame 0 rotation (toshape0)
+  double t0[3];  // frame 1 -> frame 0 translation
+  double q0[3], q1[3];  // primitive parameters of shape 0 / 1
+  double int0[3], int1[3];
+  const float4* v0;
+  const float4* v1;
+  const MFCL_LDS float* lv0;
+  const MFCL_LDS float* lv1;
+  int ls0, ls1;
+  int type0, type1, nv0, nv1;
+  float rb0, rb1;
+};
+
+MFCL_D void reloadFence() { asm volatile("" ::: "memory"); }
+
+MFCL_D void storeM33(MFCL_LDS double* o, const m33& A) {
+  o[0] = A.m00; o[1] = A.m01; o[2] = A.m02;
+  o[3] = A.m10; o[4] = A.m11; o[5] = A.m12;
+  o[6] = A.m20; o[7] = A.m21; o[8] = A.m22;
+}
+MFCL_D m33 loadM33(const MFCL_LDS double* o) {
+  m33 A;
+  A.m00 = o[0]; A.m01 = o[1]; A.m02 = o[2];
+  A.m10 = o[3]; A.m11 = o[4]; A.m12 = o[5];
+  A.m20 = o[6]; A.m21 = o[7]; A.m22 = o[8];
+  return A;
 }
 
+MFCL_D void writeCfg(MFCL_LDS MinkCfg* c, const DevShape& a, const tf3& ta, const DevShape& b, const tf3& tb,
+                     d3 int0, d3 int1) {
+  const m33 r1 = tmulm(tb.R, ta.R);
+  const tf3 t0 = relTf(ta, tb);
+  storeM33(c->r1, r1);
+  storeM33(c->r0, t0.R);
+  c->t0[0] = t0.t.x; c->t0[1] = t0.t.y; c->t0[2] = t0.t.z;
+  c->q0[0] = a.p0; c->q0[1] = a.p1; c->q0[2] = a.p2;
+  c->q1[0] = b.p0; c->q1[1] = b.p1; c->q1[2] = b.p2;
+  c->int0[0] = int0.x; c->int0[1] = int0.y; c->int0[2] = int0.z;
+  c->int1[0] = int1.x; c->int1[1] = int1.y; c->int1[2] = int1.z;
+  c->v0 = a.verts; c->v1 = b.verts;
+  c->lv0 = a.lv; c->lv1 = b.lv;
+  c->ls0 = a.ls; c->ls1 = b.ls;
+  c->type0 = a.type; c->type1 = b.type;
+  c->nv0 = a.nv; c->nv1 = b.nv;
+  c->rb0 = a.rbound; c->rb1 = b.rbound;
+}
+
+MFCL_D d3 slotDir(const MFCL_LDS MinkCfg* c, int id) {
+  reloadFence();
+  const MFCL_LDS DirSlot& sl = c->slot[id & (kDirSlots - 1)];
+  return d3{sl.d[0], sl.d[1], sl.d[2]};
+}
+
+template <int G>
+struct MinkL {
+  MFCL_LDS MinkCfg* c;
+  int lg;
+  int scanned;
+  MFCL_D DevShape sh0() const {
+    DevShape s;
+    s.type = c->type0; s.p0 = c->q0[0]; s.p1 = c->q0[1]; s.p2 = c->q0[2];
+    s.verts = c->v0; s.nv = c->nv0; s.rbound = c->rb0;
+    s.lv = c->lv0; s.ls = c->ls0;
+    return s;
+  }
+  MFCL_D DevShape sh1() const {
+    DevShape s;
+    s.type = c->type1; s.p0 = c->q1[0]; s.p1 = c->q1[1]; s.p2 = c->q1[2];
+    s.verts = c->v1; s.nv = c->nv1; s.rbound = c->rb1;
+    s.lv = c->lv1; s.ls = c->ls1;
+    return s;
+  }
+  MFCL_D m33 rot1() const { return loadM33(c->r1); }
+  MFCL_D tf3 tf0() const {
+    tf3 t;
+    t.R = loadM33(c->r0);
+    t.t = d3{c->t0[0], c->t0[1], c->t0[2]};
+    return t;
+  }
+  MFCL_D d3 support0(d3 d) {
+    reloadFence();
+    return shapeSupport<G>(sh0(), d, lg, &scanned);
+  }
+  MFCL_D d3 support1(d3 d) {
+    reloadFence();
+    const d3 x = shapeSupport<G>(sh1(), mul(rot1(), d), lg, &scanned);
+    reloadFence();
+    return apply(tf0(), x);
+  }
+  MFCL_D d3 support(d3 d) { return support0(d) - support1(-d); }
+  // support(d), recording d and the support vertex indices in slot id.
+  MFCL_D d3 supportNew(d3 d, int id) {
+    int i0, i1;
+    reloadFence();
+    const d3 a = shapeSupportI<G>(sh0(), d, lg, &scanned, i0);
+    reloadFence();
+    const d3 x = shapeSupportI<G>(sh1(), mul(rot1(), -d), lg, &scanned, i1);
+    reloadFence();
+    const d3 b = apply(tf0(), x);
+    MFCL_LDS DirSlot& sl = c->slot[id & (kDirSlots - 1)];
+    sl.d[0] = d.x; sl.d[1] = d.y; sl.d[2] = d.z;
+    sl.i0 = i0; sl.i1 = i1;
+    return a - b;
+  }
+  MFCL_D d3 dirOf(int id) {
+    reloadFence();
+    const MFCL_LDS DirSlot& sl = c->slot[id & (kDirSlots - 1)];
+    return d3{sl.d[0], sl.d[1], sl.d[2]};
+  }
+  // support0(dir of slot id), from the recorded hull vertex when there is one
+  MFCL_D d3 point0(int id) {
+    reloadFence();
+    const int i0 = c->slot[id & (kDirSlots - 1)].i0;
+    if (i0 >= 0) {
+      const float3 q = hullVert(sh0(), i0);
+      return d3{q.x, q.y, q.z};
+    }
+    return support0(dirOf(id));
+  }
+  // support1(-dir of slot id), likewise
+  MFCL_D d3 point1(int id) {
+    reloadFence();
+    const int i1 = c->slot[id & (kDirSlots - 1)].i1;
+    if (i1 >= 0) {
+      const float3 q = hullVert(sh1(), i1);
+      return apply(tf0(), d3{q.x, q.y, q.z});
+    }
+    return support1(-dirOf(id));
+  }
+  MFCL_D d3 interior() {
+    reloadFence();
+    const d3 i1 = d3{c->int1[0], c->int1[1], c->int1[2]};
+    return d3{c->int0[0], c->int0[1], c->int0[2]} - apply(tf0(), i1);
+  }
+};
+
+// A simplex vertex during GJK: the Minkowski-difference point plus the id
+// of the LDS slot holding the direction that produced it and the support
+// vertex indices (the direction is only stored, never used by GJK's own
+// arithmetic, so it stays out of registers).
 struct MV {
-  d3 v, d;
+  d3 v;
+  int64_t id;  // 64-bit so the record has no padding: copies stay whole-field
+               // (a 32-bit id plus padding is copied as one 64-bit word, which
+               // SROA cannot split, leaving the simplex in private memory)
 };
 
 // Four fixed simplex slots; every access below uses a constant slot index.
@@ -198,13 +380,54 @@ struct Smp {
   int rank;
 };
 
+MFCL_D void mvZero(MV& m) {
+  m.v.x = 0; m.v.y = 0; m.v.z = 0;
+  m.id = 0;
+}
+MFCL_D void smpInit(Smp& s) {
+  mvZero(s.a); mvZero(s.b); mvZero(s.c); mvZero(s.e);
+  s.rank = -1;
+}
+
+// The same simplex with explicit directions: the EPA stage's input.
+struct MVD {
+  d3 v, d;
+};
+struct SmpD {
+  MVD a, b, c, e;
+  int rank;
+};
+MFCL_D void storeMVD(MVD& o, d3 v, d3 d) {
+  o.v.x = v.x; o.v.y = v.y; o.v.z = v.z;
+  o.d.x = d.x; o.d.y = d.y; o.d.z = d.z;
+}
+
+MFCL_D MV mvSel(bool c, const MV& x, const MV& y) {
+  MV r;
+  r.v = d3{c ? x.v.x : y.v.x, c ? x.v.y : y.v.y, c ? x.v.z : y.v.z};
+  r.id = c ? x.id : y.id;
+  return r;
+}
+
+// First direction slot not referenced by the simplex (at most 4 are).
+MFCL_D int freeSlot(const Smp& s) {
+  int used = 0;
+  if (s.rank > 0) used |= 1 << static_cast<int>(s.a.id);
+  if (s.rank > 1) used |= 1 << static_cast<int>(s.b.id);
+  if (s.rank > 2) used |= 1 << static_cast<int>(s.c.id);
+  if (s.rank > 3) used |= 1 << static_cast<int>(s.e.id);
+  return __builtin_ctz(~used) & 7;
+}
+// Unconditional stores of selected values: conditional stores to different
+// slots would be merged into one store through a pointer select, which
+// forces the simplex out of registers into private memory.
 MFCL_D void smpAdd(Smp& s, const MV& m) {
-  if (s.rank < 0) s.rank = 0;
-  if (s.rank == 0) s.a = m;
-  else if (s.rank == 1) s.b = m;
-  else if (s.rank == 2) s.c = m;
-  else s.e = m;
-  s.rank += 1;
+  const int r = s.rank < 0 ? 0 : s.rank;
+  s.a = mvSel(r == 0, m, s.a);
+  s.b = mvSel(r == 1, m, s.b);
+  s.c = mvSel(r == 2, m, s.c);
+  s.e = mvSel(r >= 3, m, s.e);
+  s.rank = r + 1;
 }
 MFCL_D bool smpDup(const Smp& s, d3 v, double tol2) {
   if (s.rank > 0 && sqnorm(s.a.v - v) < tol2) return true;
@@ -380,12 +603,18 @@ MFCL_D int dist3(d3 x, d3 y, d3 z, d3& out, double tol) {
 }
 
 // Slot k of the simplex by value through selects (no indexed private array).
+// Passes a value through an empty asm so the optimiser cannot turn a select
+// between two simplex slots into a load from a computed slot address (which
+// would pin the simplex in private memory).
+MFCL_D void opaque(MV& m) {
+  asm("" : "+v"(m.v.x), "+v"(m.v.y), "+v"(m.v.z), "+v"(m.id));
+}
 MFCL_D MV smpSlot(const Smp& s, int k) {
-  MV r = s.a;
-  if (k == 1) r = s.b;
-  if (k == 2) r = s.c;
-  if (k == 3) r = s.e;
-  return r;
+  MV A = s.a, B = s.b, C = s.c, E = s.e;
+  opaque(A); opaque(B); opaque(C); opaque(E);
+  MV r = mvSel(k == 1, B, A);
+  r = mvSel(k == 2, C, r);
+  return mvSel(k == 3, E, r);
 }
 
 MFCL_D void smpApply(Smp& s, int code) {
@@ -453,32 +682,32 @@ template <class MK>
 MFCL_D bool extractNoSub(MK& md, int rank, const MV& v1, const MV& v2, const MV& v3, d3& p0, d3& p1) {
   if (rank == 4 || rank <= 0) return false;
   if (rank == 1) {
-    p0 = md.support0(v1.d);
-    p1 = md.support1(-v1.d);
+    p0 = md.point0(v1.id);
+    p1 = md.point1(v1.id);
     return true;
   }
   if (rank == 2) {
     const d3 d = v2.v - v1.v;
     const double l2 = sqnorm(d);
     if (l2 <= 0.0) {
-      p0 = md.support0(v1.d);
-      p1 = md.support1(-v1.d);
+      p0 = md.point0(v1.id);
+      p1 = md.point1(v1.id);
       return true;
     }
     const double w2 = -dot(v1.v, d) / l2;
     if (w2 > 1 + kBaryTol) {
-      p0 = md.support0(v2.d);
-      p1 = md.support1(-v2.d);
+      p0 = md.point0(v2.id);
+      p1 = md.point1(v2.id);
       return false;
     }
     if (w2 < -kBaryTol) {
-      p0 = md.support0(v1.d);
-      p1 = md.support1(-v1.d);
+      p0 = md.point0(v1.id);
+      p1 = md.point1(v1.id);
       return false;
     }
     const double w1 = 1.0 - w2;
-    p0 = md.support0(v1.d) * w1 + md.support0(v2.d) * w2;
-    p1 = md.support1(-v1.d) * w1 + md.support1(-v2.d) * w2;
+    p0 = md.point0(v1.id) * w1 + md.point0(v2.id) * w2;
+    p1 = md.point1(v1.id) * w1 + md.point1(v2.id) * w2;
     return true;
   }
   const d3 s12 = v2.v - v1.v, s13 = v3.v - v1.v;
@@ -491,8 +720,8 @@ MFCL_D bool extractNoSub(MK& md, int rank, const MV& v1, const MV& v2, const MV&
   const double w3 = norm(cross(s12, v1.v - proj)) / area;
   const double w1 = 1.0 - w2 - w3;
   if (w1 < -kBaryTol || w2 < -kBaryTol || w3 < -kBaryTol) return false;
-  p0 = md.support0(v1.d) * w1 + md.support0(v2.d) * w2 + md.support0(v3.d) * w3;
-  p1 = md.support1(-v1.d) * w1 + md.support1(-v2.d) * w2 + md.support1(-v3.d) * w3;
+  p0 = md.point0(v1.id) * w1 + md.point0(v2.id) * w2 + md.point0(v3.id) * w3;
+  p1 = md.point1(v1.id) * w1 + md.point1(v2.id) * w2 + md.point1(v3.id) * w3;
   return true;
 }
 template <class MK>
@@ -538,8 +767,8 @@ MFCL_D bool gjkMinDistance(MK& md, Smp& s, double tol, int max_iter, d3& p0, d3&
   for (int it = 0; it < max_iter; ++it) {
     (*iters)++;
     MV nv;
-    nv.d = nd;
-    nv.v = md.support(nd);
+    nv.id = freeSlot(s);
+    nv.v = md.supportNew(nd, nv.id);
     if (dot(nd, nv.v - cur) < tol) return extractNoSub(md, s, p0, p1);
     if (smpDup(s, nv.v, tol2)) return extractNoSub(md, s, p0, p1);
     smpAdd(s, nv);
@@ -576,8 +805,8 @@ MFCL_D void gjkEvaluate(MK& md, Smp& s, d3 guess, double tol, int max_iter, bool
   if (sqnorm(dir) <= 0.0) dir = d3{1, 0, 0};
   dir = normalized(dir);
   MV v;
-  v.d = dir;
-  v.v = md.support(dir);
+  v.id = 0;
+  v.v = md.supportNew(dir, v.id);
   s.rank = -1;
   smpAdd(s, v);
   const double tol2 = tol * tol;
@@ -592,8 +821,8 @@ MFCL_D void gjkEvaluate(MK& md, Smp& s, d3 guess, double tol, int max_iter, bool
     status = kGjkIterLimit;
     for (int it = 0; it < max_iter; ++it) {
       o.iters = it + 1;
-      v.d = dir;
-      v.v = md.support(dir);
+      v.id = freeSlot(s);
+      v.v = md.supportNew(dir, v.id);
       if (dot(v.v, dir) < 0) {
         separated = true;
         break;

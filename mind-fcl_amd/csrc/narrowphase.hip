@@ -30,7 +30,7 @@ struct PairIO {
 };
 
 struct QEntry {
-  Smp s;
+  SmpD s;
   uint32_t pair;
   uint32_t pad;
 };
@@ -175,6 +175,25 @@ MFCL_D void loadPair(const ShapeTables& T, const double4* interior, const PairIO
   tb = poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i));
 }
 
+// LDS staging capacity (vertices per group, both hulls together): 48 KB of
+// SoA float rows per 256-lane block whatever the group size.
+template <int G>
+constexpr int kStage = 48 * 1024 / 12 / (kBlock / G);
+
+template <int G>
+MFCL_D void stagePair(DevShape& A, DevShape& B, MFCL_LDS float* st, int lg) {
+  int used = 0;
+  if (A.type == kConvex && A.nv <= kStage<G>) {
+    stageHull<G>(A, st, kStage<G>, lg);
+    used = A.nv;
+  }
+  if (B.type == kConvex && B.nv <= kStage<G> - used) stageHull<G>(B, st + used, kStage<G>, lg);
+  // the group's lanes read each other's LDS writes: LDS executes a
+  // wavefront's operations in order; this keeps the compiler from moving
+  // the reads above the writes
+  __builtin_amdgcn_wave_barrier();
+}
+
 // One group of G lanes per pair (G divides 64); all lanes of a group run
 // the same FP64 logic on identical values, share the support scans, and
 // lane 0 of the group writes the result / queues the EPA work.
@@ -185,8 +204,14 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
                                                        const uint32_t* order, uint64_t n, Params P,
                                                        QEntry* queue, uint32_t* qcount,
                                                        unsigned long long* scanned_total) {
+  // Per group: the Minkowski configuration and an LDS copy of the two hulls
+  // (SoA rows of kStage<G> floats) when they fit, so the repeated support
+  // scans of GJK read LDS instead of device memory.
+  __shared__ MinkCfg cfgs[kBlock / G];
+  __shared__ float stage[kBlock / G][3 * kStage<G>];
   const uint64_t slot = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
   const int lg = static_cast<int>(threadIdx.x % G);
+  MFCL_LDS MinkCfg* cfg = (MFCL_LDS MinkCfg*)(&cfgs[threadIdx.x / G]);
   int scanned = 0;
   if (slot < n) {
     const uint32_t i = order[slot];
@@ -197,7 +222,7 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
     const bool bad_ids = io.pa[i] >= P.n_shapes || io.pb[i] >= P.n_shapes;
     bool queued = false;
     Smp s;
-    s.rank = -1;
+    smpInit(s);
     if (bad_ids) {
       r.ok = -1;  // shape id out of range
     } else {
@@ -234,10 +259,12 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
             }
           }
         } else {
-          MinkT<G> md;
-          setupMink(md, A, ta, B, tb, lg);
-          if (A.type == kConvex) { const double4 c = interior[ia]; md.int0 = d3{c.x, c.y, c.z}; }
-          if (B.type == kConvex) { const double4 c = interior[ib]; md.int1 = d3{c.x, c.y, c.z}; }
+          d3 int0 = {0, 0, 0}, int1 = {0, 0, 0};
+          if (A.type == kConvex) { const double4 c = interior[ia]; int0 = d3{c.x, c.y, c.z}; }
+          if (B.type == kConvex) { const double4 c = interior[ib]; int1 = d3{c.x, c.y, c.z}; }
+          stagePair<G>(A, B, (MFCL_LDS float*)(&stage[threadIdx.x / G][0]), lg);
+          writeCfg(cfg, A, ta, B, tb, int0, int1);
+          MinkL<G> md{cfg, lg, 0};
           r.method = 1;
           bool decided = false;
           if constexpr (MODE == kModeBool) {
@@ -269,8 +296,9 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
         }
       } else {
         // signed distance: no closed forms (gjk_solver-inl.h:820-874)
-        MinkT<G> md;
-        setupMink(md, A, ta, B, tb, lg);
+        stagePair<G>(A, B, (MFCL_LDS float*)(&stage[threadIdx.x / G][0]), lg);
+        writeCfg(cfg, A, ta, B, tb, d3{0, 0, 0}, d3{0, 0, 0});
+        MinkL<G> md{cfg, lg, 0};
         r.method = 1;
         d3 guess = {1, 0, 0};
         if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
@@ -294,8 +322,15 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
     if (lg == 0) {
       if (queued) {
         const uint32_t q = atomicAdd(qcount, 1u);
-        queue[q].s = s;
-        queue[q].pair = i;
+        // element-wise stores: an aggregate copy out of the simplex would
+        // become a memcpy that keeps the simplex in private memory
+        QEntry* qe = &queue[q];
+        qe->s.rank = s.rank;
+        storeMVD(qe->s.a, s.a.v, slotDir(cfg, s.a.id));
+        storeMVD(qe->s.b, s.b.v, slotDir(cfg, s.b.id));
+        storeMVD(qe->s.c, s.c.v, slotDir(cfg, s.c.id));
+        storeMVD(qe->s.e, s.e.v, slotDir(cfg, s.e.id));
+        qe->pair = i;
       }
       // queued pairs keep their GJK status here; the EPA stage completes them
       storeResult(&io.out[i], r, acc);
@@ -310,28 +345,37 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
 
 // EPA stage: one wavefront per queued pair; the 64 lanes share every
 // support scan and run the polytope bookkeeping in lockstep on identical
-// values.  kLds: the polytope lives in LDS with capacity cap_faces; a pair
-// that exhausts it (status MallocFailed while cap_faces < epa_max_faces) is
-// re-queued to `overflow` and rerun by the device-memory instance with the
-// requested capacity, so every result equals a run at full capacity.
-template <bool kLds>
+// values.  Each wavefront owns an LDS slice: the pair's Minkowski
+// configuration, the closest-feature record and the polytope arena with
+// capacity cap_faces.  A pair that exhausts the arena (status MallocFailed
+// while cap_faces < epa_max_faces) is re-queued to `overflow` and rerun by a
+// second launch with the requested capacity (one wavefront per block), so
+// every result equals a run at full capacity.
+constexpr size_t kCfgBytes = (sizeof(MinkCfg) + 15) & ~static_cast<size_t>(15);
+constexpr size_t kRawBytes = (sizeof(RawF) + 15) & ~static_cast<size_t>(15);
+constexpr size_t kPolyBytes = (sizeof(Poly) + 15) & ~static_cast<size_t>(15);
+inline size_t epaSliceBytes(int cap_faces) { return kCfgBytes + kRawBytes + kPolyBytes + arenaBytes(cap_faces); }
+
 __global__ void __launch_bounds__(kBlock) epaWaveKernel(ShapeTables T, PairIO io, Params P, const QEntry* queue,
-                                                        const uint32_t* qcount, int cap_faces, char* garena,
-                                                        size_t arena_bytes, QEntry* overflow, uint32_t* ocount,
+                                                        const uint32_t* qcount, int cap_faces, size_t slice_bytes,
+                                                        QEntry* overflow, uint32_t* ocount,
                                                         unsigned long long* scanned_total) {
   extern __shared__ __align__(16) char lds_arena[];
   const int lane = threadIdx.x & 63;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   const ArenaDims dims = arenaDims(cap_faces);
-  char* base = kLds ? lds_arena + static_cast<size_t>(threadIdx.x >> 6) * arena_bytes
-                    : garena + static_cast<size_t>(wave) * arena_bytes;
-  Poly poly;
-  poly.V = reinterpret_cast<EVert*>(base);
-  poly.E = reinterpret_cast<EEdge*>(base + sizeof(EVert) * dims.nv);
-  poly.F = reinterpret_cast<EFace*>(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv);
-  poly.stk = reinterpret_cast<int32_t*>(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv +
-                                        sizeof(EFace) * dims.nf);
+  MFCL_LDS char* slice = (MFCL_LDS char*)(lds_arena) + static_cast<size_t>(threadIdx.x >> 6) * slice_bytes;
+  MFCL_LDS MinkCfg* cfg = (MFCL_LDS MinkCfg*)(slice);
+  MFCL_LDS RawF* raw = (MFCL_LDS RawF*)(slice + kCfgBytes);
+  MFCL_LDS char* base = slice + kCfgBytes + kRawBytes + kPolyBytes;
+  // the polytope header (list heads, free lists) lives in LDS too, so the
+  // out-of-line helpers that take it by reference do not pin it in scratch
+  Poly& poly = *reinterpret_cast<Poly*>(lds_arena + static_cast<size_t>(threadIdx.x >> 6) * slice_bytes +
+                                        kCfgBytes + kRawBytes);
+  poly.V = (MFCL_LDS EVert*)(base);
+  poly.E = (MFCL_LDS EEdge*)(base + sizeof(EVert) * dims.nv);
+  poly.F = (MFCL_LDS EFace*)(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv);
   poly.nv = dims.nv;
   poly.nf = dims.nf;
   const uint32_t count = *qcount;
@@ -343,11 +387,11 @@ __global__ void __launch_bounds__(kBlock) epaWaveKernel(ShapeTables T, PairIO io
     tf3 ta, tb;
     uint32_t ia, ib;
     loadPair(T, nullptr, io, i, A, B, ta, tb, ia, ib);
-    MinkT<64> md;
-    setupMink(md, A, ta, B, tb, lane);
+    writeCfg(cfg, A, ta, B, tb, d3{0, 0, 0}, d3{0, 0, 0});
+    MinkL<64> md{cfg, lane, 0};
     EpaOut eo;
-    epaEvaluate(poly, md, e.s, P.epa_tol, P.epa_max_it, eo);
-    scanned += md.scanned;
+    epaEvaluate(poly, md, e.s, P.epa_tol, P.epa_max_it, eo, *raw);
+    scanned += md.scanned + eo.scanned_extra;
     if (eo.status == kEpaMallocFailed && cap_faces < P.epa_max_faces) {
       if (lane == 0) {
         const uint32_t o = atomicAdd(ocount, 1u);
@@ -436,12 +480,12 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   if (int rc = ensure(ctx, ctx->ws_misc, misc_bytes)) return rc;
   if (int rc = ensure(ctx, ctx->ws_order, n * (sizeof(uint32_t) + 1) + 256)) return rc;
   if (int rc = ensure(ctx, ctx->ws_queue, n * sizeof(QEntry))) return rc;
-  const size_t ab = arenaBytes(req->epa_max_faces);
-  if (int rc = ensure(ctx, ctx->ws_arena, ab * static_cast<size_t>(ctx->tune.epa_overflow_waves))) return rc;
   if (int rc = ensure(ctx, ctx->ws_oflow, n * sizeof(QEntry))) return rc;
-  if (arenaBytes(ctx->tune.epa_lds_faces) * 4 > 160 * 1024) {
-    ctx->err = "EPA LDS arena too large for a 4-wave block";
-    return MFCL_E_INVALID;
+  const size_t full_slice = epaSliceBytes(req->epa_max_faces);
+  if (full_slice > static_cast<size_t>(ctx->max_lds_per_block)) {
+    ctx->err = "epa_max_faces = " + std::to_string(req->epa_max_faces) + " needs " + std::to_string(full_slice) +
+               " bytes of LDS per wavefront; this device allows " + std::to_string(ctx->max_lds_per_block);
+    return MFCL_E_UNSUPPORTED;
   }
   uint32_t* hist = static_cast<uint32_t*>(ctx->ws_misc.p);
   uint32_t* offs = hist + kBuckets;
@@ -474,8 +518,8 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   else if (mode == kModePen) MFCL_NARROW(GG, kModePen); \
   else MFCL_NARROW(GG, kModeBool)
     switch (G) {
-      case 1: MFCL_NARROW_MODES(1); break;
-      case 8: MFCL_NARROW_MODES(8); break;
+      case 32: MFCL_NARROW_MODES(32); break;
+      case 64: MFCL_NARROW_MODES(64); break;
       default: MFCL_NARROW_MODES(16); break;
     }
 #undef MFCL_NARROW_MODES
@@ -483,15 +527,17 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   }
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
   {
-    const int cap_small = P.epa_max_faces < ctx->tune.epa_lds_faces ? P.epa_max_faces : ctx->tune.epa_lds_faces;
-    const size_t lds_bytes = arenaBytes(cap_small);
+    int cap_small = P.epa_max_faces < ctx->tune.epa_lds_faces ? P.epa_max_faces : ctx->tune.epa_lds_faces;
+    while (cap_small > 4 && 4 * epaSliceBytes(cap_small) > static_cast<size_t>(ctx->max_lds_per_block)) cap_small /= 2;
+    const size_t slice = epaSliceBytes(cap_small);
     const unsigned eb = static_cast<unsigned>(ctx->tune.epa_threads / kBlock);
-    epaWaveKernel<true><<<eb, kBlock, 4 * lds_bytes, s>>>(T, io, P, queue, qcount, cap_small, nullptr, lds_bytes,
-                                                        oflow, ocount, sc + 1);
-    // overflow pairs: full capacity in device memory, one wave per pair
-    const unsigned ob = static_cast<unsigned>(ctx->tune.epa_overflow_waves / 4);
-    epaWaveKernel<false><<<ob, kBlock, 0, s>>>(T, io, P, oflow, ocount, P.epa_max_faces,
-                                               static_cast<char*>(ctx->ws_arena.p), ab, oflow, ocount, sc + 1);
+    epaWaveKernel<<<eb, kBlock, 4 * slice, s>>>(T, io, P, queue, qcount, cap_small, slice, oflow, ocount, sc + 1);
+    // overflow pairs: full capacity, one wavefront per block
+    if (cap_small < P.epa_max_faces) {
+      const unsigned ob = static_cast<unsigned>(ctx->tune.epa_overflow_waves);
+      epaWaveKernel<<<ob, 64, full_slice, s>>>(T, io, P, oflow, ocount, P.epa_max_faces, full_slice, oflow, ocount,
+                                               sc + 1);
+    }
   }
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
   MFCL_HIP_TRY(ctx, hipGetLastError());

@@ -102,6 +102,14 @@ def load_library() -> C.CDLL:
     lib.mfcl_aabbtree_query.argtypes = [p, p, p, u64, i32, p, p]
     lib.mfcl_aabbtree_query_dev.argtypes = [p, p, p, u64, i32, p, p, p]
     lib.mfcl_tune.argtypes = [p, i32, i32]
+    lib.mfcl_mlp_num_params.argtypes = []
+    lib.mfcl_mlp_init.argtypes = [p, u64]
+    lib.mfcl_mlp_set_params.argtypes = [p, p, i32]
+    lib.mfcl_mlp_get_params.argtypes = [p, p, p, i32]
+    lib.mfcl_mlp_features_dev.argtypes = [p, p, p, p, p, u64, p, p]
+    lib.mfcl_mlp_guess_dev.argtypes = [p, p, p, p, p, u64, p, p, p]
+    lib.mfcl_mlp_forward_dev.argtypes = [p, p, u64, p, p]
+    lib.mfcl_mlp_train_step_dev.argtypes = [p, p, p, u64, p, i32, p, p]
     _lib = lib
     return lib
 
@@ -289,6 +297,45 @@ class Context:
         ids = np.full((n, max(cap, 1)), -1, np.int64)
         self._check(self.lib.mfcl_aabbtree_query(self.h, _ptr(qmn), _ptr(qmx), n, cap, _ptr(counts), _ptr(ids)))
         return counts, ids
+
+
+    # ---- learned warm start
+    def mlp_num_params(self) -> int:
+        return int(self.lib.mfcl_mlp_num_params())
+
+    def mlp_init(self, seed: int):
+        self._check(self.lib.mfcl_mlp_init(self.h, int(seed)))
+
+    def mlp_set_params(self, params: np.ndarray):
+        params = np.ascontiguousarray(params, np.float32)
+        self._check(self.lib.mfcl_mlp_set_params(self.h, _ptr(params), len(params)))
+
+    def mlp_get_params(self, grads: bool = False):
+        n = self.mlp_num_params()
+        p = np.zeros(n, np.float32)
+        g = np.zeros(n, np.float32) if grads else None
+        self._check(self.lib.mfcl_mlp_get_params(self.h, _ptr(p), _ptr(g), n))
+        return (p, g) if grads else p
+
+    def mlp_features_dev(self, pa: DeviceArray, pb: DeviceArray, qa: DeviceArray, qb: DeviceArray, n: int,
+                         feat: DeviceArray):
+        self._check(self.lib.mfcl_mlp_features_dev(self.h, pa.ptr, pb.ptr, qa.ptr, qb.ptr, n, feat.ptr, None))
+
+    def mlp_guess_dev(self, pa: DeviceArray, pb: DeviceArray, qa: DeviceArray, qb: DeviceArray, n: int,
+                      guess: DeviceArray, dist: Optional[DeviceArray] = None):
+        self._check(self.lib.mfcl_mlp_guess_dev(self.h, pa.ptr, pb.ptr, qa.ptr, qb.ptr, n, guess.ptr,
+                                                None if dist is None else dist.ptr, None))
+
+    def mlp_forward_dev(self, feat: DeviceArray, n: int, out4: DeviceArray):
+        self._check(self.lib.mfcl_mlp_forward_dev(self.h, feat.ptr, n, out4.ptr, None))
+
+    def mlp_train_step_dev(self, feat: DeviceArray, target: DeviceArray, n: int, hyper=None,
+                           apply_update: bool = True) -> float:
+        loss = np.zeros(1, np.float32)
+        hp = None if hyper is None else np.ascontiguousarray(hyper, np.float32)
+        self._check(self.lib.mfcl_mlp_train_step_dev(self.h, feat.ptr, target.ptr, n, _ptr(hp),
+                                                     1 if apply_update else 0, _ptr(loss), None))
+        return float(loss[0])
 
 
 # Symbols the C header declares (checked by the CPU test suite).

@@ -1,0 +1,113 @@
+"""GPU: the warm-start MLP kernels against the numpy reference, and what
+the trained network gains over plain GJK.
+
+Bars:
+* forward outputs equal the FP64 reference evaluated on the same BF16
+  operands within 2e-3 (1 + |y|) (the device accumulates in FP32 and may
+  round an activation to the neighbouring BF16 value);
+* gradients of the loss: relative L2 error <= 2e-2 per parameter block and
+  cosine >= 0.999 against the reference's backward pass;
+* one Adam step moves every parameter by the closed form of a first Adam
+  step, -lr * g / (|g| + eps);
+* warm start: on held-out pairs the warm-started signed-distance query
+  returns the same intersecting / separated verdict as the plain query for
+  every pair outside the touching band (|d| <= 1e-6 scale), is as accurate
+  as the plain query against a tighter solve (within max(1e-5 |d|, 1e-6)
+  on at least as many pairs, +1e-4 slack), and needs fewer GJK iterations.
+"""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+
+BLOCKS = [("W1", R.OFF_W1, R.OFF_B1), ("b1", R.OFF_B1, R.OFF_W2), ("W2", R.OFF_W2, R.OFF_B2), ("b2", R.OFF_B2, R.OFF_W3),
+          ("W3", R.OFF_W3, R.OFF_B3), ("b3", R.OFF_B3, R.OFF_W4), ("W4", R.OFF_W4, R.OFF_B4), ("b4", R.OFF_B4, R.N_PARAMS)]
+
+
+def _problem(seed, n):
+    rng = np.random.default_rng(seed)
+    p = R.init_params(seed)
+    for _, lo, hi in BLOCKS[1::2]:
+        p[lo:hi] = rng.normal(scale=0.05, size=hi - lo)
+    p[R.OFF_B4 + 4:] = 0
+    p = p.astype(np.float32)
+    X = np.zeros((n, R.IN_PAD), np.float32)
+    X[:, :R.IN] = rng.normal(size=(n, R.IN)).astype(np.float32)
+    T = np.zeros((n, 4), np.float32)
+    d = rng.normal(size=(n, 3))
+    T[:, :3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    T[:, 3] = rng.normal(size=n)
+    T[::17, :3] = 0  # masked rows
+    return p, X, T
+
+
+@pytest.mark.gpu
+def test_mlp_forward_matches_reference(gpu_ctx):
+    p, X, _ = _problem(1, 1000)  # not a multiple of 32: exercises the padding rows
+    gpu_ctx.mlp_set_params(p)
+    dX = gpu_ctx.to_device(X)
+    dY = gpu_ctx.alloc(len(X) * 16)
+    gpu_ctx.mlp_forward_dev(dX, len(X), dY)
+    y = np.zeros((len(X), 4), np.float32)
+    dY.download(y)
+    ref, _ = R.forward(p.astype(np.float64), X.astype(np.float64))
+    err = np.abs(y - ref[:, :4]) / (1 + np.abs(ref[:, :4]))
+    print(f"\n[mlp forward] n={len(X)} max err {err.max():.3e} mean {err.mean():.3e}")
+    assert err.max() <= 2e-3
+
+
+@pytest.mark.gpu
+def test_mlp_gradients_match_reference(gpu_ctx):
+    p, X, T = _problem(2, 4096)
+    gpu_ctx.mlp_set_params(p)
+    dX, dT = gpu_ctx.to_device(X), gpu_ctx.to_device(T)
+    loss = gpu_ctx.mlp_train_step_dev(dX, dT, len(X), apply_update=False)
+    _, g = gpu_ctx.mlp_get_params(grads=True)
+    y, saved = R.forward(p.astype(np.float64), X.astype(np.float64))
+    ref_loss, dy = R.loss_grad(y, T.astype(np.float64))
+    gr = R.backward(p.astype(np.float64), saved, dy)
+    print(f"\n[mlp grad] loss gpu {loss:.6f} ref {ref_loss:.6f}")
+    assert abs(loss - ref_loss) <= 1e-3 * (1 + abs(ref_loss))
+    for name, lo, hi in BLOCKS:
+        a, b = g[lo:hi].astype(np.float64), gr[lo:hi]
+        nb = np.linalg.norm(b)
+        rel = np.linalg.norm(a - b) / max(nb, 1e-30)
+        cos = float(a @ b / max(np.linalg.norm(a) * nb, 1e-30))
+        print(f"  {name}: |g| {nb:.3e} rel L2 err {rel:.3e} cos {cos:.6f}")
+        if nb > 0:
+            assert rel <= 2e-2 and cos >= 0.999, name
+        else:
+            assert np.all(a == 0), name
+
+
+@pytest.mark.gpu
+def test_mlp_adam_first_step(gpu_ctx):
+    p, X, T = _problem(3, 2048)
+    gpu_ctx.mlp_set_params(p)
+    dX, dT = gpu_ctx.to_device(X), gpu_ctx.to_device(T)
+    gpu_ctx.mlp_train_step_dev(dX, dT, len(X), apply_update=False)
+    _, g = gpu_ctx.mlp_get_params(grads=True)
+    gpu_ctx.mlp_set_params(p)  # resets Adam
+    lr, eps = 1e-3, 1e-8
+    gpu_ctx.mlp_train_step_dev(dX, dT, len(X), hyper=[lr, 0.9, 0.999, eps, 0, 1], apply_update=True)
+    p1 = gpu_ctx.mlp_get_params()
+    expect = p.astype(np.float64) - lr * g / (np.abs(g) + eps)
+    assert np.allclose(p1, expect, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.gpu
+def test_warm_start_gain(gpu_ctx):
+    from mfcl import warmstart as WS
+
+    X, T = WS.make_dataset(gpu_ctx, 1 << 18, 4, chunk=1 << 18, log=lambda s: None)
+    gpu_ctx.mlp_init(11)
+    losses = WS.train(gpu_ctx, X, T, epochs=6, batch=16384, lr=2e-3, seed=3, log=lambda s: None)
+    res = WS.evaluate(gpu_ctx, 1 << 17, 5, steps=2, log=lambda s: None)
+    a = res["agreement"]
+    print(f"\n[warm start] losses {losses[0]:.4f} -> {losses[-1]:.4f}; gjk iters plain "
+          f"{res['plain']['gjk_iters_mean']:.3f} warm {res['warm']['gjk_iters_mean']:.3f}; "
+          f"speedup {res['speedup']:.3f}; agreement {a}")
+    assert losses[-1] < losses[0]
+    assert a["sign_mismatch_outside_band"] == 0 and a["ok_mismatch"] == 0
+    assert a["warm_frac_outside_tol"] <= a["plain_frac_outside_tol"] + 1e-4
+    assert res["warm"]["gjk_iters_mean"] < res["plain"]["gjk_iters_mean"]
