@@ -260,7 +260,7 @@ def bench_warm(ctx, args, h, dpa, dpb, dqa, dqb, dout, n, out_plain, value_plain
         kern.append(ctx.last_timing().copy())
     ctx.sync()
     el = time.perf_counter() - t0
-    kern = np.array(kern).mean(0)
+    kern = [float(k) for k in np.array(kern).mean(0)]
     out_w = np.zeros(n, mfcl.CONVEX_RESULT_DTYPE)
     dout.download(out_w)
     ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, mfcl.request(mfcl.MODE_SIGNED_DISTANCE, gjk_tolerance=1e-10,
@@ -278,8 +278,9 @@ def bench_warm(ctx, args, h, dpa, dpb, dqa, dqb, dout, n, out_plain, value_plain
            "kernel_ms": {"order": float(kern[0]), "gjk": float(kern[1]), "epa": float(kern[2]), "mlp": float(kern[5])},
            "gjk_iters_mean": {"plain": float(out_plain["gjk_iters"].mean()), "warm": float(out_w["gjk_iters"].mean())},
            "agreement": agree,
-           "mlp_roofline": {"bound": "mfma", "achieved": mlp_flop / (kern[5] * 1e-3) / 1e12, "peak": 2500.0,
-                            "unit": "TFLOP/s", "frac": mlp_flop / (kern[5] * 1e-3) / 1e12 / 2500.0},
+           "mlp_roofline": {"bound": "mfma", "achieved": mlp_flop / max(kern[5], 1e-9) / 1e9, "peak": 2500.0,
+                            "unit": "TFLOP/s", "frac": mlp_flop / max(kern[5], 1e-9) / 1e9 / 2500.0,
+                            "note": "features + 4 products + normalisation, per launch over all pairs"},
            "trained_on": {k: meta[k] for k in ("pairs", "seed", "epochs") if k in meta}}
     log(f"[warm] {json.dumps(res)}")
     dguess.free()

@@ -123,6 +123,8 @@ int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4);
 #define MFCL_TUNE_MESH_BLOCKS_PER_CU 3 /* persistent traversal blocks (4 wavefronts) per CU, 1..16 */
 #define MFCL_TUNE_MESH_STACK 4        /* node-pair stack entries per wavefront, 1024..262144 */
 #define MFCL_TUNE_EPA_LDS_FACES 5     /* face capacity of the LDS polytope, 16..128 */
+#define MFCL_TUNE_GUESS_POLICY 6      /* mfcl_mlp_guess_dev: 0 the MLP direction for every pair; 1 (default)
+                                         the default guess where the MLP predicts penetration */
 int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value);
 
 /* ---- device memory helpers (so callers need no HIP runtime) ------------- */

@@ -197,6 +197,10 @@ int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
       if (value < 16 || value > 128) break;
       ctx->tune.epa_lds_faces = value;
       return MFCL_OK;
+    case MFCL_TUNE_GUESS_POLICY:
+      if (value < 0 || value > 1) break;
+      ctx->tune.guess_policy = value;
+      return MFCL_OK;
     case MFCL_TUNE_MESH_STACK:
       if (value < 1024 || value > (1 << 18)) break;
       ctx->tune.mesh_stack = value;

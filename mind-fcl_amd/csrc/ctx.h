@@ -64,7 +64,8 @@ struct mfcl_ctx {
   // launch-shape knobs (set by mfcl_tune / the autotuner)
   struct {
     int gjk_group{32};          // lanes per pair in the GJK stage (16, 32, 64)
-    int epa_threads{32768};     // persistent EPA lanes (multiple of 256): 64 per wavefront
+    int epa_threads{131072};    // persistent EPA lanes (multiple of 256): 64 per wavefront
+    int guess_policy{1};        // warm start: 0 MLP direction for every pair, 1 not for predicted penetration
     int epa_lds_faces{64};      // face capacity of the LDS polytope (overflow reruns at full size)
     int epa_overflow_waves{512};   // persistent wavefronts (one per block) for overflow pairs
     int mesh_blocks_per_cu{4};  // persistent mesh-traversal blocks (4 waves) per CU

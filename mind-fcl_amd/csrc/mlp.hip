@@ -250,13 +250,15 @@ __global__ void castKernel(const float* W, int rows, int cols, __bf16* Wb, __bf1
 }
 
 // Output -> unit guess direction (shape a frame) and distance estimate.
-__global__ void guessKernel(const float* Y, uint64_t n, float* guess3, float* dist) {
+// policy 1: pairs predicted to penetrate keep the solver's default guess
+// (UnitX, gjk_solver-inl.h:77-78), whose final GJK simplex seeds EPA.
+__global__ void guessKernel(const float* Y, uint64_t n, int policy, float* guess3, float* dist) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* y = Y + i * kMlpOutPad;
   float nn = sqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
   float g0 = 1.f, g1 = 0.f, g2 = 0.f;
-  if (nn > 1e-20f) { g0 = y[0] / nn; g1 = y[1] / nn; g2 = y[2] / nn; }
+  if (nn > 1e-20f && !(policy == 1 && y[3] < 0.f)) { g0 = y[0] / nn; g1 = y[1] / nn; g2 = y[2] / nn; }
   guess3[3 * i] = g0;
   guess3[3 * i + 1] = g1;
   guess3[3 * i + 2] = g2;
@@ -494,7 +496,8 @@ int mlpGuess(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float*
   featureKernel<<<static_cast<unsigned>((np + kBlk - 1) / kBlk), kBlk, 0, s>>>(
       pa, pb, posea, poseb, ctx->d_desc, ctx->n_shapes, n, np, a.x, nullptr, nullptr);
   forward(ctx, a, np, false, s);
-  guessKernel<<<static_cast<unsigned>((n + kBlk - 1) / kBlk), kBlk, 0, s>>>(a.y, n, guess3, dist);
+  guessKernel<<<static_cast<unsigned>((n + kBlk - 1) / kBlk), kBlk, 0, s>>>(a.y, n, ctx->tune.guess_policy, guess3,
+                                                                          dist);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[9], s));
   MFCL_HIP_TRY(ctx, hipGetLastError());
   return MFCL_OK;

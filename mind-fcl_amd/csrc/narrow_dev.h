@@ -88,13 +88,18 @@ MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
 // first vertex attaining the FP64 maximum of dot(d, v) -- the reference's
 // naive scan result (convex-inl.h:133-149), which the reference's adjacency
 // walk also returns whenever that maximum is unique.
+// Out of line: the scan is the largest piece of code on the support path
+// and is reached from every support call site; one copy keeps the kernels'
+// instruction footprint small.  Returns the index, with bit 30 set when the
+// FP64 rescan ran.
 template <int G>
-MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
+__device__ __attribute__((noinline)) int hullArgmaxOOL(const float4* verts, const MFCL_LDS float* lv, int ls, int nv,
+                                                      float rbound, double dx, double dy, double dz, int lg) {
   int bi = 0x7fffffff;
   float best = -INFINITY, second = -INFINITY;
   {
 #pragma clang fp contract(fast)
-    const float fx = static_cast<float>(d.x), fy = static_cast<float>(d.y), fz = static_cast<float>(d.z);
+    const float fx = static_cast<float>(dx), fy = static_cast<float>(dy), fz = static_cast<float>(dz);
     auto visit = [&](int i, float x, float y, float z) {
       const float val = fx * x + fy * y + fz * z;
       if (val > best) {
@@ -105,11 +110,11 @@ MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
         second = val;
       }
     };
-    if (s.lv) {
-      for (int i = lg; i < s.nv; i += G) visit(i, s.lv[i], s.lv[i + s.ls], s.lv[i + 2 * s.ls]);
+    if (lv) {
+      for (int i = lg; i < nv; i += G) visit(i, lv[i], lv[i + ls], lv[i + 2 * ls]);
     } else {
-      for (int i = lg; i < s.nv; i += G) {
-        const float4 q = s.verts[i];
+      for (int i = lg; i < nv; i += G) {
+        const float4 q = verts[i];
         visit(i, q.x, q.y, q.z);
       }
     }
@@ -127,18 +132,22 @@ MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
       second = fmaxf(fmaxf(second, os), ob);
     }
   }
-  *scanned += s.nv;
   // |fl32(d.v) - d.v| <= (rounding of d + 3 roundings of the dot) * |d|_1 max|v|
   // <= 8 * 2^-24 * |d|_1 * rbound; the gap test uses twice that.
-  const float bound = 2.0f * 8.0f * 5.9604645e-08f * s.rbound *
-                      (fabsf(static_cast<float>(d.x)) + fabsf(static_cast<float>(d.y)) + fabsf(static_cast<float>(d.z)));
+  const float bound = 2.0f * 8.0f * 5.9604645e-08f * rbound *
+                      (fabsf(static_cast<float>(dx)) + fabsf(static_cast<float>(dy)) + fabsf(static_cast<float>(dz)));
   if (best - second > bound) return bi;
-  *scanned += s.nv;
   double bd = -INFINITY;
   bi = 0x7fffffff;
-  for (int i = lg; i < s.nv; i += G) {
-    const float3 q = hullVert(s, i);
-    const double val = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
+  for (int i = lg; i < nv; i += G) {
+    float x, y, z;
+    if (lv) {
+      x = lv[i]; y = lv[i + ls]; z = lv[i + 2 * ls];
+    } else {
+      const float4 q = verts[i];
+      x = q.x; y = q.y; z = q.z;
+    }
+    const double val = dx * static_cast<double>(x) + dy * static_cast<double>(y) + dz * static_cast<double>(z);
     if (val > bd) {
       bd = val;
       bi = i;
@@ -153,7 +162,14 @@ MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
       bi = oi;
     }
   }
-  return bi;
+  return bi | (1 << 30);
+}
+
+template <int G>
+MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
+  const int r = hullArgmaxOOL<G>(s.verts, s.lv, s.ls, s.nv, s.rbound, d.x, d.y, d.z, lg);
+  *scanned += (r >> 30) ? 2 * s.nv : s.nv;
+  return r & ((1 << 30) - 1);
 }
 
 // include/fcl/cvx_collide/gjk_shape.hpp:41-180; shape_gjk_interface-inl.h:87-92.
@@ -639,39 +655,35 @@ MFCL_D int distUpdate(Smp& s, d3& out, double tol) {
     smpApply(s, dist2(s.a.v, s.b.v, out, tol));
     return kUpdOK;
   }
-  if (s.rank == 3) {
-    smpApply(s, dist3(s.a.v, s.b.v, s.c.v, out, tol));
-    return kUpdOK;
+  if (s.rank != 3 && s.rank != 4) return kUpdFailed;
+  // rank 3: the triangle itself; rank 4 (gjk_distance.hpp:291-373): faces
+  // (3,2,1), (3,2,0), (3,1,0) = slots (b,c,e), (a,c,e), (a,b,e).  One loop
+  // body (a single inlined copy of the triangle case) with the inputs picked
+  // by value selects.
+  MV A = s.a, B = s.b, Cc = s.c, E = s.e;
+  opaque(A); opaque(B); opaque(Cc); opaque(E);
+  const bool r4 = s.rank == 4;
+  const int nf = r4 ? 3 : 1;
+  double best = -1;
+  d3 bpt = {0, 0, 0};
+  int bc = 0;
+#pragma nounroll
+  for (int f = 0; f < nf; ++f) {
+    const MV x = r4 ? mvSel(f == 0, B, A) : A;
+    const MV y = r4 ? mvSel(f == 2, B, Cc) : B;
+    const MV z = r4 ? E : Cc;
+    const int m0 = r4 ? (f == 0 ? 1 : 0) : 0;
+    const int m1 = r4 ? (f == 2 ? 1 : 2) : 1;
+    const int m2 = r4 ? 3 : 2;
+    d3 pt;
+    const int c = codeMap(dist3(x.v, y.v, z.v, pt, tol), m0, m1, m2);
+    const double dd = sqnorm(pt);
+    if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
   }
-  if (s.rank == 4) {
-    // gjk_distance.hpp:291-373: faces (3,2,1), (3,2,0), (3,1,0)
-    double best = -1;
-    d3 bpt = {0, 0, 0};
-    int bc = 0;
-    {
-      d3 pt;
-      const int c = codeMap(dist3(s.b.v, s.c.v, s.e.v, pt, tol), 1, 2, 3);
-      const double dd = sqnorm(pt);
-      if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
-    }
-    {
-      d3 pt;
-      const int c = codeMap(dist3(s.a.v, s.c.v, s.e.v, pt, tol), 0, 2, 3);
-      const double dd = sqnorm(pt);
-      if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
-    }
-    {
-      d3 pt;
-      const int c = codeMap(dist3(s.a.v, s.b.v, s.e.v, pt, tol), 0, 1, 3);
-      const double dd = sqnorm(pt);
-      if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
-    }
-    if (best < 0) return kNoImprovement;
-    smpApply(s, bc);
-    out = bpt;
-    return kUpdOK;
-  }
-  return kUpdFailed;
+  if (best < 0) return kNoImprovement;
+  smpApply(s, bc);
+  out = bpt;
+  return kUpdOK;
 }
 
 constexpr double kBaryTol = 1e-3;  // gjk.h:125
@@ -737,18 +749,22 @@ MFCL_D bool extractTrySub(MK& md, const Smp& s, d3& p0, d3& p1) {
     extractNoSub(md, s, p0, p1);
     return true;
   }
+  // candidates (gjk_distance.hpp:488-559): triangle (a,b,c) when its
+  // extraction succeeds, then segments (a,b), (a,c), (b,c); one loop body
   double best = -1;
-  d3 bp0 = {0, 0, 0}, bp1 = {0, 0, 0}, q0, q1;
-  if (extractNoSub(md, 3, s.a, s.b, s.c, q0, q1)) {
+  d3 bp0 = {0, 0, 0}, bp1 = {0, 0, 0};
+  MV A = s.a, B = s.b, Cc = s.c;
+  opaque(A); opaque(B); opaque(Cc);
+#pragma nounroll
+  for (int k = 0; k < 4; ++k) {
+    const MV v1 = mvSel(k == 3, B, A);
+    const MV v2 = mvSel(k <= 1, B, Cc);
+    d3 q0, q1;
+    const bool ok = extractNoSub(md, k == 0 ? 3 : 2, v1, v2, Cc, q0, q1);
+    if (k == 0 && !ok) continue;
     const double dd = sqnorm(q0 - q1);
     if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; }
   }
-  extractNoSub(md, 2, s.a, s.b, s.b, q0, q1);
-  { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
-  extractNoSub(md, 2, s.a, s.c, s.c, q0, q1);
-  { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
-  extractNoSub(md, 2, s.b, s.c, s.c, q0, q1);
-  { const double dd = sqnorm(q0 - q1); if (best < 0 || dd < best) { best = dd; bp0 = q0; bp1 = q1; } }
   p0 = bp0;
   p1 = bp1;
   return true;
@@ -761,27 +777,32 @@ MFCL_D bool gjkMinDistance(MK& md, Smp& s, double tol, int max_iter, d3& p0, d3&
   d3 cur;
   if (distUpdate(s, cur, tol) != kUpdOK) return false;
   double book = norm(cur);
-  if (book <= tol) return extractNoSub(md, s, p0, p1);
-  d3 nd = -cur / book;
-  const double tol2 = tol * tol;
-  for (int it = 0; it < max_iter; ++it) {
-    (*iters)++;
-    MV nv;
-    nv.id = freeSlot(s);
-    nv.v = md.supportNew(nd, nv.id);
-    if (dot(nd, nv.v - cur) < tol) return extractNoSub(md, s, p0, p1);
-    if (smpDup(s, nv.v, tol2)) return extractNoSub(md, s, p0, p1);
-    smpAdd(s, nv);
-    const int st = distUpdate(s, cur, tol);
-    if (st == kNoImprovement) return extractNoSub(md, s, p0, p1);
-    if (st != kUpdOK) return false;
-    const double nm = norm(cur);
-    if (book - nm < tol) return extractNoSub(md, s, p0, p1);
-    if (nm < tol) return extractNoSub(md, s, p0, p1);
-    book = nm;
-    nd = -cur / book;
+  // every successful exit extracts the witnesses from the current simplex:
+  // one exit point, so one inlined copy of the extraction
+  bool extract = book <= tol;
+  if (!extract) {
+    d3 nd = -cur / book;
+    const double tol2 = tol * tol;
+    for (int it = 0; it < max_iter; ++it) {
+      (*iters)++;
+      MV nv;
+      nv.id = freeSlot(s);
+      nv.v = md.supportNew(nd, nv.id);
+      if (dot(nd, nv.v - cur) < tol) { extract = true; break; }
+      if (smpDup(s, nv.v, tol2)) { extract = true; break; }
+      smpAdd(s, nv);
+      const int st = distUpdate(s, cur, tol);
+      if (st == kNoImprovement) { extract = true; break; }
+      if (st != kUpdOK) return false;
+      const double nm = norm(cur);
+      if (book - nm < tol) { extract = true; break; }
+      if (nm < tol) { extract = true; break; }
+      book = nm;
+      nd = -cur / book;
+    }
+    if (!extract) return false;
   }
-  return false;
+  return extractNoSub(md, s, p0, p1);
 }
 
 struct GjkOut {

@@ -19,6 +19,8 @@ GJK_GROUP = 1
 EPA_THREADS = 2
 MESH_BLOCKS_PER_CU = 3
 MESH_STACK = 4
+EPA_LDS_FACES = 5
+GUESS_POLICY = 6
 
 
 def set_knob(ctx, knob: int, value: int) -> None:

@@ -216,10 +216,13 @@ def evaluate(ctx: Context, n: int, seed: int, steps: int = 5, log=print):
         dt = (time.perf_counter() - t0) / steps
         out = ps.results()
         k = np.array(kern).mean(0)
+        sep = out["distance"] > 0
         res["warm" if warm else "plain"] = dict(
             ms_per_step=1e3 * dt, pairs_per_s=n / dt, out=out, kernel_ms=dict(order=float(k[0]), gjk=float(k[1]),
                                                                            epa=float(k[2]), mlp=float(k[5])),
             gjk_iters_mean=float(out["gjk_iters"].mean()),
+            gjk_iters_separated=float(out["gjk_iters"][sep].mean()) if sep.any() else 0.0,
+            gjk_iters_intersecting=float(out["gjk_iters"][~sep].mean()) if (~sep).any() else 0.0,
             epa_iters_mean=float(out["epa_iters"][out["epa_status"] >= 0].mean()) if (out["epa_status"] >= 0).any()
             else 0.0)
     r = W.bounding_radius(ps.h["types"], ps.h["params"], ps.h["vert_cnt"], ps.h["verts"], ps.h["vert_off"])
