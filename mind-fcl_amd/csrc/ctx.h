@@ -48,6 +48,7 @@ struct MlpState {
   __bf16 *w1{nullptr}, *w2{nullptr}, *w3{nullptr}, *w4{nullptr};
   __bf16 *w1t{nullptr}, *w2t{nullptr}, *w3t{nullptr}, *w4t{nullptr};
   DevBuf ws;               // activations
+  void* pack{nullptr};     // fused-inference weight image (BF16, K-permuted) + biases
   int step{0};
 };
 

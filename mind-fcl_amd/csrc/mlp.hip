@@ -25,6 +25,8 @@
 
 namespace mfcl {
 
+#define MFCL_LDS __attribute__((address_space(3)))
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
@@ -265,6 +267,166 @@ __global__ void guessKernel(const float* Y, uint64_t n, int policy, float* guess
   if (dist) dist[i] = y[3];
 }
 
+// ------------------------------------------------------- fused inference
+// Features, the four layers and the normalisation in one kernel: weights
+// live in LDS (copied once per block from a packed image), and each
+// wavefront carries 32 pairs through the network with the activations kept
+// in registers.  Formulation: C = W . X^T (rows = units, columns = pairs),
+// so a layer's accumulator tile holds, in lane l, pair l % 32 and the 16
+// units 8 (r / 4) + 4 (l / 32) + r % 4 of the tile; the next layer takes
+// those registers directly as its B operand (8 per K step) by contracting
+// over the units in that order -- the packed weights are laid out with the
+// same permutation of K, so no transpose through LDS is needed.
+//
+// Packed image (bf16x8 per lane): [layer][out tile][K step][lane], with
+//   element i of lane l = W[32 t + l % 32][k(j, l / 32, i)],
+//   layer 1: k = 16 j + 8 h + i (features in natural order),
+//   layers 2-4: k = 32 (j / 2) + 16 (j % 2) + 8 (i / 4) + 4 h + i % 4.
+constexpr int kPackL1 = 4 * 4, kPackL2 = 4 * 8, kPackL3 = 4 * 8, kPackL4 = 1 * 8;  // (tile, K step) blocks
+constexpr int kPackBlocks = kPackL1 + kPackL2 + kPackL3 + kPackL4;
+constexpr int kPackBytes = kPackBlocks * 64 * 16;
+constexpr int kPackBiasFloats = 3 * kHid + kMlpOutPad;
+
+__device__ __forceinline__ int permK(int j, int h, int i) { return 32 * (j / 2) + 16 * (j % 2) + 8 * (i / 4) + 4 * h + (i % 4); }
+
+__global__ void packKernel(const float* P, __bf16* pack, float* bias) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // one (block, lane, element)
+  if (idx < kPackBlocks * 64 * 8) {
+    const int e = idx % 8, l = (idx / 8) % 64, b = idx / (8 * 64);
+    const int h = l / 32, m = l % 32;
+    float v;
+    if (b < kPackL1) {
+      const int t = b / 4, j = b % 4;
+      v = P[kOffW1 + (32 * t + m) * kMlpInPad + 16 * j + 8 * h + e];
+    } else if (b < kPackL1 + kPackL2) {
+      const int q = b - kPackL1, t = q / 8, j = q % 8;
+      v = P[kOffW2 + (32 * t + m) * kHid + permK(j, h, e)];
+    } else if (b < kPackL1 + kPackL2 + kPackL3) {
+      const int q = b - kPackL1 - kPackL2, t = q / 8, j = q % 8;
+      v = P[kOffW3 + (32 * t + m) * kHid + permK(j, h, e)];
+    } else {
+      const int j = b - kPackL1 - kPackL2 - kPackL3;
+      v = P[kOffW4 + m * kHid + permK(j, h, e)];
+    }
+    pack[idx] = static_cast<__bf16>(v);
+  }
+  if (idx < kPackBiasFloats) {
+    const int off = idx < kHid ? kOffB1 + idx
+                  : idx < 2 * kHid ? kOffB2 + idx - kHid
+                  : idx < 3 * kHid ? kOffB3 + idx - 2 * kHid : kOffB4 + idx - 3 * kHid;
+    bias[idx] = P[off];
+  }
+}
+
+// one hidden layer: 4 output tiles, KS K steps, B operands in x[KS]
+template <int KS>
+__device__ __forceinline__ void denseGelu(const MFCL_LDS bf16x8* W, const bf16x8* x, const float* bias, int lane,
+                                          bf16x8* out) {
+  const int h = lane >> 5;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int j = 0; j < KS; ++j) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[(t * KS + j) * 64 + lane], x[j], acc, 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      bf16x8 o;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int r = 8 * s + i;
+        const int unit = 32 * t + 8 * (r / 4) + 4 * h + (r % 4);
+        o[i] = static_cast<__bf16>(geluF(acc[r] + bias[unit]));
+      }
+      out[2 * t + s] = o;
+    }
+    // keep the scheduler from overlapping the tiles (register pressure)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+__global__ void __launch_bounds__(kBlk) mlpGuessFused(const uint32_t* pa, const uint32_t* pb, const float* posea,
+                                                      const float* poseb, const float* desc, uint32_t n_shapes,
+                                                      uint64_t n, const uint4* pack_g, const float* bias_g, int policy,
+                                                      float* guess3, float* dist) {
+  extern __shared__ __align__(16) uint4 lds_pack[];
+  for (int k = threadIdx.x; k < kPackBytes / 16; k += blockDim.x) lds_pack[k] = pack_g[k];
+  float* bias = reinterpret_cast<float*>(lds_pack + kPackBytes / 16);
+  for (int k = threadIdx.x; k < kPackBiasFloats; k += blockDim.x) bias[k] = bias_g[k];
+  __syncthreads();
+  const MFCL_LDS bf16x8* W = (const MFCL_LDS bf16x8*)(lds_pack);
+  const MFCL_LDS bf16x8* W1 = W;
+  const MFCL_LDS bf16x8* W2 = W1 + kPackL1 * 64;
+  const MFCL_LDS bf16x8* W3 = W2 + kPackL2 * 64;
+  const MFCL_LDS bf16x8* W4 = W3 + kPackL3 * 64;
+  const int lane = threadIdx.x & 63, h = lane >> 5;
+  const uint64_t ntiles = (n + 31) / 32;
+  const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t tile = wave; tile < ntiles; tile += nwaves) {
+    const uint64_t i = tile * 32 + (lane & 31);
+    // features (the same formulas as featureKernel); lanes h = 0 / 1 keep
+    // columns 16 j + 0..7 / 16 j + 8..15
+    float t0 = 0, t1 = 0, t2 = 0, r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    float4 da[4] = {}, db[4] = {};
+    if (i < n && pa[i] < n_shapes && pb[i] < n_shapes) {
+      const float* A = posea + 7 * i;
+      const float* B = poseb + 7 * i;
+      float qa[4] = {A[3], A[4], A[5], A[6]}, qb[4] = {B[3], B[4], B[5], B[6]};
+      const float na = rsqrtf(qa[0] * qa[0] + qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]);
+      const float nb = rsqrtf(qb[0] * qb[0] + qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) { qa[k] *= na; qb[k] *= nb; }
+      const float dx = B[0] - A[0], dy = B[1] - A[1], dz = B[2] - A[2];
+      const float w = qa[0], x = -qa[1], y = -qa[2], z = -qa[3];
+      const float tx = 2.f * (y * dz - z * dy), ty = 2.f * (z * dx - x * dz), tz = 2.f * (x * dy - y * dx);
+      t0 = dx + w * tx + (y * tz - z * ty);
+      t1 = dy + w * ty + (z * tx - x * tz);
+      t2 = dz + w * tz + (x * ty - y * tx);
+      r0 = w * qb[0] - x * qb[1] - y * qb[2] - z * qb[3];
+      r1 = w * qb[1] + x * qb[0] + y * qb[3] - z * qb[2];
+      r2 = w * qb[2] - x * qb[3] + y * qb[0] + z * qb[1];
+      r3 = w * qb[3] + x * qb[2] - y * qb[1] + z * qb[0];
+      if (r0 < 0) { r0 = -r0; r1 = -r1; r2 = -r2; r3 = -r3; }
+      const float4* pda = reinterpret_cast<const float4*>(desc + 16 * static_cast<size_t>(pa[i]));
+      const float4* pdb = reinterpret_cast<const float4*>(desc + 16 * static_cast<size_t>(pb[i]));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { da[q] = pda[q]; db[q] = pdb[q]; }
+    }
+    // feature k: 0-2 t, 3-6 r, 7-22 da, 23-38 db, 39-63 zero
+    auto feat = [&](int k) -> float {
+      if (k < 3) return k == 0 ? t0 : (k == 1 ? t1 : t2);
+      if (k < 7) return k == 3 ? r0 : (k == 4 ? r1 : (k == 5 ? r2 : r3));
+      if (k < 23) { const float4 v = da[(k - 7) / 4]; const int c = (k - 7) % 4; return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w)); }
+      if (k < 39) { const float4 v = db[(k - 23) / 4]; const int c = (k - 23) % 4; return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w)); }
+      return 0.f;
+    };
+    bf16x8 x0[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) x0[j][e] = static_cast<__bf16>(h ? feat(16 * j + 8 + e) : feat(16 * j + e));
+    bf16x8 a1[8], a2[8];
+    denseGelu<4>(W1, x0, bias, lane, a1);
+    denseGelu<8>(W2, a1, bias + kHid, lane, a2);
+    denseGelu<8>(W3, a2, bias + 2 * kHid, lane, a1);
+    f32x16 acc = {};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W4[j * 64 + lane], a1[j], acc, 0, 0, 0);
+    // outputs 0..3 of pair lane % 32 are rows 0..3: registers 0..3 of the lanes with h = 0
+    if (h == 0 && i < n) {
+      const float* b4 = bias + 3 * kHid;
+      const float y0 = acc[0] + b4[0], y1 = acc[1] + b4[1], y2 = acc[2] + b4[2], y3 = acc[3] + b4[3];
+      const float nn = sqrtf(y0 * y0 + y1 * y1 + y2 * y2);
+      float g0 = 1.f, g1 = 0.f, g2 = 0.f;
+      if (nn > 1e-20f && !(policy == 1 && y3 < 0.f)) { g0 = y0 / nn; g1 = y1 / nn; g2 = y2 / nn; }
+      guess3[3 * i] = g0;
+      guess3[3 * i + 1] = g1;
+      guess3[3 * i + 2] = g2;
+      if (dist) dist[i] = y3;
+    }
+  }
+}
+
 // ------------------------------------------------------------- host side
 namespace {
 
@@ -280,6 +442,9 @@ int ensureMlp(mfcl_ctx* ctx) {
 void castWeights(mfcl_ctx* ctx, hipStream_t s) {
   auto& M = ctx->mlp;
   const int T = 256;
+  packKernel<<<(kPackBlocks * 64 * 8 + T - 1) / T, T, 0, s>>>(M.params, reinterpret_cast<__bf16*>(M.pack),
+                                                               reinterpret_cast<float*>(static_cast<char*>(M.pack) +
+                                                                                        kPackBytes));
   castKernel<<<(kHid * kMlpInPad + T - 1) / T, T, 0, s>>>(M.params + kOffW1, kHid, kMlpInPad, M.w1, M.w1t);
   castKernel<<<(kHid * kHid + T - 1) / T, T, 0, s>>>(M.params + kOffW2, kHid, kHid, M.w2, M.w2t);
   castKernel<<<(kHid * kHid + T - 1) / T, T, 0, s>>>(M.params + kOffW3, kHid, kHid, M.w3, M.w3t);
@@ -429,6 +594,7 @@ int mlpSetParams(mfcl_ctx* ctx, const float* host, int n, bool reset_opt) {
     MFCL_HIP_TRY(ctx, hipMalloc(&M.m, fb));
     MFCL_HIP_TRY(ctx, hipMalloc(&M.v, fb));
     MFCL_HIP_TRY(ctx, hipMalloc(&M.loss, 256));
+    MFCL_HIP_TRY(ctx, hipMalloc(&M.pack, kPackBytes + kPackBiasFloats * 4));
     const size_t wb = (static_cast<size_t>(kHid) * kMlpInPad + 2 * kHid * kHid + kMlpOutPad * kHid) * 2;
     char* w = nullptr;
     MFCL_HIP_TRY(ctx, hipMalloc(&w, 2 * wb));
@@ -489,15 +655,15 @@ int mlpGuess(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float*
     return MFCL_E_STATE;
   }
   if (n == 0) return MFCL_OK;
-  const uint64_t np = pad32(n);
-  Acts a;
-  if (int rc = carveActs(ctx, np, false, a)) return rc;
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[8], s));
-  featureKernel<<<static_cast<unsigned>((np + kBlk - 1) / kBlk), kBlk, 0, s>>>(
-      pa, pb, posea, poseb, ctx->d_desc, ctx->n_shapes, n, np, a.x, nullptr, nullptr);
-  forward(ctx, a, np, false, s);
-  guessKernel<<<static_cast<unsigned>((n + kBlk - 1) / kBlk), kBlk, 0, s>>>(a.y, n, ctx->tune.guess_policy, guess3,
-                                                                          dist);
+  const size_t lds = kPackBytes + kPackBiasFloats * 4;
+  const uint64_t ntiles = (n + 31) / 32;
+  const uint64_t want = (ntiles + 3) / 4;
+  const unsigned grid = static_cast<unsigned>(want < 1024 ? want : 1024);  // persistent: 4 waves per block
+  mlpGuessFused<<<grid, kBlk, lds, s>>>(pa, pb, posea, poseb, ctx->d_desc, ctx->n_shapes, n,
+                                        static_cast<const uint4*>(ctx->mlp.pack),
+                                        reinterpret_cast<const float*>(static_cast<char*>(ctx->mlp.pack) + kPackBytes),
+                                        ctx->tune.guess_policy, guess3, dist);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[9], s));
   MFCL_HIP_TRY(ctx, hipGetLastError());
   return MFCL_OK;
@@ -551,7 +717,7 @@ int mlpTrainStep(mfcl_ctx* ctx, const float* xf, const float* target4, uint64_t 
 void mlpFree(mfcl_ctx* ctx) {
   auto& M = ctx->mlp;
   for (void* p : {static_cast<void*>(M.params), static_cast<void*>(M.grad), static_cast<void*>(M.m),
-                  static_cast<void*>(M.v), static_cast<void*>(M.loss), static_cast<void*>(M.wbuf), M.ws.p})
+                  static_cast<void*>(M.v), static_cast<void*>(M.loss), static_cast<void*>(M.wbuf), M.ws.p, M.pack})
     if (p) (void)hipFree(p);
   M = MlpState{};
 }

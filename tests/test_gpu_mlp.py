@@ -111,3 +111,44 @@ def test_warm_start_gain(gpu_ctx):
     assert a["sign_mismatch_outside_band"] == 0 and a["ok_mismatch"] == 0
     assert a["warm_frac_outside_tol"] <= a["plain_frac_outside_tol"] + 1e-4
     assert res["warm"]["gjk_iters_mean"] < res["plain"]["gjk_iters_mean"]
+
+
+@pytest.mark.gpu
+def test_fused_guess_matches_layered_forward(gpu_ctx):
+    """The fused inference kernel (features + 4 layers + normalisation,
+    K-permuted LDS weights) agrees with the layered forward pass and with
+    the numpy reference on the same features."""
+    from mfcl import autotune
+    from mfcl import workloads as W
+
+    h = W.hull_pairs(3000, 9, nmin=8, nmax=64)  # not a multiple of 32 pairs
+    gpu_ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"])
+    p, _, _ = _problem(4, 1)
+    gpu_ctx.mlp_set_params(p)
+    n = len(h["pair_a"])
+    dpa, dpb = gpu_ctx.to_device(h["pair_a"]), gpu_ctx.to_device(h["pair_b"])
+    dqa, dqb = gpu_ctx.to_device(h["pose_a"]), gpu_ctx.to_device(h["pose_b"])
+    dfeat, dy = gpu_ctx.alloc(n * 64 * 4), gpu_ctx.alloc(n * 16)
+    dg, dd = gpu_ctx.alloc(n * 12), gpu_ctx.alloc(n * 4)
+    gpu_ctx.mlp_features_dev(dpa, dpb, dqa, dqb, n, dfeat)
+    gpu_ctx.mlp_forward_dev(dfeat, n, dy)
+    autotune.set_knob(gpu_ctx, autotune.GUESS_POLICY, 0)
+    gpu_ctx.mlp_guess_dev(dpa, dpb, dqa, dqb, n, dg, dd)
+    autotune.set_knob(gpu_ctx, autotune.GUESS_POLICY, 1)
+    feat = np.zeros((n, 64), np.float32)
+    y = np.zeros((n, 4), np.float32)
+    g = np.zeros((n, 3), np.float32)
+    d = np.zeros(n, np.float32)
+    dfeat.download(feat)
+    dy.download(y)
+    dg.download(g)
+    dd.download(d)
+    ref, _ = R.forward(p.astype(np.float64), feat.astype(np.float64))
+    yd = y[:, :3] / np.linalg.norm(y[:, :3], axis=1, keepdims=True)
+    cos = (g * yd).sum(1)
+    err_d = np.abs(d - y[:, 3]) / (1 + np.abs(y[:, 3]))
+    err_ref = np.abs(d - ref[:, 3]) / (1 + np.abs(ref[:, 3]))
+    print(f"\n[fused guess] n={n} min cos {cos.min():.6f} dist err vs layered {err_d.max():.2e} vs ref {err_ref.max():.2e}")
+    assert np.all(np.abs(np.linalg.norm(g, axis=1) - 1) < 1e-5)
+    assert cos.min() >= 0.999
+    assert err_d.max() <= 1e-2 and err_ref.max() <= 1e-2
