@@ -147,7 +147,7 @@ def main():
         def run():
             ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, req, dout)
 
-        t_group = autotune.tune(ctx, autotune.GJK_GROUP, [16, 32, 64], run, reps=2, log=log)
+        t_group = autotune.tune(ctx, autotune.GJK_GROUP, [1, 16, 32, 64], run, reps=2, log=log)
         t_epa = autotune.tune(ctx, autotune.EPA_THREADS, [8192, 32768, 131072], run, reps=2, log=log)
         tuned = {"gjk_group": min(t_group, key=t_group.get), "epa_threads": min(t_epa, key=t_epa.get)}
     ctx.set_timing(True)

@@ -118,7 +118,8 @@ int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4);
 
 /* Launch-shape knobs (no counterpart in the reference: GPU scheduling
  * parameters, chosen by the autotuner in mind-fcl_amd/mfcl/autotune.py). */
-#define MFCL_TUNE_GJK_GROUP 1         /* lanes per pair in the GJK stage: 16, 32, 64 */
+#define MFCL_TUNE_GJK_GROUP 1         /* lanes per pair in the GJK stage: 1 (penetration and signed-distance
+                                         modes; cooperative hull scans), 16, 32, 64 */
 #define MFCL_TUNE_EPA_THREADS 2       /* persistent EPA lanes (64 per pair), multiple of 256 */
 #define MFCL_TUNE_MESH_BLOCKS_PER_CU 3 /* persistent traversal blocks (4 wavefronts) per CU, 1..16 */
 #define MFCL_TUNE_MESH_STACK 4        /* node-pair stack entries per wavefront, 1024..262144 */

@@ -70,6 +70,7 @@ void freeAll(mfcl_ctx* c) {
   freePtr(c->ws_misc.p);
   freePtr(c->ws_io.p);
   freePtr(c->ws_oflow.p);
+  freePtr(c->ws_lane.p);
   mlpFree(c);
 }
 
@@ -182,7 +183,7 @@ int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
   if (!ctx) return MFCL_E_INVALID;
   switch (knob) {
     case MFCL_TUNE_GJK_GROUP:
-      if (value != 16 && value != 32 && value != 64) break;
+      if (value != 1 && value != 16 && value != 32 && value != 64) break;
       ctx->tune.gjk_group = value;
       return MFCL_OK;
     case MFCL_TUNE_EPA_THREADS:

@@ -1,0 +1,445 @@
+// GJK with one pair per lane and wavefront-cooperative hull scans
+// (GJKSolver::shapeSignedDistance / shapeIntersect semantics,
+// gjk_solver-inl.h:93-155 and :820-874; GJK gjk.hpp:11-140, distance
+// gjk_distance.hpp:11-105).
+//
+// The FP64 simplex logic of GJK is scalar per pair, so it runs with one pair
+// per lane: the 64 pairs of a wavefront share every FP64 instruction it
+// issues.  The support searches over convex hulls are the data-parallel
+// part; they are served cooperatively -- whenever lanes need a support
+// point, the wavefront scans each requesting lane's hull in turn with all
+// 64 lanes (coalesced loads, a butterfly argmax, the same FP32 scan with a
+// rounding bound and FP64 rescan as the group kernel), so every support
+// point is the same vertex.  The GJK loop is a state machine so that all
+// lanes meet at the one support stage:
+//   support stage (converged) -> advance every lane by one step -> ...
+// Primitive supports are closed forms evaluated by the requesting lane.
+#pragma once
+
+#include "narrow_dev.h"
+
+namespace mfcl {
+
+// A simplex holds at most 4 vertices, so freeSlot() returns at most 4.
+constexpr int kLaneSlots = 5;
+
+// Per-wavefront LDS block ([field][lane], 8.5 KB): the relative transform
+// of each lane's pair and the hull
+// vertex indices of the support points recorded in each direction slot
+// (-1 for a primitive).
+struct LaneWs {
+  double tf[12][64];  // R row-major (9), t (3)
+  int i0[kLaneSlots][64], i1[kLaneSlots][64];
+};
+// The directions themselves live in device memory, [slot][xyz][lane] per
+// wavefront: they are written once per support point and read only for
+// primitive witnesses and for the EPA queue.
+constexpr int kLaneDirDoubles = kLaneSlots * 3 * 64;
+
+// The Minkowski difference of one lane's pair.  Only the two shape ids and
+// types stay in registers; shape parameters are re-read from the shape
+// tables and the transform from LDS where needed, which keeps the long-lived
+// register state of the lane machine small (occupancy hides the latency of
+// the hull scans).  The rotation into frame 1 (toshape1 = R_b^T R_a) is the
+// exact transpose of t0.R = R_a^T R_b (the same products summed in the same
+// order), so it is applied as tmul(t0.R, .).
+struct MinkLane {
+  ShapeTables T;
+  uint32_t ia, ib;
+  int type0, type1;
+  MFCL_LDS LaneWs* w;
+  double* dirs;
+  int lane;
+  // reloadFence(): re-read at every use; hoisted out of the lane machine's
+  // loop these would occupy registers for its whole duration
+  MFCL_D tf3 tf0() const {
+    reloadFence();
+    tf3 t;
+    t.R.m00 = w->tf[0][lane]; t.R.m01 = w->tf[1][lane]; t.R.m02 = w->tf[2][lane];
+    t.R.m10 = w->tf[3][lane]; t.R.m11 = w->tf[4][lane]; t.R.m12 = w->tf[5][lane];
+    t.R.m20 = w->tf[6][lane]; t.R.m21 = w->tf[7][lane]; t.R.m22 = w->tf[8][lane];
+    t.t = d3{w->tf[9][lane], w->tf[10][lane], w->tf[11][lane]};
+    return t;
+  }
+  MFCL_D void setTf0(const tf3& t) const {
+    w->tf[0][lane] = t.R.m00; w->tf[1][lane] = t.R.m01; w->tf[2][lane] = t.R.m02;
+    w->tf[3][lane] = t.R.m10; w->tf[4][lane] = t.R.m11; w->tf[5][lane] = t.R.m12;
+    w->tf[6][lane] = t.R.m20; w->tf[7][lane] = t.R.m21; w->tf[8][lane] = t.R.m22;
+    w->tf[9][lane] = t.t.x; w->tf[10][lane] = t.t.y; w->tf[11][lane] = t.t.z;
+  }
+  MFCL_D DevShape prim(int k) const {
+    reloadFence();
+    const float4 p = T.param[k ? ib : ia];
+    DevShape s;
+    s.type = k ? type1 : type0;
+    s.p0 = p.x; s.p1 = p.y; s.p2 = p.z;
+    s.verts = nullptr; s.nv = 0; s.rbound = p.w;
+    s.lv = nullptr; s.ls = 0;
+    return s;
+  }
+  MFCL_D const float4* verts(int k) const {
+    reloadFence();
+    return T.verts + T.vert_off[k ? ib : ia]; }
+  MFCL_D void setDir(int id, d3 d) const {
+    dirs[(3 * id + 0) * 64 + lane] = d.x;
+    dirs[(3 * id + 1) * 64 + lane] = d.y;
+    dirs[(3 * id + 2) * 64 + lane] = d.z;
+  }
+  MFCL_D d3 dirOf(int id) const {
+    return d3{dirs[(3 * id + 0) * 64 + lane], dirs[(3 * id + 1) * 64 + lane], dirs[(3 * id + 2) * 64 + lane]};
+  }
+  // support0(dir of slot id), from the recorded hull vertex when there is one
+  MFCL_D d3 point0(int id) const {
+    const int i0 = w->i0[id][lane];
+    if (i0 >= 0) {
+      const float4 q = verts(0)[i0];
+      return d3{q.x, q.y, q.z};
+    }
+    return primSupport(prim(0), dirOf(id));
+  }
+  // support1(-dir of slot id), likewise
+  MFCL_D d3 point1(int id) const {
+    const int i1 = w->i1[id][lane];
+    const tf3 t0 = tf0();
+    if (i1 >= 0) {
+      const float4 q = verts(1)[i1];
+      return apply(t0, d3{q.x, q.y, q.z});
+    }
+    return apply(t0, primSupport(prim(1), tmul(t0.R, -dirOf(id))));
+  }
+};
+
+MFCL_D int readLaneI(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+MFCL_D uint64_t readLaneU64(uint64_t v, int src) {
+  const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(v), src);
+  const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(v >> 32), src);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+MFCL_D double readLaneD(double v, int src) {
+  return __longlong_as_double(static_cast<long long>(readLaneU64(static_cast<uint64_t>(__double_as_longlong(v)), src)));
+}
+
+// FP64 pass of hullArgmaxOOL<64> (first index attaining the FP64 maximum of
+// dot(d, v)), inline: a call inside the lane machine's loop would make every
+// value live across it occupy callee-saved registers.
+MFCL_D int waveRescan64(const float4* verts, int nv, double dx, double dy, double dz, int lane) {
+  double bd = -INFINITY;
+  int bi = 0x7fffffff;
+  for (int i = lane; i < nv; i += 64) {
+    const float4 q = verts[i];
+    const double val = dx * static_cast<double>(q.x) + dy * static_cast<double>(q.y) + dz * static_cast<double>(q.z);
+    if (val > bd) {
+      bd = val;
+      bi = i;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ob = __shfl_xor(bd, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    if (ob > bd || (ob == bd && oi < bi)) {
+      bd = ob;
+      bi = oi;
+    }
+  }
+  return bi < nv ? bi : 0;
+}
+
+// Argmax of dot(d_j, v) over the hulls of up to K requesting lanes at once
+// (wave-uniform parameters): lane l visits vertices l, l + 64, ... of every
+// hull, with the K hulls' loads issued together so their latencies overlap;
+// then one butterfly per hull.  The visiting order, the merge and the FP32
+// bound / FP64 rescan rule are those of hullArgmaxOOL<64>, so the indices
+// are the same.  Takes requests from `m` (lowest lanes first); the lane that
+// made request j receives its index in `res`.
+template <int K>
+MFCL_D void coopScanBatch(uint64_t& m, uint64_t vp, int nv, float rb, d3 d, int lane, int& res,
+                          uint64_t& scanned) {
+  int p[K], pn[K];
+  const float4* pv[K];
+  float prb[K];
+  double dx[K], dy[K], dz[K];
+  int nmax = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    p[j] = m ? __builtin_ctzll(m) : -1;
+    m &= m - 1;
+    const int src = p[j] < 0 ? 0 : p[j];
+    pv[j] = reinterpret_cast<const float4*>(readLaneU64(vp, src));
+    pn[j] = p[j] < 0 ? 0 : readLaneI(nv, src);
+    prb[j] = __int_as_float(readLaneI(__float_as_int(rb), src));
+    dx[j] = readLaneD(d.x, src);
+    dy[j] = readLaneD(d.y, src);
+    dz[j] = readLaneD(d.z, src);
+    nmax = max(nmax, pn[j]);
+  }
+  float best[K], second[K];
+  int bi[K];
+  {
+#pragma clang fp contract(fast)
+    float fx[K], fy[K], fz[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      best[j] = -INFINITY;
+      second[j] = -INFINITY;
+      bi[j] = 0x7fffffff;
+      fx[j] = static_cast<float>(dx[j]);
+      fy[j] = static_cast<float>(dy[j]);
+      fz[j] = static_cast<float>(dz[j]);
+    }
+    for (int i = lane; i < nmax; i += 64) {
+      float4 q[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) q[j] = i < pn[j] ? pv[j][i] : make_float4(0, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        if (i < pn[j]) {
+          const float val = fx[j] * q[j].x + fy[j] * q[j].y + fz[j] * q[j].z;
+          if (val > best[j]) {
+            second[j] = best[j];
+            best[j] = val;
+            bi[j] = i;
+          } else if (val > second[j]) {
+            second[j] = val;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const float ob = __shfl_xor(best[j], off, 64);
+      const float os = __shfl_xor(second[j], off, 64);
+      const int oi = __shfl_xor(bi[j], off, 64);
+      if (ob > best[j] || (ob == best[j] && oi < bi[j])) {
+        second[j] = fmaxf(fmaxf(second[j], os), best[j]);
+        best[j] = ob;
+        bi[j] = oi;
+      } else {
+        second[j] = fmaxf(fmaxf(second[j], os), ob);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (p[j] < 0) continue;
+    const float bound = 2.0f * 8.0f * 5.9604645e-08f * prb[j] *
+                        (fabsf(static_cast<float>(dx[j])) + fabsf(static_cast<float>(dy[j])) +
+                         fabsf(static_cast<float>(dz[j])));
+    int idx;
+    if (best[j] - second[j] > bound) {
+      idx = bi[j] < pn[j] ? bi[j] : 0;
+      scanned += static_cast<uint64_t>(pn[j]);
+    } else {
+      idx = waveRescan64(pv[j], pn[j], dx[j], dy[j], dz[j], lane);
+      scanned += static_cast<uint64_t>(2 * pn[j]);
+    }
+    if (lane == p[j]) res = idx;
+  }
+}
+
+// Support stage: every lane with `need` gets support(d) of its Minkowski
+// difference, recorded in its slot `id` (direction and vertex indices).
+// Wavefront-cooperative: all 64 lanes must call this together.  `scanned`
+// (wave-uniform) counts the hull vertices read, as the group kernel does.
+constexpr int kScanBatch = 8;
+MFCL_D void laneSupport(bool need, const MinkLane& md, d3 d, int id, MV& out, uint64_t& scanned) {
+  const d3 d1 = tmul(md.tf0().R, -d);
+  int idx0 = -1, idx1 = -1;
+#pragma nounroll
+  for (int k = 0; k < 2; ++k) {
+    const bool want = need && (k ? md.type1 : md.type0) == kConvex;
+    const uint32_t si = k ? md.ib : md.ia;
+    uint64_t vp = 0;
+    int nv = 0;
+    float rb = 0;
+    if (want) {
+      reloadFence();
+      vp = reinterpret_cast<uint64_t>(md.T.verts + md.T.vert_off[si]);
+      nv = static_cast<int>(md.T.vert_cnt[si]);
+      rb = md.T.param[si].w;
+    }
+    const d3 dd = k ? d1 : d;
+    uint64_t m = __ballot(want);
+    int res = -1;
+    while (m) coopScanBatch<kScanBatch>(m, vp, nv, rb, dd, md.lane, res, scanned);
+    if (k) idx1 = res;
+    else idx0 = res;
+  }
+  if (need) {
+    d3 a, x;
+    if (idx0 >= 0) {
+      const float4 q = md.verts(0)[idx0];
+      a = d3{q.x, q.y, q.z};
+    } else {
+      a = primSupport(md.prim(0), d);
+    }
+    if (idx1 >= 0) {
+      const float4 q = md.verts(1)[idx1];
+      x = d3{q.x, q.y, q.z};
+    } else {
+      x = primSupport(md.prim(1), d1);
+    }
+    out.v = a - apply(md.tf0(), x);
+    out.id = id;
+    md.setDir(id, d);
+    md.w->i0[id][md.lane] = idx0;
+    md.w->i1[id][md.lane] = idx1;
+  }
+}
+
+// States of the lane machine; the ones marked * consume a support point.
+enum LaneState : int {
+  kLsB0 = 0,       // * first support point of GJK (gjk.hpp:26-38)
+  kLsB1 = 1,       // * boolean iteration (gjk.hpp:40-93)
+  kLsSep = 2,      //   separated: start the distance loop (gjk_distance.hpp:11-40)
+  kLsD1 = 3,       // * distance iteration (gjk_distance.hpp:42-100)
+  kLsExtract = 4,  //   witnesses without sub-simplex search (gjk_distance.hpp:375-486)
+  kLsNoSep = 5,    //   not separated: witnesses with sub-simplex search (gjk_distance.hpp:488-559)
+  kLsDone = 6
+};
+
+// gjkEvaluate + gjkMinDistance of narrow_dev.h, one lane per pair, every
+// branch and operation in the same order (the results are bit-identical).
+// All 64 lanes of the wavefront must call this; lanes with active == false
+// only take part in the support stages.
+MFCL_D void laneGjk(const MinkLane& md, bool active, d3 guess, double tol, int max_iter, bool want_distance,
+                    GjkOut& o, Smp& s, uint64_t& scanned) {
+  smpInit(s);
+  o.status = -1;
+  o.sep_valid = false;
+  o.iters = 0;
+  o.dist_iters = 0;
+  o.p0 = d3{0, 0, 0};
+  o.p1 = d3{0, 0, 0};
+  d3 dir = guess;
+  if (sqnorm(dir) <= 0.0) dir = d3{1, 0, 0};
+  dir = normalized(dir);
+  const double tol2 = tol * tol;
+  int st = active ? kLsB0 : kLsDone;
+  bool req = active;
+  int rqid = 0, it = 0, ext = 0;
+  double book = 0;
+  d3 cur = {0, 0, 0};
+  MV v;
+  mvZero(v);
+  while (__ballot(st != kLsDone)) {
+    laneSupport(req, md, dir, rqid, v, scanned);
+    req = false;
+    if (st == kLsB0) {
+      s.rank = -1;
+      smpAdd(s, v);
+      if (sqnorm(v.v) <= tol2) {
+        o.status = kGjkIntersect;
+        st = kLsDone;
+      } else if (dot(v.v, dir) < 0) {
+        st = kLsSep;
+      } else {
+        dir = -dir;
+        o.status = kGjkIterLimit;
+        it = 0;
+        if (it < max_iter) {
+          o.iters = 1;
+          rqid = freeSlot(s);
+          req = true;
+          st = kLsB1;
+        } else {
+          st = kLsNoSep;
+        }
+      }
+    } else if (st == kLsB1) {
+      if (dot(v.v, dir) < 0) {
+        st = kLsSep;
+      } else if (smpDup(s, v.v, tol2)) {
+        o.status = kGjkNoProgress;
+        st = kLsNoSep;
+      } else if (sqnorm(v.v) <= tol2) {
+        smpAdd(s, v);
+        o.status = kGjkIntersect;
+        st = kLsDone;
+      } else {
+        smpAdd(s, v);
+        const int p = gjkProject(s, dir, tol);
+        if (p == kProjFailed) {
+          o.status = kGjkFailed;
+          st = kLsDone;
+        } else if (p == kProjIntersect) {
+          o.status = kGjkIntersect;
+          st = kLsDone;
+        } else if (p == kProjZeroVolume) {
+          o.status = kGjkNoProgress;
+          st = kLsNoSep;
+        } else if (++it < max_iter) {
+          o.iters = it + 1;
+          rqid = freeSlot(s);
+          req = true;
+        } else {
+          st = kLsNoSep;  // status stays IterLimit
+        }
+      }
+    } else if (st == kLsD1) {
+      if (dot(dir, v.v - cur) < tol || smpDup(s, v.v, tol2)) {
+        st = kLsExtract;
+      } else {
+        smpAdd(s, v);
+        const int u = distUpdate(s, cur, tol);
+        if (u == kNoImprovement) {
+          st = kLsExtract;
+        } else if (u != kUpdOK) {
+          st = kLsDone;  // sep_valid stays false
+        } else {
+          const double nm = norm(cur);
+          if (book - nm < tol || nm < tol) {
+            st = kLsExtract;
+          } else {
+            book = nm;
+            dir = -cur / book;
+            if (++it < max_iter) {
+              o.dist_iters++;
+              rqid = freeSlot(s);
+              req = true;
+            } else {
+              st = kLsDone;
+            }
+          }
+        }
+      }
+    }
+    if (st == kLsSep) {
+      o.status = kGjkSeparated;
+      if (!want_distance) {
+        st = kLsDone;
+      } else {
+        // gjkMinDistance from the one-vertex simplex {v}: its closest point is v
+        s.rank = -1;
+        smpAdd(s, v);
+        cur = v.v;
+        book = norm(cur);
+        if (book <= tol) {
+          st = kLsExtract;
+        } else {
+          dir = -cur / book;
+          it = 0;
+          if (it < max_iter) {
+            o.dist_iters++;
+            rqid = freeSlot(s);
+            req = true;
+            st = kLsD1;
+          } else {
+            st = kLsDone;
+          }
+        }
+      }
+    }
+    if (st == kLsExtract) ext = 1;
+    if (st == kLsNoSep && want_distance) ext = 2;
+    if (st == kLsExtract || st == kLsNoSep) st = kLsDone;
+  }
+  // witnesses after the loop, where the machine's state is no longer live
+  if (ext == 1) o.sep_valid = extractNoSub(md, s, o.p0, o.p1);
+  if (ext == 2) o.sep_valid = extractTrySub(md, s, o.p0, o.p1);
+}
+
+}  // namespace mfcl

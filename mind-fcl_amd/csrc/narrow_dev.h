@@ -136,7 +136,9 @@ __device__ __attribute__((noinline)) int hullArgmaxOOL(const float4* verts, cons
   // <= 8 * 2^-24 * |d|_1 * rbound; the gap test uses twice that.
   const float bound = 2.0f * 8.0f * 5.9604645e-08f * rbound *
                       (fabsf(static_cast<float>(dx)) + fabsf(static_cast<float>(dy)) + fabsf(static_cast<float>(dz)));
-  if (best - second > bound) return bi;
+  // a NaN direction or vertex leaves no candidate: vertex 0 keeps every
+  // read in bounds
+  if (best - second > bound) return bi < nv ? bi : 0;
   double bd = -INFINITY;
   bi = 0x7fffffff;
   for (int i = lg; i < nv; i += G) {
@@ -162,7 +164,7 @@ __device__ __attribute__((noinline)) int hullArgmaxOOL(const float4* verts, cons
       bi = oi;
     }
   }
-  return bi | (1 << 30);
+  return (bi < nv ? bi : 0) | (1 << 30);
 }
 
 template <int G>
@@ -172,9 +174,9 @@ MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
   return r & ((1 << 30) - 1);
 }
 
-// include/fcl/cvx_collide/gjk_shape.hpp:41-180; shape_gjk_interface-inl.h:87-92.
-template <int G>
-MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int lg, int* scanned) {
+// Closed-form support points of the primitives
+// (include/fcl/cvx_collide/gjk_shape.hpp:41-180); zero for a hull.
+MFCL_D d3 primSupport(const DevShape& s, d3 dir) {
   switch (s.type) {
     case kBox:
       return {dir.x > 0 ? s.p0 / 2 : -s.p0 / 2, dir.y > 0 ? s.p1 / 2 : -s.p1 / 2, dir.z > 0 ? s.p2 / 2 : -s.p2 / 2};
@@ -193,13 +195,19 @@ MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int lg, int* scanned) {
       const double k = r / zd;
       return {k * dir.x, k * dir.y, dir.z > 0 ? hh : -hh};
     }
-    case kConvex: {
-      const float3 q = hullVert(s, hullArgmax<G>(s, dir, lg, scanned));
-      return {q.x, q.y, q.z};
-    }
     default:
       return {0, 0, 0};
   }
+}
+
+// include/fcl/cvx_collide/gjk_shape.hpp:41-180; shape_gjk_interface-inl.h:87-92.
+template <int G>
+MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int lg, int* scanned) {
+  if (s.type == kConvex) {
+    const float3 q = hullVert(s, hullArgmax<G>(s, dir, lg, scanned));
+    return {q.x, q.y, q.z};
+  }
+  return primSupport(s, dir);
 }
 
 // shapeSupport plus the index of the hull vertex returned (-1 for primitives).
@@ -625,23 +633,22 @@ MFCL_D int dist3(d3 x, d3 y, d3 z, d3& out, double tol) {
 MFCL_D void opaque(MV& m) {
   asm("" : "+v"(m.v.x), "+v"(m.v.y), "+v"(m.v.z), "+v"(m.id));
 }
-MFCL_D MV smpSlot(const Smp& s, int k) {
-  MV A = s.a, B = s.b, C = s.c, E = s.e;
-  opaque(A); opaque(B); opaque(C); opaque(E);
+MFCL_D MV pick4(const MV& A, const MV& B, const MV& C, const MV& E, int k) {
   MV r = mvSel(k == 1, B, A);
   r = mvSel(k == 2, C, r);
   return mvSel(k == 3, E, r);
 }
 
-MFCL_D void smpApply(Smp& s, int code) {
-  const int r = codeRank(code);
-  const MV n0 = smpSlot(s, codeSlot(code, 0));
-  const MV n1 = smpSlot(s, codeSlot(code, 1));
-  const MV n2 = smpSlot(s, codeSlot(code, 2));
-  s.a = n0;
-  if (r > 1) s.b = n1;
-  if (r > 2) s.c = n2;
-  s.rank = r;
+// The simplex becomes the vertices that `code` keeps of (A, B, C, E), the
+// opaque copies of its slots.  Every slot is rewritten (slots beyond the
+// rank hold don't-care values), so the original slots are dead once copied
+// and the copies take their registers.
+MFCL_D void smpApplyFrom(Smp& s, const MV& A, const MV& B, const MV& C, const MV& E, int code) {
+  s.a = pick4(A, B, C, E, codeSlot(code, 0));
+  s.b = pick4(A, B, C, E, codeSlot(code, 1));
+  s.c = pick4(A, B, C, E, codeSlot(code, 2));
+  s.e = E;
+  s.rank = codeRank(code);
 }
 
 enum DistUpd : int { kNoImprovement, kUpdOK, kUpdFailed };
@@ -651,17 +658,17 @@ MFCL_D int distUpdate(Smp& s, d3& out, double tol) {
     out = s.a.v;
     return kUpdOK;
   }
-  if (s.rank == 2) {
-    smpApply(s, dist2(s.a.v, s.b.v, out, tol));
-    return kUpdOK;
-  }
-  if (s.rank != 3 && s.rank != 4) return kUpdFailed;
+  if (s.rank != 2 && s.rank != 3 && s.rank != 4) return kUpdFailed;
   // rank 3: the triangle itself; rank 4 (gjk_distance.hpp:291-373): faces
   // (3,2,1), (3,2,0), (3,1,0) = slots (b,c,e), (a,c,e), (a,b,e).  One loop
   // body (a single inlined copy of the triangle case) with the inputs picked
   // by value selects.
   MV A = s.a, B = s.b, Cc = s.c, E = s.e;
   opaque(A); opaque(B); opaque(Cc); opaque(E);
+  if (s.rank == 2) {
+    smpApplyFrom(s, A, B, Cc, E, dist2(A.v, B.v, out, tol));
+    return kUpdOK;
+  }
   const bool r4 = s.rank == 4;
   const int nf = r4 ? 3 : 1;
   double best = -1;
@@ -680,8 +687,11 @@ MFCL_D int distUpdate(Smp& s, d3& out, double tol) {
     const double dd = sqnorm(pt);
     if (best < 0 || dd < best) { best = dd; bpt = pt; bc = c; }
   }
-  if (best < 0) return kNoImprovement;
-  smpApply(s, bc);
+  if (best < 0) {
+    s.a = A; s.b = B; s.c = Cc; s.e = E;  // unchanged, from the copies
+    return kNoImprovement;
+  }
+  smpApplyFrom(s, A, B, Cc, E, bc);
   out = bpt;
   return kUpdOK;
 }

@@ -13,6 +13,7 @@
 
 #include "ctx.h"
 #include "epa_dev.h"
+#include "gjk_lane.h"
 #include "prim_dev.h"
 
 namespace mfcl {
@@ -343,6 +344,120 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(scanned_total, v);
 }
 
+// One lane per pair (penetration and signed-distance modes): closed forms
+// per lane, GJK through the lane machine of gjk_lane.h.  Results, EPA queue
+// entries and scan counts equal those of narrowKernel.  dir_ws holds
+// kLaneDirDoubles per wavefront of the grid.
+template <int MODE>
+__global__ void __launch_bounds__(kBlock, 3) laneKernel(ShapeTables T, PairIO io, const uint32_t* order, uint64_t n,
+                                                     Params P, QEntry* queue, uint32_t* qcount,
+                                                     unsigned long long* scanned_total, double* dir_ws) {
+  __shared__ LaneWs ws[kBlock / 64];
+  const uint64_t slot = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  MinkLane md;
+  md.T = T;
+  md.w = (MFCL_LDS LaneWs*)(&ws[threadIdx.x >> 6]);
+  md.dirs = dir_ws + (slot >> 6) * kLaneDirDoubles;
+  md.lane = static_cast<int>(threadIdx.x & 63);
+  md.ia = md.ib = 0;
+  md.type0 = md.type1 = kSphere;
+  uint32_t i = 0;
+  bool run_gjk = false;
+  d3 guess = {1, 0, 0};
+  if (slot < n) {
+    i = order[slot];
+    mfcl_convex_result r;
+    clearResult(r);
+    ResAcc acc;
+    acc.normal = acc.pos = acc.pa = acc.pb = d3{0, 0, 0};
+    const uint32_t ia = io.pa[i], ib = io.pb[i];
+    if (ia >= P.n_shapes || ib >= P.n_shapes) {
+      r.ok = -1;  // shape id out of range
+    } else {
+      const int ta_ = T.type[ia], tb_ = T.type[ib];
+      if (MODE == kModePen && P.max_contacts <= 0) {
+        // the reference warns and adds nothing (collision-inl.h:81-84)
+      } else if (MODE == kModePen && closedForm(ta_, tb_)) {
+        const tf3 ta = poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i));
+        const tf3 tb = poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i));
+        PContacts pc;
+        pc.n = 0;
+        const bool hit = runClosedForm(loadShape(T, ia), ta, loadShape(T, ib), tb, &pc);
+        r.hit = hit;
+        r.ok = 1;
+        if (hit) {
+          // shape_pair_intersect-inl.h:91-113: deepest contacts first when truncated
+          int k0 = 0;
+          if (pc.n > P.max_contacts) {
+            for (int k = 1; k < pc.n; ++k)
+              if (pc.c[k].depth > pc.c[k0].depth) k0 = k;
+          }
+          r.n_contacts = pc.n < P.max_contacts ? pc.n : P.max_contacts;
+          if (pc.n > 0) {
+            r.depth = pc.c[k0].depth;
+            r.distance = -r.depth;
+            acc.normal = pc.c[k0].n;
+            acc.pos = pc.c[k0].p;
+          }
+        }
+      } else {
+        md.ia = ia;
+        md.ib = ib;
+        md.type0 = ta_;
+        md.type1 = tb_;
+        md.setTf0(relTf(poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i)),
+                        poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i))));
+        if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
+        run_gjk = true;
+      }
+    }
+    if (!run_gjk) storeResult(&io.out[i], r, acc);
+  }
+  GjkOut g;
+  Smp s;
+  uint64_t scanned = 0;
+  laneGjk(md, run_gjk, -guess, P.gjk_tol, P.gjk_max_it, MODE == kModeSd, g, s, scanned);
+  if (run_gjk) {
+    mfcl_convex_result r;
+    clearResult(r);
+    ResAcc acc;
+    acc.normal = acc.pos = acc.pa = acc.pb = d3{0, 0, 0};
+    r.method = 1;
+    r.gjk_status = static_cast<int8_t>(g.status);
+    bool queued = false;
+    if constexpr (MODE == kModeSd) {
+      r.gjk_iters = sat16(g.iters + g.dist_iters);
+      if (g.status == kGjkSeparated && g.sep_valid) {
+        const tf3 ta = poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i));
+        acc.pa = apply(ta, g.p0);
+        acc.pb = apply(ta, g.p1);
+        r.distance = norm(g.p0 - g.p1);
+        r.ok = 1;
+      } else if (g.status == kGjkIntersect) {
+        queued = true;
+      } else {
+        r.distance = -1;
+      }
+    } else {
+      r.gjk_iters = sat16(g.iters);
+      r.ok = 1;
+      queued = g.status == kGjkIntersect;
+    }
+    if (queued) {
+      QEntry* qe = &queue[atomicAdd(qcount, 1u)];
+      qe->s.rank = s.rank;
+      storeMVD(qe->s.a, s.a.v, md.dirOf(static_cast<int>(s.a.id)));
+      storeMVD(qe->s.b, s.b.v, md.dirOf(static_cast<int>(s.b.id)));
+      storeMVD(qe->s.c, s.c.v, md.dirOf(static_cast<int>(s.c.id)));
+      storeMVD(qe->s.e, s.e.v, md.dirOf(static_cast<int>(s.e.id)));
+      qe->pair = i;
+    }
+    // queued pairs keep their GJK status here; the EPA stage completes them
+    storeResult(&io.out[i], r, acc);
+  }
+  if (md.lane == 0 && scanned) atomicAdd(scanned_total, static_cast<unsigned long long>(scanned));
+}
+
 // EPA stage: one wavefront per queued pair; the 64 lanes share every
 // support scan and run the polytope bookkeeping in lockstep on identical
 // values.  Each wavefront owns an LDS slice: the pair's Minkowski
@@ -517,7 +632,16 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   if (mode == kModeSd) MFCL_NARROW(GG, kModeSd); \
   else if (mode == kModePen) MFCL_NARROW(GG, kModePen); \
   else MFCL_NARROW(GG, kModeBool)
-    switch (G) {
+    if (G == 1 && mode != kModeBool) {
+      const unsigned lb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+      if (int rc = ensure(ctx, ctx->ws_lane, static_cast<size_t>(lb) * (kBlock / 64) * kLaneDirDoubles * sizeof(double)))
+        return rc;
+      double* dws = static_cast<double*>(ctx->ws_lane.p);
+      if (mode == kModeSd)
+        laneKernel<kModeSd><<<lb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws);
+      else
+        laneKernel<kModePen><<<lb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws);
+    } else switch (G) {
       case 32: MFCL_NARROW_MODES(32); break;
       case 64: MFCL_NARROW_MODES(64); break;
       default: MFCL_NARROW_MODES(16); break;
