@@ -145,6 +145,22 @@ int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
                        const uint64_t* vert_off, const uint32_t* vert_cnt, uint32_t n_shapes,
                        const float* verts_xyz, uint64_t n_verts);
 
+/* Attaches faces to the convex shapes of the current shape table (the faces
+ * argument of fcl::Convex's constructor, include/fcl/geometry/shape/
+ * convex-inl.h:50-73).  Shape i owns num_faces[i] faces starting at
+ * faces[face_off[i]], each in the reference's encoding: the vertex count,
+ * then that many indices into the shape's vertices.  As in the reference, a
+ * hull of more than 32 vertices whose faces are watertight and cover every
+ * vertex (convex-inl.h:293-375) then finds support points by the vertex-
+ * adjacency walk (convex-inl.h:202-290); other hulls, and all hulls before
+ * this call or with num_faces[i] == 0, by the naive scan (convex-inl.h:
+ * 133-149).  Shapes with equal (face_off, num_faces, vertex count) share one
+ * adjacency table, as Convex objects built from one shared face vector do.
+ * faces_len: int32 entries in faces.  Errors: a face with fewer than 3
+ * vertices or an index outside its hull (MFCL_E_INVALID). */
+int mfcl_upload_convex_faces(mfcl_ctx* ctx, const int32_t* faces, uint64_t faces_len,
+                             const uint64_t* face_off, const uint32_t* num_faces);
+
 /* Batched fcl::collide / GJKSolver::shapeSignedDistance over shape pairs
  * (include/fcl/narrowphase/collision_interface-inl.h:13-32,
  * gjk_solver-inl.h:820-874).  pair_a/pair_b: shape ids; pose_a/pose_b: 7

@@ -46,6 +46,9 @@ void freeShapes(mfcl_ctx* c) {
   freePtr(reinterpret_cast<void*&>(c->d_verts));
   freePtr(reinterpret_cast<void*&>(c->d_interior));
   freePtr(reinterpret_cast<void*&>(c->d_desc));
+  freePtr(reinterpret_cast<void*&>(c->d_nbr));
+  freePtr(reinterpret_cast<void*&>(c->d_nbr_off));
+  freePtr(reinterpret_cast<void*&>(c->d_seed));
   c->n_shapes = 0;
 }
 
@@ -88,6 +91,13 @@ static int upload(mfcl_ctx* ctx, T** dst, const T* src, size_t count) {
   if (count == 0) count = 1;
   MFCL_HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * count));
   if (src) MFCL_HIP_TRY(ctx, hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  return MFCL_OK;
+}
+
+int mfcl::uploadVec(mfcl_ctx* ctx, void** dptr, const void* src, size_t bytes) {
+  freePtr(*dptr);
+  MFCL_HIP_TRY(ctx, hipMalloc(dptr, bytes ? bytes : 4));
+  if (src && bytes) MFCL_HIP_TRY(ctx, hipMemcpy(*dptr, src, bytes, hipMemcpyHostToDevice));
   return MFCL_OK;
 }
 
@@ -334,8 +344,28 @@ int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
   if ((rc = upload(ctx, &ctx->d_interior, interior.data(), n_shapes))) return rc;
   if ((rc = upload(ctx, &ctx->d_desc, desc.data(), desc.size()))) return rc;
   ctx->h_vcnt = vcnt;
+  ctx->h_voff = voff;
+  ctx->h_type.assign(type, type + n_shapes);
   ctx->n_shapes = n_shapes;
   return MFCL_OK;
+}
+
+int mfcl_upload_convex_faces(mfcl_ctx* ctx, const int32_t* faces, uint64_t faces_len, const uint64_t* face_off,
+                             const uint32_t* num_faces) {
+  if (!ctx || !face_off || !num_faces || (faces_len && !faces)) return MFCL_E_INVALID;
+  if (ctx->n_shapes == 0) {
+    ctx->err = "no shapes uploaded";
+    return MFCL_E_STATE;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const int rc = uploadConvexFaces(ctx, faces, face_off, num_faces, faces_len);
+  if (rc) {
+    freePtr(reinterpret_cast<void*&>(ctx->d_nbr));
+    freePtr(reinterpret_cast<void*&>(ctx->d_nbr_off));
+    freePtr(reinterpret_cast<void*&>(ctx->d_seed));
+  }
+  return rc;
 }
 
 static int finishTiming(mfcl_ctx* ctx, int first_ev, int n_ev, int slot0) {

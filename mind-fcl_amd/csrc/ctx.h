@@ -84,7 +84,12 @@ struct mfcl_ctx {
   float4* d_verts{nullptr};
   double4* d_interior{nullptr};
   float* d_desc{nullptr};  // 16-float shape descriptors (warm-start MLP inputs)
-  std::vector<uint32_t> h_vcnt;
+  // convex faces (mfcl_upload_convex_faces): adjacency for the support walk
+  int32_t* d_nbr{nullptr};      // adjacency tables in the reference's encoding
+  int64_t* d_nbr_off{nullptr};  // per shape: table offset, -1 = naive scans; null = no faces
+  int32_t* d_seed{nullptr};     // per shape: 8 ints, the walk's six axis seed vertices
+  std::vector<uint32_t> h_vcnt, h_voff;
+  std::vector<int32_t> h_type;
 
   // mesh set
   uint32_t n_meshes{0};
@@ -117,6 +122,13 @@ namespace mfcl {
 
 int ensure(mfcl_ctx* ctx, DevBuf& b, size_t bytes);
 void freeAll(mfcl_ctx* ctx);
+// hipMalloc + copy of `bytes` bytes (frees *dptr first).
+int uploadVec(mfcl_ctx* ctx, void** dptr, const void* src, size_t bytes);
+// convex_faces.cpp: adjacency tables for the support walk; narrowphase.hip:
+// the walk's per-shape seed vertices (device pass over the uploaded hulls).
+int uploadConvexFaces(mfcl_ctx* ctx, const int32_t* faces, const uint64_t* face_off, const uint32_t* num_faces,
+                      uint64_t faces_len);
+int computeWalkSeeds(mfcl_ctx* ctx);
 
 // Launchers (narrowphase.hip, bvh.hip, broadphase.hip).
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,

@@ -75,6 +75,7 @@ struct MinkLane {
     s.p0 = p.x; s.p1 = p.y; s.p2 = p.z;
     s.verts = nullptr; s.nv = 0; s.rbound = p.w;
     s.lv = nullptr; s.ls = 0;
+    s.nbr = nullptr; s.seed = nullptr;
     return s;
   }
   MFCL_D const float4* verts(int k) const {
@@ -243,15 +244,18 @@ MFCL_D void coopScanBatch(uint64_t& m, uint64_t vp, int nv, float rb, d3 d, int 
 // Support stage: every lane with `need` gets support(d) of its Minkowski
 // difference, recorded in its slot `id` (direction and vertex indices).
 // Wavefront-cooperative: all 64 lanes must call this together.  `scanned`
-// (wave-uniform) counts the hull vertices read, as the group kernel does.
+// (wave-uniform) counts the hull vertices read by the cooperative scans,
+// `walked` (per lane) those read by the lane's vertex walks.
 constexpr int kScanBatch = 8;
-MFCL_D void laneSupport(bool need, const MinkLane& md, d3 d, int id, MV& out, uint64_t& scanned) {
+MFCL_D void laneSupport(bool need, const MinkLane& md, d3 d, int id, MV& out, uint64_t& scanned, int& walked) {
   const d3 d1 = tmul(md.tf0().R, -d);
   int idx0 = -1, idx1 = -1;
 #pragma nounroll
   for (int k = 0; k < 2; ++k) {
-    const bool want = need && (k ? md.type1 : md.type0) == kConvex;
+    const bool hull = need && (k ? md.type1 : md.type0) == kConvex;
     const uint32_t si = k ? md.ib : md.ia;
+    const bool walk = hull && md.T.nbr_off && md.T.nbr_off[si] >= 0;
+    const bool want = hull && !walk;
     uint64_t vp = 0;
     int nv = 0;
     float rb = 0;
@@ -262,9 +266,11 @@ MFCL_D void laneSupport(bool need, const MinkLane& md, d3 d, int id, MV& out, ui
       rb = md.T.param[si].w;
     }
     const d3 dd = k ? d1 : d;
+    // naive hulls: cooperative scans; walking hulls: each lane walks its own
     uint64_t m = __ballot(want);
     int res = -1;
     while (m) coopScanBatch<kScanBatch>(m, vp, nv, rb, dd, md.lane, res, scanned);
+    if (walk) res = hullWalk(loadShape(md.T, si), dd, &walked);
     if (k) idx1 = res;
     else idx0 = res;
   }
@@ -306,7 +312,7 @@ enum LaneState : int {
 // All 64 lanes of the wavefront must call this; lanes with active == false
 // only take part in the support stages.
 MFCL_D void laneGjk(const MinkLane& md, bool active, d3 guess, double tol, int max_iter, bool want_distance,
-                    GjkOut& o, Smp& s, uint64_t& scanned) {
+                    GjkOut& o, Smp& s, uint64_t& scanned, int& walked) {
   smpInit(s);
   o.status = -1;
   o.sep_valid = false;
@@ -326,7 +332,7 @@ MFCL_D void laneGjk(const MinkLane& md, bool active, d3 guess, double tol, int m
   MV v;
   mvZero(v);
   while (__ballot(st != kLsDone)) {
-    laneSupport(req, md, dir, rqid, v, scanned);
+    laneSupport(req, md, dir, rqid, v, scanned, walked);
     req = false;
     if (st == kLsB0) {
       s.rank = -1;

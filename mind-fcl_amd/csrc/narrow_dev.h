@@ -30,6 +30,11 @@ struct DevShape {
   // z at lv[i + 2 ls]); nullptr when the hull is read from device memory
   const MFCL_LDS float* lv;
   int ls;
+  // convex with faces and more than 32 vertices: adjacency table in the
+  // reference's encoding and the six axis seed vertices (support by the
+  // vertex walk); nullptr: support by the naive scan
+  const int* nbr;
+  const int* seed;
 };
 
 // Vertex i of a hull, from the LDS copy when there is one (same values).
@@ -58,6 +63,9 @@ struct ShapeTables {
   const uint32_t* vert_off;
   const uint32_t* vert_cnt;
   const float4* verts;
+  const int* nbr;         // adjacency tables (mfcl_upload_convex_faces)
+  const int64_t* nbr_off; // per shape, -1: naive scans; nullptr: no faces uploaded
+  const int* seed;        // per shape: 8 ints, the walk's axis seed vertices
 };
 
 MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
@@ -68,9 +76,18 @@ MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
   s.p1 = p.y;
   s.p2 = p.z;
   s.rbound = p.w;
+  s.nbr = nullptr;
+  s.seed = nullptr;
   if (s.type == kConvex) {
     s.verts = T.verts + T.vert_off[i];
     s.nv = static_cast<int>(T.vert_cnt[i]);
+    if (T.nbr_off) {
+      const int64_t o = T.nbr_off[i];
+      if (o >= 0) {
+        s.nbr = T.nbr + o;
+        s.seed = T.seed + 8 * static_cast<uint64_t>(i);
+      }
+    }
   } else {
     s.verts = nullptr;
     s.nv = 0;
@@ -167,8 +184,87 @@ __device__ __attribute__((noinline)) int hullArgmaxOOL(const float4* verts, cons
   return (bi < nv ? bi : 0) | (1 << 30);
 }
 
+// convex-inl.h:202-290: the walk starts from the seed vertex of the axis
+// direction (+x, -x, +y, -y, +z, -z) with the largest dot product with d
+// (first on ties), then moves to the best strictly improving neighbour of the
+// current vertex, skipping the two most recently left vertices, until no
+// neighbour improves.  FP64 dot products as the reference's
+// v_C.dot(vertices[n]).  Every step strictly increases the value, so the walk
+// ends (after at most nv steps; at once for a NaN direction).
+//
+// Latency: a step fetches the neighbour record (count and up to 8 ids; the
+// tables are followed by 32 ints of padding, so the fixed-size reads stay in
+// bounds), then the neighbours' vertices and their own record offsets
+// together, and only then runs the reference's sequential comparisons in
+// registers -- two dependent memory round trips per step instead of one per
+// neighbour.
+constexpr int kWalkChunk = 8;
+
+MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
+  int init = -1;
+  double max_dot = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    // the seed directions of convex-inl.h:151-199: +-UnitX, +-UnitY, +-UnitZ
+    const double sg = (i & 1) ? -1.0 : 1.0;
+    const double ax = (i >> 1) == 0 ? sg : sg * 0.0;
+    const double ay = (i >> 1) == 1 ? sg : sg * 0.0;
+    const double az = (i >> 1) == 2 ? sg : sg * 0.0;
+    const double dv = ax * d.x + ay * d.y + az * d.z;
+    if (init < 0 || dv > max_dot) {
+      init = s.seed[i];
+      max_dot = dv;
+    }
+  }
+  int ext = init;
+  float3 q = hullVert(s, ext);
+  double ext_v = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
+  int start = s.nbr[ext];
+  int parent0 = init, parent1 = init, visited = 1;
+  bool keep = true;
+  while (keep) {
+    keep = false;
+    const int old = ext;
+    int next_start = start;
+    const int count = s.nbr[start];
+    for (int c0 = 0; c0 < count; c0 += kWalkChunk) {
+      int id[kWalkChunk], st[kWalkChunk];
+      float3 w[kWalkChunk];
+#pragma unroll
+      for (int j = 0; j < kWalkChunk; ++j) {
+        const int nb = s.nbr[start + 1 + c0 + j];
+        id[j] = c0 + j < count ? nb : old;  // in range for the loads below
+      }
+#pragma unroll
+      for (int j = 0; j < kWalkChunk; ++j) {
+        w[j] = hullVert(s, id[j]);
+        st[j] = s.nbr[id[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < kWalkChunk; ++j) {
+        if (c0 + j >= count || id[j] == parent0 || id[j] == parent1) continue;
+        const double v = d.x * static_cast<double>(w[j].x) + d.y * static_cast<double>(w[j].y) +
+                         d.z * static_cast<double>(w[j].z);
+        ++visited;
+        if (v > ext_v) {
+          parent1 = ext;
+          keep = true;
+          ext = id[j];
+          ext_v = v;
+          next_start = st[j];
+        }
+      }
+    }
+    parent0 = old;
+    start = next_start;
+  }
+  *scanned += visited;
+  return ext;
+}
+
 template <int G>
 MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
+  if (s.nbr) return hullWalk(s, d, scanned);
   const int r = hullArgmaxOOL<G>(s.verts, s.lv, s.ls, s.nv, s.rbound, d.x, d.y, d.z, lg);
   *scanned += (r >> 30) ? 2 * s.nv : s.nv;
   return r & ((1 << 30) - 1);
@@ -259,6 +355,10 @@ struct MinkCfg {
   const float4* v1;
   const MFCL_LDS float* lv0;
   const MFCL_LDS float* lv1;
+  const int* nbr0;
+  const int* nbr1;
+  const int* seed0;
+  const int* seed1;
   int ls0, ls1;
   int type0, type1, nv0, nv1;
   float rb0, rb1;
@@ -292,6 +392,8 @@ MFCL_D void writeCfg(MFCL_LDS MinkCfg* c, const DevShape& a, const tf3& ta, cons
   c->int1[0] = int1.x; c->int1[1] = int1.y; c->int1[2] = int1.z;
   c->v0 = a.verts; c->v1 = b.verts;
   c->lv0 = a.lv; c->lv1 = b.lv;
+  c->nbr0 = a.nbr; c->nbr1 = b.nbr;
+  c->seed0 = a.seed; c->seed1 = b.seed;
   c->ls0 = a.ls; c->ls1 = b.ls;
   c->type0 = a.type; c->type1 = b.type;
   c->nv0 = a.nv; c->nv1 = b.nv;
@@ -314,6 +416,7 @@ struct MinkL {
     s.type = c->type0; s.p0 = c->q0[0]; s.p1 = c->q0[1]; s.p2 = c->q0[2];
     s.verts = c->v0; s.nv = c->nv0; s.rbound = c->rb0;
     s.lv = c->lv0; s.ls = c->ls0;
+    s.nbr = c->nbr0; s.seed = c->seed0;
     return s;
   }
   MFCL_D DevShape sh1() const {
@@ -321,6 +424,7 @@ struct MinkL {
     s.type = c->type1; s.p0 = c->q1[0]; s.p1 = c->q1[1]; s.p2 = c->q1[2];
     s.verts = c->v1; s.nv = c->nv1; s.rbound = c->rb1;
     s.lv = c->lv1; s.ls = c->ls1;
+    s.nbr = c->nbr1; s.seed = c->seed1;
     return s;
   }
   MFCL_D m33 rot1() const { return loadM33(c->r1); }

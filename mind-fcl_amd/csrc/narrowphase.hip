@@ -416,7 +416,8 @@ __global__ void __launch_bounds__(kBlock, 3) laneKernel(ShapeTables T, PairIO io
   GjkOut g;
   Smp s;
   uint64_t scanned = 0;
-  laneGjk(md, run_gjk, -guess, P.gjk_tol, P.gjk_max_it, MODE == kModeSd, g, s, scanned);
+  int walked = 0;
+  laneGjk(md, run_gjk, -guess, P.gjk_tol, P.gjk_max_it, MODE == kModeSd, g, s, scanned, walked);
   if (run_gjk) {
     mfcl_convex_result r;
     clearResult(r);
@@ -455,7 +456,11 @@ __global__ void __launch_bounds__(kBlock, 3) laneKernel(ShapeTables T, PairIO io
     // queued pairs keep their GJK status here; the EPA stage completes them
     storeResult(&io.out[i], r, acc);
   }
-  if (md.lane == 0 && scanned) atomicAdd(scanned_total, static_cast<unsigned long long>(scanned));
+  // cooperative scans are counted once per wavefront, walks per lane
+  unsigned long long v = static_cast<unsigned long long>(walked);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  v += scanned;
+  if (md.lane == 0 && v) atomicAdd(scanned_total, v);
 }
 
 // EPA stage: one wavefront per queued pair; the 64 lanes share every
@@ -557,6 +562,48 @@ __global__ void zeroKernel(uint32_t* p, int n) {
 
 
 
+// Seed vertices of the support walk (convex-inl.h:151-199): per walking
+// hull, the first vertex attaining the maximum of +x, -x, +y, -y, +z, -z
+// (convex-inl.h:133-149; the dot product with a unit axis is the coordinate).
+__global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || T.nbr_off[i] < 0) return;
+  const float4* v = T.verts + T.vert_off[i];
+  const int nv = static_cast<int>(T.vert_cnt[i]);
+  int idx[6] = {0, 0, 0, 0, 0, 0};
+  double best[6];
+  {
+    const float4 q = v[0];
+    best[0] = q.x; best[1] = -static_cast<double>(q.x);
+    best[2] = q.y; best[3] = -static_cast<double>(q.y);
+    best[4] = q.z; best[5] = -static_cast<double>(q.z);
+  }
+  for (int k = 1; k < nv; ++k) {
+    const float4 q = v[k];
+    const double c[6] = {q.x, -static_cast<double>(q.x), q.y, -static_cast<double>(q.y), q.z,
+                         -static_cast<double>(q.z)};
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+      if (c[a] > best[a]) {
+        best[a] = c[a];
+        idx[a] = k;
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) seed[8 * static_cast<uint64_t>(i) + a] = idx[a];
+  seed[8 * static_cast<uint64_t>(i) + 6] = seed[8 * static_cast<uint64_t>(i) + 7] = 0;
+}
+
+int computeWalkSeeds(mfcl_ctx* ctx) {
+  ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
+                ctx->d_seed};
+  const uint32_t n = ctx->n_shapes;
+  walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed);
+  MFCL_HIP_TRY(ctx, hipGetLastError());
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MFCL_OK;
+}
+
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
                       mfcl_convex_result* out, hipStream_t s) {
@@ -613,7 +660,8 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   uint8_t* key = reinterpret_cast<uint8_t*>(order + n);
   QEntry* queue = static_cast<QEntry*>(ctx->ws_queue.p);
 
-  ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts};
+  ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
+                ctx->d_seed};
   PairIO io{pa, pb, posea, poseb, guess, out};
   const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));

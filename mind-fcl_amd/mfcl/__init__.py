@@ -92,6 +92,7 @@ def load_library() -> C.CDLL:
     lib.mfcl_h2d.argtypes = [p, p, p, u64]
     lib.mfcl_d2h.argtypes = [p, p, p, u64]
     lib.mfcl_upload_shapes.argtypes = [p, p, p, p, p, u32, p, u64]
+    lib.mfcl_upload_convex_faces.argtypes = [p, p, u64, p, p]
     lib.mfcl_convex_query.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p]
     lib.mfcl_convex_query_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
@@ -116,6 +117,17 @@ def load_library() -> C.CDLL:
 
 def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def faces_from_triangles(tris, tri_off, tri_cnt):
+    """Triangles (k x 3 vertex indices; shape i owns tris[tri_off[i] ..
+    tri_off[i] + tri_cnt[i])) -> (faces, face_off, num_faces) in the
+    reference's face encoding [3, a, b, c] per triangle."""
+    tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    faces = np.empty((len(tris), 4), np.int32)
+    faces[:, 0] = 3
+    faces[:, 1:] = tris
+    return faces.reshape(-1), 4 * np.asarray(tri_off, np.uint64), np.asarray(tri_cnt, np.uint32)
 
 
 def request(mode: int = MODE_COLLIDE, **kw) -> Request:
@@ -204,7 +216,10 @@ class Context:
         return d
 
     # ---- shapes
-    def upload_shapes(self, types, params4, vert_off=None, vert_cnt=None, verts=None):
+    def upload_shapes(self, types, params4, vert_off=None, vert_cnt=None, verts=None, faces=None):
+        """faces: optional (faces, face_off, num_faces) in the reference's
+        encoding (see faces_from_triangles); hulls then use the vertex walk
+        where the reference does."""
         types = np.ascontiguousarray(types, np.int32)
         params4 = np.ascontiguousarray(params4, np.float32).reshape(-1, 4)
         n = len(types)
@@ -220,6 +235,15 @@ class Context:
                 self.h, _ptr(types), _ptr(params4), _ptr(vert_off), _ptr(vert_cnt), n, _ptr(verts), len(verts)
             )
         )
+        if faces is not None:
+            self.upload_convex_faces(*faces)
+
+    def upload_convex_faces(self, faces, face_off, num_faces):
+        faces = np.ascontiguousarray(faces, np.int32)
+        face_off = np.ascontiguousarray(face_off, np.uint64)
+        num_faces = np.ascontiguousarray(num_faces, np.uint32)
+        self._check(self.lib.mfcl_upload_convex_faces(self.h, _ptr(faces), len(faces), _ptr(face_off),
+                                                      _ptr(num_faces)))
 
     def convex_query(self, pair_a, pair_b, pose_a, pose_b, req: Request, guess=None) -> np.ndarray:
         pair_a = np.ascontiguousarray(pair_a, np.uint32)

@@ -114,12 +114,65 @@ def ellipsoid_hulls(rng: np.random.Generator, n_hulls: int, nmin: int = 8, nmax:
     return verts, off, counts, axes
 
 
-def hull_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int = 512):
+def template_hulls(rng: np.random.Generator, n_hulls: int, nmin: int = 8, nmax: int = 512, amin: float = 0.05,
+                   amax: float = 0.4, variants: int = 0):
+    """Convex hulls with faces: N = round(exp(U[ln nmin, ln nmax])) points
+    uniform on the unit sphere, taken from `variants` template point sets per
+    N (0: about n_hulls / (4 x distinct N), between 1 and 16) whose hull
+    triangles come from qhull, then scaled by axes U[amin,
+    amax] (a linear map keeps the hull's faces), so every point lies on an
+    ellipsoid and is a hull vertex.  Hulls of one template share its
+    triangles (tri_off / tri_cnt point into one array)."""
+    counts = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=n_hulls))).astype(np.uint32)
+    axes = rng.uniform(amin, amax, size=(n_hulls, 3)).astype(np.float32)
+    sizes = np.unique(counts)
+    if variants <= 0:
+        variants = int(min(16, max(1, n_hulls // (4 * len(sizes)))))
+    t_pts, t_off, t_tris, t_toff, t_tcnt = [], [], [], [], []
+    po = to = 0
+    t_index = {}
+    for c in sizes:
+        for v in range(variants):
+            u = rng.normal(size=(int(c), 3))
+            u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+            from scipy.spatial import ConvexHull
+
+            tri = ConvexHull(u.astype(np.float64)).simplices.astype(np.int32)
+            t_index[(int(c), v)] = len(t_off)
+            t_pts.append(u)
+            t_off.append(po)
+            t_tris.append(tri)
+            t_toff.append(to)
+            t_tcnt.append(len(tri))
+            po += int(c)
+            to += len(tri)
+    t_pts = np.concatenate(t_pts)
+    t_off = np.array(t_off, np.int64)
+    which = rng.integers(0, variants, size=n_hulls)
+    tid = np.array([t_index[(int(c), int(w))] for c, w in zip(counts, which)], np.int64)
+    off = np.zeros(n_hulls, np.uint64)
+    off[1:] = np.cumsum(counts[:-1], dtype=np.uint64)
+    total = int(counts.sum(dtype=np.uint64))
+    owner = np.repeat(np.arange(n_hulls, dtype=np.int64), counts.astype(np.intp))
+    local = np.arange(total, dtype=np.int64) - np.repeat(off.astype(np.int64), counts.astype(np.intp))
+    verts = t_pts[t_off[tid][owner] + local] * axes[owner]
+    tri_off = np.array(t_toff, np.int64)[tid]
+    tri_cnt = np.array(t_tcnt, np.int64)[tid]
+    return verts.astype(np.float32), off, counts, axes, np.concatenate(t_tris), tri_off, tri_cnt
+
+
+def hull_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int = 512, faces: bool = False):
     """Batched convex-hull workload: 2 hulls per pair (shape 2i, 2i+1),
-    centres U[0, 2]^3 m, uniform rotations."""
+    centres U[0, 2]^3 m, uniform rotations.  faces=True: hulls from
+    template_hulls, with triangles (keys tris, tri_off, tri_cnt)."""
     rng = np.random.default_rng(seed)
     n = 2 * n_pairs
-    verts, off, cnt, axes = ellipsoid_hulls(rng, n, nmin, nmax)
+    extra = {}
+    if faces:
+        verts, off, cnt, axes, tris, toff, tcnt = template_hulls(rng, n, nmin, nmax)
+        extra = dict(tris=tris, tri_off=toff, tri_cnt=tcnt)
+    else:
+        verts, off, cnt, axes = ellipsoid_hulls(rng, n, nmin, nmax)
     types = np.full(n, CONVEX, np.int32)
     params = np.zeros((n, 4), np.float32)
     pa = np.arange(0, n, 2, dtype=np.uint32)
@@ -127,7 +180,7 @@ def hull_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int =
     posea = poses(rng, n_pairs, 0.0, 2.0)
     poseb = poses(rng, n_pairs, 0.0, 2.0)
     return dict(types=types, params=params, verts=verts, vert_off=off, vert_cnt=cnt, axes=axes,
-                pair_a=pa, pair_b=pb, pose_a=posea, pose_b=poseb)
+                pair_a=pa, pair_b=pb, pose_a=posea, pose_b=poseb, **extra)
 
 
 def hull_faces(v: np.ndarray):
