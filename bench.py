@@ -1,16 +1,18 @@
 """Benchmark: batched convex-hull GJK/EPA pair queries per second on one MI355X.
 
 Workload (default): 1,048,576 pairs of convex hulls of N points on random
-ellipsoids (axes U[0.05, 0.4] m, N log-uniform in [8, 512]), centres
-U[0, 2]^3 m, uniform rotations; GJKSolver::shapeSignedDistance semantics
-(GJK distance, EPA depth when intersecting).  Inputs are resident in HBM
-before the timed region; one step = one batched query over all pairs.
+ellipsoids (axes U[0.05, 0.4] m, N log-uniform in [8, 512]) with their hull
+faces, so hulls above 32 vertices find support points by the vertex walk as
+fcl::Convex does; centres U[0, 2]^3 m, uniform rotations;
+GJKSolver::shapeSignedDistance semantics (GJK distance, EPA depth when
+intersecting).  Inputs are resident in HBM before the timed region; one
+step = one batched query over all pairs.
 
-Prints one JSON line.  Also reports the roofline of the dominant kernel
-(GJK stage, HBM-bound accounting: 12 B per hull vertex of both shapes +
-120 B of pose/index/result traffic per pair) from HIP events on the query
-stream, and the oracle's FP64 multithreaded CPU throughput on a bounded
-sample of the same workload.
+Prints one JSON line.  Also reports the roofline of the GJK stage (HBM
+accounting: 16 B per hull vertex record the support searches read plus 4 B
+of neighbour id, 184 B of ids, poses and result per pair) from HIP events on
+the query stream, and the oracle's FP64 multithreaded CPU throughput on a
+bounded sample of the same workload.
 """
 import argparse
 import json
@@ -30,52 +32,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def _faces_worker(args):
-    from mfcl import workloads as W
-
-    verts, off, cnt, idx = args
-    out = []
-    for i in idx:
-        pts = verts[off[i]:off[i] + cnt[i]]
-        out.append(W.hull_faces(pts))
-    return out
-
-
 def cpu_baseline(h, n_sample, threads, target_cpu_s=10.0):
     """Oracle FP64 shapeSignedDistance over the first n_sample pairs, with
-    the reference's hull support search (vertex-adjacency walk above 32
-    vertices); repeated until ~target_cpu_s of CPU work has been timed."""
+    the same hull faces as the device (vertex-adjacency walk above 32
+    vertices, as fcl::Convex); repeated until ~target_cpu_s of CPU work has
+    been timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
-    idx_hulls = np.arange(2 * n_sample)
-    chunks = np.array_split(idx_hulls, max(1, threads))
-    sub = (h["verts"][: int(h["vert_off"][2 * n_sample - 1] + h["vert_cnt"][2 * n_sample - 1])],
-           h["vert_off"], h["vert_cnt"])
-    t0 = time.time()
-    if threads > 1:
-        from concurrent.futures import ProcessPoolExecutor
-
-        with ProcessPoolExecutor(max_workers=threads) as ex:
-            parts = list(ex.map(_faces_worker, [(sub[0], sub[1], sub[2], c) for c in chunks]))
-    else:
-        parts = [_faces_worker((sub[0], sub[1], sub[2], c)) for c in chunks]
-    faces = [f for p in parts for f in p]
-    log(f"[cpu] qhull faces for {len(faces)} hulls in {time.time() - t0:.1f}s")
-    vs, ts, vo, vc, to, tc = [], [], [], [], [], []
-    o = q = 0
-    for i, (keep, tri) in enumerate(faces):
-        v = h["verts"][h["vert_off"][i]:h["vert_off"][i] + h["vert_cnt"][i]][keep]
-        vs.append(v)
-        ts.append(tri)
-        vo.append(o)
-        vc.append(len(v))
-        to.append(q)
-        tc.append(len(tri))
-        o += len(v)
-        q += len(tri)
-    S = pyoracle.ShapeSet(h["types"][: 2 * n_sample], h["params"][: 2 * n_sample], np.concatenate(vs), vo, vc,
-                          np.concatenate(ts), to, tc, threads=threads)
+    m = 2 * n_sample
+    nverts = int(h["vert_off"][m - 1] + h["vert_cnt"][m - 1])
+    S = pyoracle.ShapeSet(h["types"][:m], h["params"][:m], h["verts"][:nverts], h["vert_off"][:m],
+                          h["vert_cnt"][:m], h["tris"], h["tri_off"][:m], h["tri_cnt"][:m], threads=threads)
     ta = pyoracle.pose12(h["pose_a"][:n_sample])
     tb = pyoracle.pose12(h["pose_b"][:n_sample])
     pa, pb = h["pair_a"][:n_sample], h["pair_b"][:n_sample]
@@ -128,12 +96,13 @@ def main():
     from mfcl import workloads as W
 
     t0 = time.time()
-    h = W.hull_pairs(args.pairs, args.seed)
+    h = W.hull_pairs(args.pairs, args.seed, faces=True)
     n = args.pairs
     log(f"[gen] {n} pairs, {len(h['verts'])} hull vertices in {time.time() - t0:.1f}s")
     ctx = mfcl.Context(0)
     t0 = time.time()
-    ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"])
+    ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"],
+                      faces=mfcl.faces_from_triangles(h["tris"], h["tri_off"], h["tri_cnt"]))
     dpa, dpb = ctx.to_device(h["pair_a"]), ctx.to_device(h["pair_b"])
     dqa, dqb = ctx.to_device(h["pose_a"]), ctx.to_device(h["pose_b"])
     dout = ctx.alloc(n * mfcl.CONVEX_RESULT_DTYPE.itemsize)
@@ -147,8 +116,8 @@ def main():
         def run():
             ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, req, dout)
 
-        t_group = autotune.tune(ctx, autotune.GJK_GROUP, [1, 16, 32, 64], run, reps=2, log=log)
-        t_epa = autotune.tune(ctx, autotune.EPA_THREADS, [8192, 32768, 131072], run, reps=2, log=log)
+        t_group = autotune.tune(ctx, autotune.GJK_GROUP, [1, 32], run, reps=2, log=log)
+        t_epa = autotune.tune(ctx, autotune.EPA_THREADS, [32768, 131072], run, reps=2, log=log)
         tuned = {"gjk_group": min(t_group, key=t_group.get), "epa_threads": min(t_epa, key=t_epa.get)}
     ctx.set_timing(True)
     for _ in range(args.warmup):
@@ -168,7 +137,12 @@ def main():
     value = n * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
     nv = h["vert_cnt"].astype(np.int64)
-    alg_bytes = 12.0 * float(nv[h["pair_a"]].sum() + nv[h["pair_b"]].sum()) + 120.0 * n
+    # GJK stage, per launch: every hull vertex record the support searches
+    # read (16 B; counted by the library, EPA's share excluded below), one
+    # 4 B neighbour id per walked vertex, and per pair 8 B of ids + 56 B of
+    # poses read and the 120 B result written
+    gjk_vertices = float(counters[1]) - float(counters[4])
+    alg_bytes = 20.0 * gjk_vertices + 184.0 * n
     t_gjk = float(kern[:, 1].mean()) * 1e-3
     achieved = alg_bytes / t_gjk / 1e9
     log(f"[gpu] {ms_step:.3f} ms/step; kernels ms (order, gjk, epa) = {kern.mean(0)}; "
@@ -196,7 +170,8 @@ def main():
         "dtype": "f64 (f32 support scans)",
         "data": "synthetic",
         "config": {
-            "workload": f"{n} convex-hull pairs, N log-uniform [8,512] points on ellipsoids (axes U[0.05,0.4] m), "
+            "workload": f"{n} convex-hull pairs, N log-uniform [8,512] points on ellipsoids (axes U[0.05,0.4] m) "
+                        "with hull faces (support by the vertex walk above 32 vertices, as fcl::Convex), "
                         "centres U[0,2]^3, GJKSolver::shapeSignedDistance (GJK distance + EPA)",
             "pairs": n,
             "mean_vertices_per_hull": float(nv.mean()),
@@ -204,7 +179,7 @@ def main():
             "launch": tuned,
         },
         "roofline": {
-            "kernel": "narrowKernel (GJK stage)",
+            "kernel": "laneKernel (GJK stage)" if tuned.get("gjk_group", 1) == 1 else "narrowKernel (GJK stage)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -224,8 +199,10 @@ def main():
         rate, ref, info = cpu_baseline(h, min(args.cpu_pairs, n), threads)
         m = min(args.cpu_pairs, n)
         sign_mism = int(np.sum((ref["distance"] < 0) != (out["distance"][:m] < 0)))
+        not_exact = int(np.sum((ref["distance"] != out["distance"][:m]) | (ref["ok"] != out["ok"][:m])))
         result["cpu_baseline"] = {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
-                                  "sample": info["sample"], "sample_sign_mismatches_vs_gpu": sign_mism}
+                                  "sample": info["sample"], "sample_sign_mismatches_vs_gpu": sign_mism,
+                                  "sample_pairs_not_bit_identical_to_gpu": not_exact}
         result["speedup_vs_cpu"] = value / rate
     del h
     if not args.no_mesh:

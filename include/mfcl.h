@@ -115,6 +115,9 @@ int mfcl_last_timing(mfcl_ctx* ctx, float* ms6);
  * scanned by support mapping, [2] EPA pairs, [3] EPA pairs rerun at full
  * polytope capacity after exhausting the on-chip polytope. */
 int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4);
+/* The same counters and more, the first n of: [0..3] as above, [4] hull
+ * vertices read by the EPA stage alone (entries beyond 4 read 0). */
+int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n);
 
 /* Launch-shape knobs (no counterpart in the reference: GPU scheduling
  * parameters, chosen by the autotuner in mind-fcl_amd/mfcl/autotune.py). */

@@ -173,8 +173,7 @@ int mfcl_last_timing(mfcl_ctx* ctx, float* ms6) {
   return MFCL_OK;
 }
 
-int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4) {
-  if (!ctx || !c4) return MFCL_E_INVALID;
+static int refreshCounters(mfcl_ctx* ctx) {
   if (ctx->d_counters) {
     MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     uint64_t dev[2] = {0, 0};
@@ -184,8 +183,22 @@ int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4) {
     ctx->counters[1] = dev[0] + dev[1];
     ctx->counters[2] = q[0];
     ctx->counters[3] = q[1];
+    ctx->counters[4] = dev[1];
   }
-  std::memcpy(c4, ctx->counters, sizeof(ctx->counters));
+  return MFCL_OK;
+}
+
+int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4) {
+  if (!ctx || !c4) return MFCL_E_INVALID;
+  if (int rc = refreshCounters(ctx)) return rc;
+  std::memcpy(c4, ctx->counters, 4 * sizeof(uint64_t));
+  return MFCL_OK;
+}
+
+int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n) {
+  if (!ctx || !c || n < 0) return MFCL_E_INVALID;
+  if (int rc = refreshCounters(ctx)) return rc;
+  for (int32_t k = 0; k < n; ++k) c[k] = k < 5 ? ctx->counters[k] : 0;
   return MFCL_OK;
 }
 

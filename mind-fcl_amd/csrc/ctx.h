@@ -64,7 +64,7 @@ struct mfcl_ctx {
   int max_lds_per_block{65536};  // device limit on dynamic + static LDS per workgroup
   // launch-shape knobs (set by mfcl_tune / the autotuner)
   struct {
-    int gjk_group{32};          // lanes per pair in the GJK stage (1, 16, 32, 64)
+    int gjk_group{1};           // lanes per pair in the GJK stage (1, 16, 32, 64)
     int epa_threads{131072};    // persistent EPA lanes (multiple of 256): 64 per wavefront
     int guess_policy{1};        // warm start: 0 MLP direction for every pair, 1 not for predicted penetration
     int epa_lds_faces{64};      // face capacity of the LDS polytope (overflow reruns at full size)
@@ -114,7 +114,7 @@ struct mfcl_ctx {
   bool timing{false};
   hipEvent_t ev[16]{};
   float last_ms[6]{0, 0, 0, 0, 0, 0};
-  uint64_t counters[4]{0, 0, 0, 0};
+  uint64_t counters[5]{0, 0, 0, 0, 0};
   uint64_t* d_counters{nullptr};  // device-side work counters
 };
 

@@ -478,12 +478,10 @@ inline size_t epaSliceBytes(int cap_faces) { return kCfgBytes + kRawBytes + kPol
 
 __global__ void __launch_bounds__(kBlock) epaWaveKernel(ShapeTables T, PairIO io, Params P, const QEntry* queue,
                                                         const uint32_t* qcount, int cap_faces, size_t slice_bytes,
-                                                        QEntry* overflow, uint32_t* ocount,
+                                                        QEntry* overflow, uint32_t* ocount, uint32_t* next,
                                                         unsigned long long* scanned_total) {
   extern __shared__ __align__(16) char lds_arena[];
   const int lane = threadIdx.x & 63;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   const ArenaDims dims = arenaDims(cap_faces);
   MFCL_LDS char* slice = (MFCL_LDS char*)(lds_arena) + static_cast<size_t>(threadIdx.x >> 6) * slice_bytes;
   MFCL_LDS MinkCfg* cfg = (MFCL_LDS MinkCfg*)(slice);
@@ -500,7 +498,14 @@ __global__ void __launch_bounds__(kBlock) epaWaveKernel(ShapeTables T, PairIO io
   poly.nf = dims.nf;
   const uint32_t count = *qcount;
   int scanned = 0;
-  for (uint32_t q = wave; q < count; q += nwaves) {
+  // pairs are taken one at a time from a shared counter: EPA iteration
+  // counts vary widely (p99 / mean ~ 3), so a static share per wavefront
+  // would leave the grid waiting on its most loaded wavefronts
+  for (;;) {
+    uint32_t q = 0;
+    if (lane == 0) q = atomicAdd(next, 1u);
+    q = __shfl(q, 0);
+    if (q >= count) break;
     const QEntry e = queue[q];
     const uint32_t i = e.pair;
     DevShape A, B;
@@ -636,8 +641,9 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   P.use_guess = req->use_guess;
   P.n_shapes = ctx->n_shapes;
 
-  // workspace: hist + offs (2 x 128 x 4 B), qcount at byte 1024, work
-  // counters at byte 1536; keys (n B) after the order array (4n B); EPA queue; EPA arenas
+  // workspace: hist + offs (2 x 128 x 4 B), EPA queue count at byte 1024,
+  // overflow count at 1028, EPA work cursors at 1032 / 1036, work counters
+  // at byte 1536 (all cleared by zeroKernel); keys (n B) after the order array (4n B); EPA queue; EPA arenas
   const size_t misc_bytes = 2048;
   if (int rc = ensure(ctx, ctx->ws_misc, misc_bytes)) return rc;
   if (int rc = ensure(ctx, ctx->ws_order, n * (sizeof(uint32_t) + 1) + 256)) return rc;
@@ -703,11 +709,12 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
     while (cap_small > 4 && 4 * epaSliceBytes(cap_small) > static_cast<size_t>(ctx->max_lds_per_block)) cap_small /= 2;
     const size_t slice = epaSliceBytes(cap_small);
     const unsigned eb = static_cast<unsigned>(ctx->tune.epa_threads / kBlock);
-    epaWaveKernel<<<eb, kBlock, 4 * slice, s>>>(T, io, P, queue, qcount, cap_small, slice, oflow, ocount, sc + 1);
+    epaWaveKernel<<<eb, kBlock, 4 * slice, s>>>(T, io, P, queue, qcount, cap_small, slice, oflow, ocount, ocount + 1,
+                                                sc + 1);
     // overflow pairs: full capacity, one wavefront per block
     if (cap_small < P.epa_max_faces) {
       const unsigned ob = static_cast<unsigned>(ctx->tune.epa_overflow_waves);
-      epaWaveKernel<<<ob, 64, full_slice, s>>>(T, io, P, oflow, ocount, P.epa_max_faces, full_slice, oflow, ocount,
+      epaWaveKernel<<<ob, 64, full_slice, s>>>(T, io, P, oflow, ocount, P.epa_max_faces, full_slice, oflow, ocount, ocount + 2,
                                                sc + 1);
     }
   }

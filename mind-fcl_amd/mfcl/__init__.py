@@ -85,6 +85,7 @@ def load_library() -> C.CDLL:
     lib.mfcl_set_timing.argtypes = [p, C.c_int]
     lib.mfcl_last_timing.argtypes = [p, p]
     lib.mfcl_last_counters.argtypes = [p, p]
+    lib.mfcl_last_counters_n.argtypes = [p, p, i32]
     lib.mfcl_request_defaults.argtypes = [C.POINTER(Request), i32]
     lib.mfcl_request_defaults.restype = None
     lib.mfcl_dev_alloc.argtypes = [p, u64, C.POINTER(p)]
@@ -202,8 +203,10 @@ class Context:
         return ms
 
     def last_counters(self) -> np.ndarray:
-        c = np.zeros(4, np.uint64)
-        self._check(self.lib.mfcl_last_counters(self.h, _ptr(c)))
+        """[pairs, hull vertices read (GJK + EPA stages), EPA pairs, EPA
+        reruns at full capacity, hull vertices read by the EPA stage]"""
+        c = np.zeros(5, np.uint64)
+        self._check(self.lib.mfcl_last_counters_n(self.h, _ptr(c), 5))
         return c
 
     def alloc(self, nbytes: int) -> DeviceArray:

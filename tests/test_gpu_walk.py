@@ -36,7 +36,7 @@ def _run(ctx, h, req, group):
     try:
         return ctx.convex_query(h["pair_a"], h["pair_b"], h["pose_a"], h["pose_b"], req)
     finally:
-        autotune.set_knob(ctx, autotune.GJK_GROUP, 32)
+        autotune.set_knob(ctx, autotune.GJK_GROUP, 1)
 
 
 FIELDS_SD = ("ok", "gjk_status", "epa_status", "gjk_iters", "epa_iters", "distance", "pa", "pb")
