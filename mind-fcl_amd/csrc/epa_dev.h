@@ -45,9 +45,16 @@ __host__ __device__ inline ArenaDims arenaDims(int max_faces) {
   d.nf = max_faces;
   return d;
 }
+// Scratch record per border edge of one expansion (see Poly::expand).
+struct BorderRec {
+  int32_t pre;     // elements (edges and faces) created before this edge's face
+  int16_t e, rank; // edge slot, rank in descending stamp order
+  int16_t fe, ff;  // the list of free edge / face slots (entry k: the k-th free slot)
+  int8_t t0, t1;   // lower-ranked border edges sharing endpoint v0 / v1
+};
 __host__ __device__ inline size_t arenaBytes(int max_faces) {
   const ArenaDims d = arenaDims(max_faces);
-  size_t b = sizeof(EVert) * d.nv + sizeof(EEdge) * d.nv + sizeof(EFace) * d.nf;
+  size_t b = sizeof(EVert) * d.nv + sizeof(EEdge) * d.nv + sizeof(EFace) * d.nf + sizeof(BorderRec) * d.nv;
   return (b + 255) & ~static_cast<size_t>(255);
 }
 
@@ -143,6 +150,7 @@ struct Poly {
   MFCL_LDS EVert* V;
   MFCL_LDS EEdge* E;
   MFCL_LDS EFace* F;
+  MFCL_LDS BorderRec* BL;  // nv records
   int nv, nf;   // slot capacities: vertices and edges nv, faces nf
   int stamp;    // creation counter
 
@@ -417,39 +425,151 @@ struct Poly {
     __builtin_amdgcn_wave_barrier();
     const int16_t nvi = addVertex(nv_, nd);
     if (nvi == kNil) return 2;
-    // new faces over the border edges, newest border edge first
-    bool ok = true;
-    int last = 0x7fffffff;
-    while (true) {
-      int bs = -1, bi = kNil;
-      for (int s = l; s < nv; s += 64) {
-        const MFCL_LDS EEdge& e = E[s];
-        const int st = e.stamp;
-        if (st < 0 || e.vis != kBorder || st >= last) continue;
-        if (st > bs) { bs = st; bi = s; }
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const int os = __shfl_xor(bs, off), oi = __shfl_xor(bi, off);
-        if (os > bs) { bs = os; bi = oi; }
-      }
-      if (bs < 0) break;
-      last = bs;
-      const int16_t i = static_cast<int16_t>(bi);
-      int16_t ne[2];
-      const int16_t vs[2] = {E[i].v0, E[i].v1};
-      for (int q = 0; q < 2; ++q) {
-        MFCL_LDS EVert& vq = V[vs[q]];
-        if (vq.new_edge == kNil) {
-          const int16_t x = addEdge(nvi, vs[q]);
-          vq.new_edge = x;
-          __builtin_amdgcn_wave_barrier();
-        }
-        ne[q] = vq.new_edge;
-      }
-      if (addFace(i, ne[0], ne[1]) == kNil) ok = false;
+    return coverBorder(nvi);
+  }
+
+  // New faces over the border edges (epa_polytope_expand.hpp:233-281).  The
+  // reference walks the border edges newest first; for each it creates the
+  // edge (new vertex, endpoint) unless an earlier border edge already did,
+  // then the face, each taking the next creation stamp; a new edge's first
+  // face becomes its f0, the second its f1, a third is an error.  All of
+  // that follows from each border edge's rank in that order, so the lanes
+  // build the faces together: rank and sharing counts by comparison with the
+  // other border edges, creation positions by a prefix over the ranks, slots
+  // from the free lists (which free slot is used does not matter).  A slot
+  // shortage fails the expansion as the reference's allocation failure does.
+  MFCL_D int coverBorder(int16_t nvi) {
+    const int l = __lane_id();
+    const unsigned long long below = (1ull << l) - 1;
+    int nb = 0;
+    for (int base = 0; base < nv; base += 64) {
+      const int s = base + l;
+      const bool isb = s < nv && E[s].stamp >= 0 && E[s].vis == kBorder;
+      const unsigned long long m = __ballot(isb);
+      if (isb) BL[nb + __popcll(m & below)].e = static_cast<int16_t>(s);
+      nb += __popcll(m);
     }
-    return ok ? 0 : 2;
+    __builtin_amdgcn_wave_barrier();
+    for (int j = l; j < nb; j += 64) {
+      const int st = E[BL[j].e].stamp;
+      int rank = 0;
+      for (int k = 0; k < nb; ++k) rank += E[BL[k].e].stamp > st;
+      BL[j].rank = static_cast<int16_t>(rank);
+    }
+    __builtin_amdgcn_wave_barrier();
+    bool bad = false;
+    int new_edges = 0;
+    for (int j = l; j < nb; j += 64) {
+      const MFCL_LDS EEdge& ej = E[BL[j].e];
+      const int r = BL[j].rank, a = ej.v0, b = ej.v1;
+      int t0 = 0, t1 = 0;
+      for (int k = 0; k < nb; ++k) {
+        if (BL[k].rank >= r) continue;
+        const MFCL_LDS EEdge& ek = E[BL[k].e];
+        t0 += ek.v0 == a || ek.v1 == a;
+        t1 += ek.v0 == b || ek.v1 == b;
+      }
+      if (t0 > 1 || t1 > 1) bad = true;  // a third face on a new edge
+      BL[j].t0 = static_cast<int8_t>(t0 > 1 ? 2 : t0);
+      BL[j].t1 = static_cast<int8_t>(t1 > 1 ? 2 : t1);
+      new_edges += (t0 == 0) + (t1 == 0);
+    }
+    if (anyLane(bad)) return 2;
+    __builtin_amdgcn_wave_barrier();
+    for (int j = l; j < nb; j += 64) {
+      const int r = BL[j].rank;
+      int pre = 0;
+      for (int k = 0; k < nb; ++k)
+        if (BL[k].rank < r) pre += 1 + (BL[k].t0 == 0) + (BL[k].t1 == 0);
+      BL[j].pre = pre;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) new_edges += __shfl_xor(new_edges, off);
+    // capacity
+    int free_e = 0, free_f = 0;
+    for (int base = 0; base < nv; base += 64) free_e += __popcll(__ballot(base + l < nv && E[base + l].stamp < 0));
+    for (int base = 0; base < nf; base += 64) free_f += __popcll(__ballot(base + l < nf && F[base + l].stamp < 0));
+    if (new_edges > free_e || nb > free_f) return 2;
+    // the free slot lists (before any slot is taken)
+    {
+      int ce = 0, cf = 0;
+      for (int base = 0; base < nv; base += 64) {
+        const int s = base + l;
+        const bool fr = s < nv && E[s].stamp < 0;
+        const unsigned long long m = __ballot(fr);
+        const int pos = ce + __popcll(m & below);
+        if (fr && pos < new_edges) BL[pos].fe = static_cast<int16_t>(s);
+        ce += __popcll(m);
+      }
+      for (int base = 0; base < nf; base += 64) {
+        const int s = base + l;
+        const bool fr = s < nf && F[s].stamp < 0;
+        const unsigned long long m = __ballot(fr);
+        const int pos = cf + __popcll(m & below);
+        if (fr && pos < nb) BL[pos].ff = static_cast<int16_t>(s);
+        cf += __popcll(m);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // new edges: the k-th created edge takes the k-th free edge slot
+    for (int j = l; j < nb; j += 64) {
+      const MFCL_LDS BorderRec& bj = BL[j];
+      const MFCL_LDS EEdge& ej = E[bj.e];
+      const int edges_before = bj.pre - bj.rank;
+      int q = 0;
+      for (int side = 0; side < 2; ++side) {
+        if ((side ? bj.t1 : bj.t0) != 0) continue;
+        const int16_t vq = side ? ej.v1 : ej.v0;
+        const int16_t slot = BL[edges_before + q].fe;
+        MFCL_LDS EEdge& x = E[slot];
+        x.v0 = nvi;
+        x.v1 = vq;
+        x.f0 = x.f1 = kNil;
+        x.vis = kUnknown;
+        MinD m;
+        segDist(d3{0, 0, 0}, vloc(nvi), vloc(vq), m);
+        x.in = m.in; x.d2 = m.d2; x.wx = m.w.x; x.wy = m.w.y; x.wz = m.w.z;
+        x.stamp = stamp + bj.pre + q;
+        V[vq].new_edge = slot;
+        ++q;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // faces: the face of rank r takes the r-th free face slot
+    for (int j = l; j < nb; j += 64) {
+      const MFCL_LDS BorderRec& bj = BL[j];
+      const int16_t ei = bj.e;
+      MFCL_LDS EEdge& ej = E[ei];
+      const int16_t a = ej.v0, b = ej.v1;
+      const int16_t ne0 = V[a].new_edge, ne1 = V[b].new_edge;
+      const int16_t slot = BL[bj.rank].ff;
+      MFCL_LDS EFace& f = F[slot];
+      f.e0 = ei; f.e1 = ne0; f.e2 = ne1;
+      f.vis = kFUnknown;
+      MinD m;
+      triDist0(vloc(a), vloc(b), vloc(nvi), m);
+      f.in = m.in; f.d2 = m.d2; f.wx = m.w.x; f.wy = m.w.y; f.wz = m.w.z;
+      f.stamp = stamp + bj.pre + (bj.t0 == 0) + (bj.t1 == 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // links: the border edge keeps its hidden face as f0; each new edge gets
+    // its lower-ranked face as f0 and the other as f1
+    bool clash = false;
+    for (int j = l; j < nb; j += 64) {
+      const MFCL_LDS BorderRec& bj = BL[j];
+      MFCL_LDS EEdge& ej = E[bj.e];
+      const int16_t slot = BL[bj.rank].ff;
+      if (ej.f0 == kNil) ej.f0 = slot;
+      else if (ej.f1 != kNil) clash = true;
+      else ej.f1 = slot;
+      MFCL_LDS EEdge& x0 = E[V[ej.v0].new_edge];
+      MFCL_LDS EEdge& x1 = E[V[ej.v1].new_edge];
+      if (bj.t0 == 0) x0.f0 = slot; else x0.f1 = slot;
+      if (bj.t1 == 0) x1.f0 = slot; else x1.f1 = slot;
+    }
+    stamp += new_edges + nb;
+    __builtin_amdgcn_wave_barrier();
+    return anyLane(clash) ? 2 : 0;
   }
 
  private:

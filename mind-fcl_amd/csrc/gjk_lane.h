@@ -254,8 +254,7 @@ MFCL_D void laneSupport(bool need, const MinkLane& md, d3 d, int id, MV& out, ui
   for (int k = 0; k < 2; ++k) {
     const bool hull = need && (k ? md.type1 : md.type0) == kConvex;
     const uint32_t si = k ? md.ib : md.ia;
-    const bool walk = hull && md.T.nbr_off && md.T.nbr_off[si] >= 0;
-    const bool want = hull && !walk;
+    const bool want = hull && !(md.T.nbr_off && md.T.nbr_off[si] >= 0);
     uint64_t vp = 0;
     int nv = 0;
     float rb = 0;
@@ -266,13 +265,24 @@ MFCL_D void laneSupport(bool need, const MinkLane& md, d3 d, int id, MV& out, ui
       rb = md.T.param[si].w;
     }
     const d3 dd = k ? d1 : d;
-    // naive hulls: cooperative scans; walking hulls: each lane walks its own
+    // naive hulls: cooperative scans; walking hulls: each lane walks its
+    // own (below)
     uint64_t m = __ballot(want);
     int res = -1;
     while (m) coopScanBatch<kScanBatch>(m, vp, nv, rb, dd, md.lane, res, scanned);
-    if (walk) res = hullWalk(loadShape(md.T, si), dd, &walked);
     if (k) idx1 = res;
     else idx0 = res;
+  }
+  {
+    const bool w0 = need && md.type0 == kConvex && md.T.nbr_off && md.T.nbr_off[md.ia] >= 0;
+    const bool w1 = need && md.type1 == kConvex && md.T.nbr_off && md.T.nbr_off[md.ib] >= 0;
+    if (w0 || w1) {
+      const DevShape s0 = loadShape(md.T, md.ia), s1 = loadShape(md.T, md.ib);
+      // one walk after the other: the lanes' independent walks already keep
+      // enough memory requests in flight, and lock-step pairs cost registers
+      if (w0) idx0 = hullWalk(s0, d, &walked);
+      if (w1) idx1 = hullWalk(s1, d1, &walked);
+    }
   }
   if (need) {
     d3 a, x;

@@ -192,15 +192,33 @@ __device__ __attribute__((noinline)) int hullArgmaxOOL(const float4* verts, cons
 // v_C.dot(vertices[n]).  Every step strictly increases the value, so the walk
 // ends (after at most nv steps; at once for a NaN direction).
 //
-// Latency: a step fetches the neighbour record (count and up to 8 ids; the
-// tables are followed by 32 ints of padding, so the fixed-size reads stay in
-// bounds), then the neighbours' vertices and their own record offsets
-// together, and only then runs the reference's sequential comparisons in
-// registers -- two dependent memory round trips per step instead of one per
-// neighbour.
+// Latency: a step reads the current vertex's neighbour record (its count and
+// up to kWalkChunk ids in one batch; the tables are followed by 32 ints of
+// padding, so the fixed-size read stays in bounds), then the neighbours'
+// vertices and their own record offsets in a second batch, and only then
+// runs the reference's sequential comparisons in registers: two dependent
+// memory round trips per step.  The two walks of one support query (one per
+// shape) advance in lock step so their round trips overlap.
 constexpr int kWalkChunk = 8;
 
-MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
+struct Walk {
+  d3 d;
+  double ext_v;
+  int ext, start, parent0, parent1, count, visited;
+  bool keep;
+  int id[kWalkChunk], st[kWalkChunk];
+  float3 w[kWalkChunk];
+};
+
+MFCL_D void walkInit(Walk& W, const DevShape& s, d3 d, bool active) {
+  W.d = d;
+  W.keep = active;
+  W.visited = 0;
+  W.ext = 0;
+  W.ext_v = 0;
+  W.start = 0;
+  W.count = 0;
+  if (!active) return;
   int init = -1;
   double max_dot = 0;
 #pragma unroll
@@ -216,50 +234,98 @@ MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
       max_dot = dv;
     }
   }
-  int ext = init;
-  float3 q = hullVert(s, ext);
-  double ext_v = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
-  int start = s.nbr[ext];
-  int parent0 = init, parent1 = init, visited = 1;
-  bool keep = true;
-  while (keep) {
-    keep = false;
-    const int old = ext;
-    int next_start = start;
-    const int count = s.nbr[start];
-    for (int c0 = 0; c0 < count; c0 += kWalkChunk) {
-      int id[kWalkChunk], st[kWalkChunk];
-      float3 w[kWalkChunk];
+  W.ext = init;
+  const float3 q = hullVert(s, init);
+  W.ext_v = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
+  W.start = s.nbr[init];
+  W.parent0 = W.parent1 = init;
+  W.visited = 1;
+}
+// first batch: the neighbour record of the current vertex
+MFCL_D void walkFetchRecord(Walk& W, const DevShape& s) {
+  if (!W.keep) return;
+  W.count = s.nbr[W.start];
+#pragma unroll
+  for (int j = 0; j < kWalkChunk; ++j) W.id[j] = s.nbr[W.start + 1 + j];
+}
+// second batch: the neighbours' vertices and record offsets
+MFCL_D void walkFetchNeighbours(Walk& W, const DevShape& s) {
+  if (!W.keep) return;
+#pragma unroll
+  for (int j = 0; j < kWalkChunk; ++j) {
+    const int v = j < W.count ? W.id[j] : W.ext;  // in range for the loads
+    W.id[j] = v;
+    W.w[j] = hullVert(s, v);
+    W.st[j] = s.nbr[v];
+  }
+}
+// the reference's comparisons over the record (neighbours beyond the first
+// kWalkChunk, rare on hulls, are read here)
+MFCL_D void walkCompare(Walk& W, const DevShape& s) {
+  if (!W.keep) return;
+  W.keep = false;
+  const int old = W.ext;
+  int next_start = W.start;
+  for (int c0 = 0; c0 < W.count; c0 += kWalkChunk) {
+    if (c0 > 0) {
 #pragma unroll
       for (int j = 0; j < kWalkChunk; ++j) {
-        const int nb = s.nbr[start + 1 + c0 + j];
-        id[j] = c0 + j < count ? nb : old;  // in range for the loads below
-      }
-#pragma unroll
-      for (int j = 0; j < kWalkChunk; ++j) {
-        w[j] = hullVert(s, id[j]);
-        st[j] = s.nbr[id[j]];
-      }
-#pragma unroll
-      for (int j = 0; j < kWalkChunk; ++j) {
-        if (c0 + j >= count || id[j] == parent0 || id[j] == parent1) continue;
-        const double v = d.x * static_cast<double>(w[j].x) + d.y * static_cast<double>(w[j].y) +
-                         d.z * static_cast<double>(w[j].z);
-        ++visited;
-        if (v > ext_v) {
-          parent1 = ext;
-          keep = true;
-          ext = id[j];
-          ext_v = v;
-          next_start = st[j];
-        }
+        const int nb = s.nbr[W.start + 1 + c0 + j];
+        const int v = c0 + j < W.count ? nb : old;
+        W.id[j] = v;
+        W.w[j] = hullVert(s, v);
+        W.st[j] = s.nbr[v];
       }
     }
-    parent0 = old;
-    start = next_start;
+#pragma unroll
+    for (int j = 0; j < kWalkChunk; ++j) {
+      if (c0 + j >= W.count || W.id[j] == W.parent0 || W.id[j] == W.parent1) continue;
+      const double v = W.d.x * static_cast<double>(W.w[j].x) + W.d.y * static_cast<double>(W.w[j].y) +
+                       W.d.z * static_cast<double>(W.w[j].z);
+      ++W.visited;
+      if (v > W.ext_v) {
+        W.parent1 = W.ext;
+        W.keep = true;
+        W.ext = W.id[j];
+        W.ext_v = v;
+        next_start = W.st[j];
+      }
+    }
   }
-  *scanned += visited;
-  return ext;
+  W.parent0 = old;
+  W.start = next_start;
+}
+
+MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
+  Walk W;
+  walkInit(W, s, d, true);
+  while (W.keep) {
+    walkFetchRecord(W, s);
+    walkFetchNeighbours(W, s);
+    walkCompare(W, s);
+  }
+  *scanned += W.visited;
+  return W.ext;
+}
+
+// Two walks (either may be inactive) advanced together; ia / ib receive
+// the extreme vertices of the active ones.
+MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevShape& sb, d3 db, bool act_b,
+                      int& ib, int* scanned) {
+  Walk A, B;
+  walkInit(A, sa, da, act_a);
+  walkInit(B, sb, db, act_b);
+  while (A.keep || B.keep) {
+    walkFetchRecord(A, sa);
+    walkFetchRecord(B, sb);
+    walkFetchNeighbours(A, sa);
+    walkFetchNeighbours(B, sb);
+    walkCompare(A, sa);
+    walkCompare(B, sb);
+  }
+  if (act_a) ia = A.ext;
+  if (act_b) ib = B.ext;
+  *scanned += A.visited + B.visited;
 }
 
 template <int G>
@@ -444,14 +510,35 @@ struct MinkL {
     reloadFence();
     return apply(tf0(), x);
   }
-  MFCL_D d3 support(d3 d) { return support0(d) - support1(-d); }
+  // support points of both shapes (vertex indices, -1 for a primitive);
+  // two walking hulls walk together
+  MFCL_D void supportPair(d3 d, d3& a, d3& x, int& i0, int& i1) {
+    reloadFence();
+    const DevShape s0 = sh0(), s1 = sh1();
+    const d3 d1 = mul(rot1(), -d);
+    if (s0.nbr && s1.nbr) {
+      hullWalk2(s0, d, true, i0, s1, d1, true, i1, &scanned);
+      const float3 q0 = hullVert(s0, i0), q1 = hullVert(s1, i1);
+      a = d3{q0.x, q0.y, q0.z};
+      x = d3{q1.x, q1.y, q1.z};
+      return;
+    }
+    a = shapeSupportI<G>(s0, d, lg, &scanned, i0);
+    reloadFence();
+    x = shapeSupportI<G>(sh1(), d1, lg, &scanned, i1);
+  }
+  MFCL_D d3 support(d3 d) {
+    d3 a, x;
+    int i0, i1;
+    supportPair(d, a, x, i0, i1);
+    reloadFence();
+    return a - apply(tf0(), x);
+  }
   // support(d), recording d and the support vertex indices in slot id.
   MFCL_D d3 supportNew(d3 d, int id) {
     int i0, i1;
-    reloadFence();
-    const d3 a = shapeSupportI<G>(sh0(), d, lg, &scanned, i0);
-    reloadFence();
-    const d3 x = shapeSupportI<G>(sh1(), mul(rot1(), -d), lg, &scanned, i1);
+    d3 a, x;
+    supportPair(d, a, x, i0, i1);
     reloadFence();
     const d3 b = apply(tf0(), x);
     MFCL_LDS DirSlot& sl = c->slot[id & (kDirSlots - 1)];

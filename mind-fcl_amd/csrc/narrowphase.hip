@@ -476,7 +476,7 @@ constexpr size_t kRawBytes = (sizeof(RawF) + 15) & ~static_cast<size_t>(15);
 constexpr size_t kPolyBytes = (sizeof(Poly) + 15) & ~static_cast<size_t>(15);
 inline size_t epaSliceBytes(int cap_faces) { return kCfgBytes + kRawBytes + kPolyBytes + arenaBytes(cap_faces); }
 
-__global__ void __launch_bounds__(kBlock) epaWaveKernel(ShapeTables T, PairIO io, Params P, const QEntry* queue,
+__global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO io, Params P, const QEntry* queue,
                                                         const uint32_t* qcount, int cap_faces, size_t slice_bytes,
                                                         QEntry* overflow, uint32_t* ocount, uint32_t* next,
                                                         unsigned long long* scanned_total) {
@@ -494,6 +494,8 @@ __global__ void __launch_bounds__(kBlock) epaWaveKernel(ShapeTables T, PairIO io
   poly.V = (MFCL_LDS EVert*)(base);
   poly.E = (MFCL_LDS EEdge*)(base + sizeof(EVert) * dims.nv);
   poly.F = (MFCL_LDS EFace*)(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv);
+  poly.BL = (MFCL_LDS BorderRec*)(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv +
+                                  sizeof(EFace) * dims.nf);
   poly.nv = dims.nv;
   poly.nf = dims.nf;
   const uint32_t count = *qcount;
