@@ -248,7 +248,9 @@ MFCL_D void walkFetchRecord(Walk& W, const DevShape& s) {
 #pragma unroll
   for (int j = 0; j < kWalkChunk; ++j) W.id[j] = s.nbr[W.start + 1 + j];
 }
-// second batch: the neighbours' vertices and record offsets
+// second batch: the neighbours' vertices, and with kPrefetch their record
+// offsets (one round trip less per step for eight more reads)
+template <bool kPrefetch>
 MFCL_D void walkFetchNeighbours(Walk& W, const DevShape& s) {
   if (!W.keep) return;
 #pragma unroll
@@ -256,11 +258,12 @@ MFCL_D void walkFetchNeighbours(Walk& W, const DevShape& s) {
     const int v = j < W.count ? W.id[j] : W.ext;  // in range for the loads
     W.id[j] = v;
     W.w[j] = hullVert(s, v);
-    W.st[j] = s.nbr[v];
+    if (kPrefetch) W.st[j] = s.nbr[v];
   }
 }
 // the reference's comparisons over the record (neighbours beyond the first
 // kWalkChunk, rare on hulls, are read here)
+template <bool kPrefetch>
 MFCL_D void walkCompare(Walk& W, const DevShape& s) {
   if (!W.keep) return;
   W.keep = false;
@@ -274,7 +277,7 @@ MFCL_D void walkCompare(Walk& W, const DevShape& s) {
         const int v = c0 + j < W.count ? nb : old;
         W.id[j] = v;
         W.w[j] = hullVert(s, v);
-        W.st[j] = s.nbr[v];
+        if (kPrefetch) W.st[j] = s.nbr[v];
       }
     }
 #pragma unroll
@@ -288,21 +291,26 @@ MFCL_D void walkCompare(Walk& W, const DevShape& s) {
         W.keep = true;
         W.ext = W.id[j];
         W.ext_v = v;
-        next_start = W.st[j];
+        if (kPrefetch) next_start = W.st[j];
       }
     }
   }
   W.parent0 = old;
-  W.start = next_start;
+  if (kPrefetch) W.start = next_start;
+  else if (W.keep) W.start = s.nbr[W.ext];
 }
 
+// kPrefetch: where few walks run at once (one pair per group or wavefront)
+// latency decides; the lane kernel's many concurrent walks are bound by
+// memory traffic instead and skip the prefetch.
+template <bool kPrefetch>
 MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
   Walk W;
   walkInit(W, s, d, true);
   while (W.keep) {
     walkFetchRecord(W, s);
-    walkFetchNeighbours(W, s);
-    walkCompare(W, s);
+    walkFetchNeighbours<kPrefetch>(W, s);
+    walkCompare<kPrefetch>(W, s);
   }
   *scanned += W.visited;
   return W.ext;
@@ -318,10 +326,10 @@ MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevS
   while (A.keep || B.keep) {
     walkFetchRecord(A, sa);
     walkFetchRecord(B, sb);
-    walkFetchNeighbours(A, sa);
-    walkFetchNeighbours(B, sb);
-    walkCompare(A, sa);
-    walkCompare(B, sb);
+    walkFetchNeighbours<true>(A, sa);
+    walkFetchNeighbours<true>(B, sb);
+    walkCompare<true>(A, sa);
+    walkCompare<true>(B, sb);
   }
   if (act_a) ia = A.ext;
   if (act_b) ib = B.ext;
@@ -330,7 +338,7 @@ MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevS
 
 template <int G>
 MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
-  if (s.nbr) return hullWalk(s, d, scanned);
+  if (s.nbr) return hullWalk<true>(s, d, scanned);
   const int r = hullArgmaxOOL<G>(s.verts, s.lv, s.ls, s.nv, s.rbound, d.x, d.y, d.z, lg);
   *scanned += (r >> 30) ? 2 * s.nv : s.nv;
   return r & ((1 << 30) - 1);

@@ -349,7 +349,7 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
 // entries and scan counts equal those of narrowKernel.  dir_ws holds
 // kLaneDirDoubles per wavefront of the grid.
 template <int MODE>
-__global__ void __launch_bounds__(kBlock, 3) laneKernel(ShapeTables T, PairIO io, const uint32_t* order, uint64_t n,
+__global__ void __launch_bounds__(kBlock) laneKernel(ShapeTables T, PairIO io, const uint32_t* order, uint64_t n,
                                                      Params P, QEntry* queue, uint32_t* qcount,
                                                      unsigned long long* scanned_total, double* dir_ws) {
   __shared__ LaneWs ws[kBlock / 64];

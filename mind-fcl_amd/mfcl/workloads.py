@@ -114,8 +114,19 @@ def ellipsoid_hulls(rng: np.random.Generator, n_hulls: int, nmin: int = 8, nmax:
     return verts, off, counts, axes
 
 
+def morton_order(u: np.ndarray) -> np.ndarray:
+    """Permutation sorting points of [-1, 1]^3 along a Morton (Z-order)
+    curve of their 10-bit quantised coordinates."""
+    q = np.clip(((u + 1.0) * 511.5).astype(np.int64), 0, 1023)
+    code = np.zeros(len(u), np.int64)
+    for b in range(10):
+        for a in range(3):
+            code |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    return np.argsort(code, kind="stable")
+
+
 def template_hulls(rng: np.random.Generator, n_hulls: int, nmin: int = 8, nmax: int = 512, amin: float = 0.05,
-                   amax: float = 0.4, variants: int = 0):
+                   amax: float = 0.4, variants: int = 0, vertex_order: str = "random"):
     """Convex hulls with faces: N = round(exp(U[ln nmin, ln nmax])) points
     uniform on the unit sphere, taken from `variants` template point sets per
     N (0: about n_hulls / (4 x distinct N), between 1 and 16) whose hull
@@ -135,6 +146,8 @@ def template_hulls(rng: np.random.Generator, n_hulls: int, nmin: int = 8, nmax: 
         for v in range(variants):
             u = rng.normal(size=(int(c), 3))
             u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+            if vertex_order == "morton":
+                u = u[morton_order(u)]
             from scipy.spatial import ConvexHull
 
             tri = ConvexHull(u.astype(np.float64)).simplices.astype(np.int32)
@@ -161,7 +174,8 @@ def template_hulls(rng: np.random.Generator, n_hulls: int, nmin: int = 8, nmax: 
     return verts.astype(np.float32), off, counts, axes, np.concatenate(t_tris), tri_off, tri_cnt
 
 
-def hull_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int = 512, faces: bool = False):
+def hull_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int = 512, faces: bool = False,
+               vertex_order: str = "random"):
     """Batched convex-hull workload: 2 hulls per pair (shape 2i, 2i+1),
     centres U[0, 2]^3 m, uniform rotations.  faces=True: hulls from
     template_hulls, with triangles (keys tris, tri_off, tri_cnt)."""
@@ -169,7 +183,7 @@ def hull_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int =
     n = 2 * n_pairs
     extra = {}
     if faces:
-        verts, off, cnt, axes, tris, toff, tcnt = template_hulls(rng, n, nmin, nmax)
+        verts, off, cnt, axes, tris, toff, tcnt = template_hulls(rng, n, nmin, nmax, vertex_order=vertex_order)
         extra = dict(tris=tris, tri_off=toff, tri_cnt=tcnt)
     else:
         verts, off, cnt, axes = ellipsoid_hulls(rng, n, nmin, nmax)
