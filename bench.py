@@ -58,6 +58,20 @@ def cpu_baseline(h, n_sample, threads, target_cpu_s=10.0):
                                   f"({reps * n_sample} pair queries, {total:.2f}s wall on {threads} threads)")
 
 
+def measured_traffic(tuned):
+    """HBM bytes per launch of the lane GJK kernel on this workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (each
+    scaled by the same counter's reading on a 1 GiB copy); None when the
+    group kernel was chosen (not profiled)."""
+    if tuned.get("gjk_group", 1) != 1:
+        return None
+    path = os.path.join(ROOT, "profiles", "lane_kernel_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,7 +199,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(tuned),
             "algorithmic_bytes_per_launch": alg_bytes,
             "kernel_ms": {"order": float(kern[:, 0].mean()), "gjk": float(kern[:, 1].mean()),
                           "epa": float(kern[:, 2].mean())},
