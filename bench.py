@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start measurement")
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-mesh measurement")
+    ap.add_argument("--no-arm", action="store_true", help="skip the robot-arm configuration measurement")
+    ap.add_argument("--arm-configs", type=int, default=65536)
+    ap.add_argument("--arm-cpu-configs", type=int, default=4096)
     ap.add_argument("--mesh-pool", type=int, default=256)
     ap.add_argument("--mesh-cpu-pairs", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -221,6 +224,8 @@ def main():
     del h
     if not args.no_mesh:
         result["mesh"] = bench_mesh(ctx, args, threads=args.cpu_threads or min(16, os.cpu_count() or 1))
+    if not args.no_arm:
+        result["arm"] = bench_arm(ctx, args, threads=args.cpu_threads or min(16, os.cpu_count() or 1))
     ctx.close()
     print(json.dumps(result), flush=True)
 
@@ -275,6 +280,72 @@ def bench_warm(ctx, args, h, dpa, dpb, dqa, dqb, dout, n, out_plain, value_plain
            "trained_on": {k: meta[k] for k in ("pairs", "seed", "epochs") if k in meta}}
     log(f"[warm] {json.dumps(res)}")
     dguess.free()
+    return res
+
+
+def bench_arm(ctx, args, threads, margin=0.05, cap=64):
+    """Robot-arm configurations: 65,536 joint vectors of a 7-link chain of
+    convex links against 512 oriented boxes -- kinematics, AABB-tree
+    broadphase, signed distance of every candidate pair within the margin,
+    per-configuration collision flag and minimum distance.  The heightmap
+    of the full configuration is not part of this build."""
+    import mfcl
+    from mfcl import workloads as W
+
+    scene = W.arm_scene(args.arm_configs, seed=6)
+    L, nb = scene["n_links"], scene["n_boxes"]
+    ctx.upload_shapes(scene["types"], scene["params"], scene["vert_off"], scene["vert_cnt"], scene["verts"],
+                      faces=mfcl.faces_from_triangles(scene["tris"], scene["tri_off"], scene["tri_cnt"]))
+    mn, mx = W.box_aabbs(scene["box_pose"], scene["box_half"])
+    ctx.aabbtree_build(mn, mx, np.arange(nb))
+    n = len(scene["q"])
+    arm = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb, broadphase_cap=cap,
+                        max_pairs=16 * n * L)
+    dq, dbox = ctx.to_device(scene["q"]), ctx.to_device(scene["box_pose"])
+    dany, dmin = ctx.alloc(4 * n), ctx.alloc(4 * n)
+    for _ in range(max(1, args.warmup)):
+        npairs = ctx.arm_query_dev(arm, dq, n, margin, dbox, dany, dmin)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        npairs = ctx.arm_query_dev(arm, dq, n, margin, dbox, dany, dmin)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    any_hit, min_d = np.zeros(n, np.int32), np.zeros(n, np.float32)
+    dany.download(any_hit)
+    dmin.download(min_d)
+    value = n * args.steps / el
+    res = {"value": value, "unit": "configurations/s", "ms_per_step": 1e3 * el / args.steps,
+           "pairs_per_step": npairs, "colliding_fraction": float(np.mean(any_hit & 1)),
+           "failed_fraction": float(np.mean((any_hit & 2) > 0)),
+           "workload": f"{n} configurations of a {L}-link arm (hull links of 64..128 vertices with faces) vs {nb} "
+                       "oriented boxes in a 2 m cube 0.5 m in front of the base; candidates within 0.05 m; "
+                       "no heightmap"}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import arm_ref
+        import pyoracle
+
+        m = min(args.arm_cpu_configs, n)
+        sub = dict(scene)
+        S = arm_ref.shape_set(scene)
+        tree = pyoracle.AABBTree(mn, mx, np.arange(nb))
+        arm_ref.pipeline(sub, scene["q"][:m], margin, cap, S, tree, threads=threads)  # warm-up
+        reps, total = 0, 0.0
+        while total < 5.0 and reps < 50:
+            t0 = time.perf_counter()
+            flags, mind, _ = arm_ref.pipeline(sub, scene["q"][:m], margin, cap, S, tree, threads=threads)
+            total += time.perf_counter() - t0
+            reps += 1
+        rate = reps * m / total
+        band = np.abs(mind) <= 1e-5
+        res["cpu_baseline"] = {"value": rate, "unit": "configurations/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} configurations, {reps} timed passes ({total:.2f}s wall)",
+                               "sample_hit_mismatches_vs_gpu": int(np.sum(((flags & 1) != (any_hit[:m] & 1)) & ~band))}
+        res["speedup_vs_cpu"] = value / rate
+    log(f"[arm] {json.dumps(res)}")
+    for d in (dq, dbox, dany, dmin):
+        d.free()
     return res
 
 

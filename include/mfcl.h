@@ -214,6 +214,38 @@ int mfcl_aabbtree_query(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uin
 int mfcl_aabbtree_query_dev(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uint64_t nq,
                             int32_t cap, int32_t* counts, int64_t* ids, void* stream);
 
+/* ---- robot-arm configurations ------------------------------------------- */
+/* No counterpart in the reference (it has no arm model): forward kinematics
+ * of a serial chain, then per link BinaryAABB_Tree::SingleObjectCollision
+ * against the tree built by mfcl_aabbtree_build (binary_AABB_tree-inl.h:
+ * 400-440) and per candidate pair GJKSolver::shapeSignedDistance
+ * (gjk_solver-inl.h:820-874), reduced per configuration.  Joint i turns
+ * about z (even i) or y (odd i); link i runs along its local x over
+ * link_len[i], its shape centred at link_len[i] / 2; the base is the world
+ * origin.  Obstacle k is shape box_shape0 + k with world pose box_pose7[k]
+ * and leaf id k in the AABB tree. */
+#define MFCL_ARM_MAX_LINKS 16
+typedef struct mfcl_arm {
+  int32_t n_links;
+  uint32_t link_shape[MFCL_ARM_MAX_LINKS];  /* shape id of each link */
+  float link_len[MFCL_ARM_MAX_LINKS];
+  float link_radius[MFCL_ARM_MAX_LINKS];    /* bound on |x| over the link shape (broadphase sphere) */
+  uint32_t box_shape0, n_boxes;
+  int32_t broadphase_cap;                   /* candidates kept per link, 1..512 */
+  int32_t gjk_max_iterations;               /* 0: the default */
+  uint64_t max_pairs;                       /* candidate pair capacity, 0: 8 per link */
+} mfcl_arm;
+/* q: n_configs x n_links joint angles (device).  Per configuration:
+ * any_hit bit 0 = some candidate pair has signed distance < 0, bit 1 = some
+ * candidate was dropped (broadphase cap, pair capacity) or its query failed;
+ * min_dist = the minimum signed distance over the candidates (+inf when no
+ * obstacle is within `margin` of a link's bounding sphere).  n_pairs: the
+ * candidate pairs queried (host, may be NULL).  Synchronises the stream once
+ * (the pair count sizes the narrowphase launch). */
+int mfcl_arm_query_dev(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_configs, float margin,
+                       const float* box_pose7, int32_t* any_hit, float* min_dist, uint64_t* n_pairs,
+                       void* stream);
+
 /* ---- learned GJK warm start --------------------------------------------- */
 /* No counterpart in the reference: a 39-128-128-128-4 GELU MLP whose output
  * direction is passed as the per-pair GJK guess (the reference's hook

@@ -74,6 +74,7 @@ void freeAll(mfcl_ctx* c) {
   freePtr(c->ws_io.p);
   freePtr(c->ws_oflow.p);
   freePtr(c->ws_lane.p);
+  freePtr(c->ws_arm.p);
   mlpFree(c);
 }
 
@@ -628,6 +629,19 @@ int mfcl_mlp_train_step_dev(mfcl_ctx* ctx, const float* feat64, const float* tar
   if (rc) return rc;
   if (loss_host) MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
   return MFCL_OK;
+}
+
+int mfcl_arm_query_dev(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_configs, float margin,
+                       const float* box_pose7, int32_t* any_hit, float* min_dist, uint64_t* n_pairs, void* stream) {
+  if (!ctx || !arm) return MFCL_E_INVALID;
+  if (n_configs && (!q || !box_pose7 || !any_hit || !min_dist)) {
+    ctx->err = "null device pointer";
+    return MFCL_E_INVALID;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  if (n_pairs) *n_pairs = 0;
+  return armQueryLaunch(ctx, arm, q, n_configs, margin, box_pose7, any_hit, min_dist, n_pairs, s);
 }
 
 }  // extern "C"

@@ -106,7 +106,7 @@ struct mfcl_ctx {
   int64_t* d_bp_ids{nullptr};
 
   // growable device workspaces, keyed by purpose
-  mfcl::DevBuf ws_order, ws_queue, ws_arena, ws_misc, ws_io, ws_oflow, ws_lane;
+  mfcl::DevBuf ws_order, ws_queue, ws_arena, ws_misc, ws_io, ws_oflow, ws_lane, ws_arm;
 
   mfcl::MlpState mlp;
 
@@ -129,6 +129,9 @@ int uploadVec(mfcl_ctx* ctx, void** dptr, const void* src, size_t bytes);
 int uploadConvexFaces(mfcl_ctx* ctx, const int32_t* faces, const uint64_t* face_off, const uint32_t* num_faces,
                       uint64_t faces_len);
 int computeWalkSeeds(mfcl_ctx* ctx);
+// arm.hip
+int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_cfg, float margin,
+                   const float* box_pose7, int32_t* any_hit, float* min_dist, uint64_t* n_pairs_out, hipStream_t s);
 
 // Launchers (narrowphase.hip, bvh.hip, broadphase.hip).
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,

@@ -39,6 +39,36 @@ class Request(C.Structure):
     ]
 
 
+ARM_MAX_LINKS = 16
+
+
+class Arm(C.Structure):
+    """mfcl_arm (include/mfcl.h)."""
+    _fields_ = [
+        ("n_links", C.c_int32),
+        ("link_shape", C.c_uint32 * ARM_MAX_LINKS),
+        ("link_len", C.c_float * ARM_MAX_LINKS),
+        ("link_radius", C.c_float * ARM_MAX_LINKS),
+        ("box_shape0", C.c_uint32),
+        ("n_boxes", C.c_uint32),
+        ("broadphase_cap", C.c_int32),
+        ("gjk_max_iterations", C.c_int32),
+        ("max_pairs", C.c_uint64),
+    ]
+
+
+def make_arm(link_shape, link_len, link_radius, box_shape0, n_boxes, broadphase_cap=32, max_pairs=0) -> Arm:
+    a = Arm()
+    a.n_links = len(link_shape)
+    for i in range(a.n_links):
+        a.link_shape[i] = int(link_shape[i])
+        a.link_len[i] = float(link_len[i])
+        a.link_radius[i] = float(link_radius[i])
+    a.box_shape0, a.n_boxes = int(box_shape0), int(n_boxes)
+    a.broadphase_cap, a.gjk_max_iterations, a.max_pairs = int(broadphase_cap), 0, int(max_pairs)
+    return a
+
+
 CONVEX_RESULT_DTYPE = np.dtype(
     [
         ("hit", np.int32),
@@ -94,6 +124,7 @@ def load_library() -> C.CDLL:
     lib.mfcl_d2h.argtypes = [p, p, p, u64]
     lib.mfcl_upload_shapes.argtypes = [p, p, p, p, p, u32, p, u64]
     lib.mfcl_upload_convex_faces.argtypes = [p, p, u64, p, p]
+    lib.mfcl_arm_query_dev.argtypes = [p, C.POINTER(Arm), p, u64, C.c_float, p, p, p, C.POINTER(u64), p]
     lib.mfcl_convex_query.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p]
     lib.mfcl_convex_query_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
@@ -325,6 +356,14 @@ class Context:
         self._check(self.lib.mfcl_aabbtree_query(self.h, _ptr(qmn), _ptr(qmx), n, cap, _ptr(counts), _ptr(ids)))
         return counts, ids
 
+
+    # ---- robot-arm configurations
+    def arm_query_dev(self, arm: Arm, q: DeviceArray, n_configs: int, margin: float, box_pose7: DeviceArray,
+                      any_hit: DeviceArray, min_dist: DeviceArray) -> int:
+        n_pairs = C.c_uint64(0)
+        self._check(self.lib.mfcl_arm_query_dev(self.h, C.byref(arm), q.ptr, n_configs, margin, box_pose7.ptr,
+                                                any_hit.ptr, min_dist.ptr, C.byref(n_pairs), None))
+        return int(n_pairs.value)
 
     # ---- learned warm start
     def mlp_num_params(self) -> int:

@@ -290,3 +290,58 @@ def mesh_pairs(pool: dict, n_pairs: int = 4096, seed: int = 30, spread: float = 
     pb[:, 0:3] = pa[:, 0:3] + u * d[:, None]
     pb[:, 3:7] = random_quats(rng, n_pairs)
     return dict(pair_a=a, pair_b=b, pose_a=pa, pose_b=pb)
+
+
+def arm_scene(n_configs: int = 65536, seed: int = 6, n_links: int = 7, n_boxes: int = 512):
+    """Robot-arm configuration workload: a serial chain of n_links convex
+    links (64..128 hull vertices each, with faces; link length U[0.1, 0.4] m,
+    hull radius U[0.03, 0.08] m about the link's long axis), n_boxes
+    obstacle boxes (half-extents U[0.02, 0.3] m, centres uniform in a 2 m
+    cube whose near face is 0.5 m in front of the arm's base, i.e.
+    [0.5, 2.5] x [-1, 1] x [-1, 1] m, uniform rotations) and n_configs joint
+    vectors U[-pi, pi].  With the cube around the base every configuration
+    would collide; in front of it about a third do.  Shapes: the links first
+    (ids 0..n_links-1), then the boxes."""
+    rng = np.random.default_rng(seed)
+    lens = rng.uniform(0.1, 0.4, size=n_links).astype(np.float32)
+    rad = rng.uniform(0.03, 0.08, size=n_links).astype(np.float32)
+    verts, off, cnt, taxes, tris, toff, tcnt = template_hulls(rng, n_links, 64, 128, variants=1)
+    # template_hulls scales unit-sphere template points by random axes (a
+    # linear map, so the faces stay those of the hull); map them back and on
+    # to the link's own axes instead
+    axes = np.stack([lens / 2, rad, rad], 1).astype(np.float64)
+    owner = np.repeat(np.arange(n_links), cnt.astype(np.intp))
+    verts = (verts.astype(np.float64) / taxes[owner].astype(np.float64) * axes[owner]).astype(np.float32)
+    link_radius = np.array([np.nextafter(np.float32(np.sqrt((verts[owner == i].astype(np.float64) ** 2).sum(1).max())),
+                                         np.float32(np.inf)) for i in range(n_links)], np.float32)
+    half = rng.uniform(0.02, 0.3, size=(n_boxes, 3))
+    centres = rng.uniform(-1.0, 1.0, size=(n_boxes, 3)) + np.array([1.5, 0.0, 0.0])
+    quats = random_quats(rng, n_boxes)
+    box_pose = np.concatenate([centres, quats], 1).astype(np.float32)
+    types = np.concatenate([np.full(n_links, CONVEX), np.full(n_boxes, BOX)]).astype(np.int32)
+    params = np.zeros((n_links + n_boxes, 4), np.float32)
+    params[n_links:, :3] = 2.0 * half
+    vert_off = np.concatenate([off, np.zeros(n_boxes, np.uint64)])
+    vert_cnt = np.concatenate([cnt, np.zeros(n_boxes, np.uint32)])
+    tri_off = np.concatenate([toff, np.zeros(n_boxes, np.int64)])
+    tri_cnt = np.concatenate([tcnt, np.zeros(n_boxes, np.int64)])
+    q = rng.uniform(-np.pi, np.pi, size=(n_configs, n_links)).astype(np.float32)
+    return dict(types=types, params=params, verts=verts, vert_off=vert_off, vert_cnt=vert_cnt, tris=tris,
+                tri_off=tri_off, tri_cnt=tri_cnt, link_len=lens, link_radius=link_radius, box_pose=box_pose,
+                box_half=half.astype(np.float32), q=q, n_links=n_links, n_boxes=n_boxes)
+
+
+def box_aabbs(box_pose: np.ndarray, half: np.ndarray):
+    """World AABBs of oriented boxes (FP32 corners of |R| h around the centre,
+    rounded outward by one ulp)."""
+    q = box_pose[:, 3:7].astype(np.float64)
+    q = q / np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                  2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                  2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1).reshape(-1, 3, 3)
+    ext = np.abs(R) @ half.astype(np.float64)[:, :, None]
+    c = box_pose[:, :3].astype(np.float64)
+    mn = np.nextafter((c - ext[:, :, 0]).astype(np.float32), np.float32(-np.inf))
+    mx = np.nextafter((c + ext[:, :, 0]).astype(np.float32), np.float32(np.inf))
+    return mn, mx
