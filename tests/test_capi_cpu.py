@@ -48,6 +48,10 @@ def _c_layout(tmp_path):
           printf("sizeof.request %zu\\n", sizeof(mfcl_request));
           printf("sizeof.convex_result %zu\\n", sizeof(mfcl_convex_result));
           printf("sizeof.mesh_result %zu\\n", sizeof(mfcl_mesh_result));
+          printf("sizeof.arm %zu\\n", sizeof(mfcl_arm));
+          OFF(mfcl_arm, n_links); OFF(mfcl_arm, link_shape); OFF(mfcl_arm, link_len); OFF(mfcl_arm, link_radius);
+          OFF(mfcl_arm, box_shape0); OFF(mfcl_arm, n_boxes); OFF(mfcl_arm, broadphase_cap);
+          OFF(mfcl_arm, gjk_max_iterations); OFF(mfcl_arm, max_pairs);
           OFF(mfcl_request, mode); OFF(mfcl_request, max_contacts); OFF(mfcl_request, penetration);
           OFF(mfcl_request, gjk_max_iterations); OFF(mfcl_request, gjk_tolerance);
           OFF(mfcl_request, epa_max_faces); OFF(mfcl_request, epa_max_iterations);
@@ -81,6 +85,9 @@ def test_struct_layout_matches_binding(tmp_path):
         key = f"mfcl_convex_result.{name}"
         if key in lay:
             assert lay[key] == mfcl.CONVEX_RESULT_DTYPE.fields[name][1], name
+    assert lay["sizeof.arm"] == C.sizeof(mfcl.Arm)
+    for name, _ in mfcl.Arm._fields_:
+        assert lay[f"mfcl_arm.{name}"] == getattr(mfcl.Arm, name).offset, name
     assert lay["sizeof.mesh_result"] == mfcl.MESH_RESULT_DTYPE.itemsize
     for name in mfcl.MESH_RESULT_DTYPE.names:
         key = f"mfcl_mesh_result.{name}"
