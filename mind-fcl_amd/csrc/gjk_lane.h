@@ -280,8 +280,8 @@ MFCL_D void laneSupport(bool need, const MinkLane& md, d3 d, int id, MV& out, ui
       const DevShape s0 = loadShape(md.T, md.ia), s1 = loadShape(md.T, md.ib);
       // one walk after the other: the lanes' independent walks already keep
       // enough memory requests in flight, and lock-step pairs cost registers
-      if (w0) idx0 = hullWalk<false>(s0, d, &walked);
-      if (w1) idx1 = hullWalk<false>(s1, d1, &walked);
+      if (w0) idx0 = hullWalk(s0, d, &walked);
+      if (w1) idx1 = hullWalk(s1, d1, &walked);
     }
   }
   if (need) {

@@ -44,6 +44,20 @@ MFCL_D float3 hullVert(const DevShape& s, int i) {
   return make_float3(q.x, q.y, q.z);
 }
 
+// Vertex i of a walking hull and the offset of its neighbour record: the
+// record offset rides in the vertex's fourth (otherwise unused) word in
+// device memory, written when the faces are uploaded, so one 16-byte load
+// serves both; the LDS copy has no fourth row and reads the table instead.
+MFCL_D float3 hullVertRec(const DevShape& s, int i, int& rec) {
+  if (s.lv) {
+    rec = s.nbr[i];
+    return make_float3(s.lv[i], s.lv[i + s.ls], s.lv[i + 2 * s.ls]);
+  }
+  const float4 q = s.verts[i];
+  rec = __float_as_int(q.w);
+  return make_float3(q.x, q.y, q.z);
+}
+
 // Copies a hull's vertices into LDS (SoA rows of stride ls), lanes lg, lg+G, ...
 template <int G>
 MFCL_D void stageHull(DevShape& s, MFCL_LDS float* dst, int ls, int lg) {
@@ -195,9 +209,9 @@ __device__ __attribute__((noinline)) int hullArgmaxOOL(const float4* verts, cons
 // Latency: a step reads the current vertex's neighbour record (its count and
 // up to kWalkChunk ids in one batch; the tables are followed by 32 ints of
 // padding, so the fixed-size read stays in bounds), then the neighbours'
-// vertices and their own record offsets in a second batch, and only then
-// runs the reference's sequential comparisons in registers: two dependent
-// memory round trips per step.  The two walks of one support query (one per
+// vertices -- each 16-byte vertex also carrying its own record offset -- in a
+// second batch, and only then runs the reference's sequential comparisons in
+// registers: two dependent memory round trips per step.  The two walks of one support query (one per
 // shape) advance in lock step so their round trips overlap.
 constexpr int kWalkChunk = 8;
 
@@ -235,9 +249,8 @@ MFCL_D void walkInit(Walk& W, const DevShape& s, d3 d, bool active) {
     }
   }
   W.ext = init;
-  const float3 q = hullVert(s, init);
+  const float3 q = hullVertRec(s, init, W.start);
   W.ext_v = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
-  W.start = s.nbr[init];
   W.parent0 = W.parent1 = init;
   W.visited = 1;
 }
@@ -248,22 +261,18 @@ MFCL_D void walkFetchRecord(Walk& W, const DevShape& s) {
 #pragma unroll
   for (int j = 0; j < kWalkChunk; ++j) W.id[j] = s.nbr[W.start + 1 + j];
 }
-// second batch: the neighbours' vertices, and with kPrefetch their record
-// offsets (one round trip less per step for eight more reads)
-template <bool kPrefetch>
+// second batch: the neighbours' vertices with their record offsets
 MFCL_D void walkFetchNeighbours(Walk& W, const DevShape& s) {
   if (!W.keep) return;
 #pragma unroll
   for (int j = 0; j < kWalkChunk; ++j) {
     const int v = j < W.count ? W.id[j] : W.ext;  // in range for the loads
     W.id[j] = v;
-    W.w[j] = hullVert(s, v);
-    if (kPrefetch) W.st[j] = s.nbr[v];
+    W.w[j] = hullVertRec(s, v, W.st[j]);
   }
 }
 // the reference's comparisons over the record (neighbours beyond the first
 // kWalkChunk, rare on hulls, are read here)
-template <bool kPrefetch>
 MFCL_D void walkCompare(Walk& W, const DevShape& s) {
   if (!W.keep) return;
   W.keep = false;
@@ -276,8 +285,7 @@ MFCL_D void walkCompare(Walk& W, const DevShape& s) {
         const int nb = s.nbr[W.start + 1 + c0 + j];
         const int v = c0 + j < W.count ? nb : old;
         W.id[j] = v;
-        W.w[j] = hullVert(s, v);
-        if (kPrefetch) W.st[j] = s.nbr[v];
+        W.w[j] = hullVertRec(s, v, W.st[j]);
       }
     }
 #pragma unroll
@@ -291,26 +299,21 @@ MFCL_D void walkCompare(Walk& W, const DevShape& s) {
         W.keep = true;
         W.ext = W.id[j];
         W.ext_v = v;
-        if (kPrefetch) next_start = W.st[j];
+        next_start = W.st[j];
       }
     }
   }
   W.parent0 = old;
-  if (kPrefetch) W.start = next_start;
-  else if (W.keep) W.start = s.nbr[W.ext];
+  W.start = next_start;
 }
 
-// kPrefetch: where few walks run at once (one pair per group or wavefront)
-// latency decides; the lane kernel's many concurrent walks are bound by
-// memory traffic instead and skip the prefetch.
-template <bool kPrefetch>
 MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
   Walk W;
   walkInit(W, s, d, true);
   while (W.keep) {
     walkFetchRecord(W, s);
-    walkFetchNeighbours<kPrefetch>(W, s);
-    walkCompare<kPrefetch>(W, s);
+    walkFetchNeighbours(W, s);
+    walkCompare(W, s);
   }
   *scanned += W.visited;
   return W.ext;
@@ -326,10 +329,10 @@ MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevS
   while (A.keep || B.keep) {
     walkFetchRecord(A, sa);
     walkFetchRecord(B, sb);
-    walkFetchNeighbours<true>(A, sa);
-    walkFetchNeighbours<true>(B, sb);
-    walkCompare<true>(A, sa);
-    walkCompare<true>(B, sb);
+    walkFetchNeighbours(A, sa);
+    walkFetchNeighbours(B, sb);
+    walkCompare(A, sa);
+    walkCompare(B, sb);
   }
   if (act_a) ia = A.ext;
   if (act_b) ib = B.ext;
@@ -338,7 +341,7 @@ MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevS
 
 template <int G>
 MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
-  if (s.nbr) return hullWalk<true>(s, d, scanned);
+  if (s.nbr) return hullWalk(s, d, scanned);
   const int r = hullArgmaxOOL<G>(s.verts, s.lv, s.ls, s.nv, s.rbound, d.x, d.y, d.z, lg);
   *scanned += (r >> 30) ? 2 * s.nv : s.nv;
   return r & ((1 << 30) - 1);

@@ -572,11 +572,15 @@ __global__ void zeroKernel(uint32_t* p, int n) {
 // Seed vertices of the support walk (convex-inl.h:151-199): per walking
 // hull, the first vertex attaining the maximum of +x, -x, +y, -y, +z, -z
 // (convex-inl.h:133-149; the dot product with a unit axis is the coordinate).
-__global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed) {
+// It also stores each vertex's neighbour-record offset in the vertex's
+// fourth word (read together with the vertex by the walk).
+__global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed, float4* verts_rw) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || T.nbr_off[i] < 0) return;
   const float4* v = T.verts + T.vert_off[i];
   const int nv = static_cast<int>(T.vert_cnt[i]);
+  const int* tab = T.nbr + T.nbr_off[i];
+  for (int k = 0; k < nv; ++k) verts_rw[T.vert_off[i] + k].w = __int_as_float(tab[k]);
   int idx[6] = {0, 0, 0, 0, 0, 0};
   double best[6];
   {
@@ -605,7 +609,7 @@ int computeWalkSeeds(mfcl_ctx* ctx) {
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
                 ctx->d_seed};
   const uint32_t n = ctx->n_shapes;
-  walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed);
+  walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed, ctx->d_verts);
   MFCL_HIP_TRY(ctx, hipGetLastError());
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return MFCL_OK;
