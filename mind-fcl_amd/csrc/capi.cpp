@@ -49,7 +49,9 @@ void freeShapes(mfcl_ctx* c) {
   freePtr(reinterpret_cast<void*&>(c->d_nbr));
   freePtr(reinterpret_cast<void*&>(c->d_nbr_off));
   freePtr(reinterpret_cast<void*&>(c->d_seed));
+  freePtr(reinterpret_cast<void*&>(c->d_wrec));
   c->n_shapes = 0;
+  c->n_verts_alloc = 0;
 }
 
 void freeMeshes(mfcl_ctx* c) {
@@ -359,6 +361,7 @@ int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
   if ((rc = upload(ctx, &ctx->d_desc, desc.data(), desc.size()))) return rc;
   ctx->h_vcnt = vcnt;
   ctx->h_voff = voff;
+  ctx->n_verts_alloc = verts.size();
   ctx->h_type.assign(type, type + n_shapes);
   ctx->n_shapes = n_shapes;
   return MFCL_OK;
@@ -378,6 +381,7 @@ int mfcl_upload_convex_faces(mfcl_ctx* ctx, const int32_t* faces, uint64_t faces
     freePtr(reinterpret_cast<void*&>(ctx->d_nbr));
     freePtr(reinterpret_cast<void*&>(ctx->d_nbr_off));
     freePtr(reinterpret_cast<void*&>(ctx->d_seed));
+    freePtr(reinterpret_cast<void*&>(ctx->d_wrec));
   }
   return rc;
 }

@@ -573,14 +573,28 @@ __global__ void zeroKernel(uint32_t* p, int n) {
 // hull, the first vertex attaining the maximum of +x, -x, +y, -y, +z, -z
 // (convex-inl.h:133-149; the dot product with a unit axis is the coordinate).
 // It also stores each vertex's neighbour-record offset in the vertex's
-// fourth word (read together with the vertex by the walk).
-__global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed, float4* verts_rw) {
+// fourth word (read together with the vertex by the walk) and, for hulls of
+// at most 65535 vertices, the vertex's walk record: (x, y, z, neighbour
+// count) and the first eight neighbour ids packed as 16-bit pairs.
+__global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed, float4* verts_rw, float4* wrec) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || T.nbr_off[i] < 0) return;
   const float4* v = T.verts + T.vert_off[i];
   const int nv = static_cast<int>(T.vert_cnt[i]);
   const int* tab = T.nbr + T.nbr_off[i];
-  for (int k = 0; k < nv; ++k) verts_rw[T.vert_off[i] + k].w = __int_as_float(tab[k]);
+  for (int k = 0; k < nv; ++k) {
+    const uint64_t g = T.vert_off[i] + static_cast<uint64_t>(k);
+    verts_rw[g].w = __int_as_float(tab[k]);
+    if (nv > 65535) continue;
+    const int* rec = tab + tab[k];
+    const int cnt = rec[0];
+    uint32_t u[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 8 && j < cnt; ++j) u[j >> 1] |= static_cast<uint32_t>(rec[1 + j]) << (16 * (j & 1));
+    const float4 q = v[k];
+    wrec[2 * g] = make_float4(q.x, q.y, q.z, __int_as_float(cnt));
+    wrec[2 * g + 1] = make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]), __uint_as_float(u[2]),
+                                  __uint_as_float(u[3]));
+  }
   int idx[6] = {0, 0, 0, 0, 0, 0};
   double best[6];
   {
@@ -607,9 +621,9 @@ __global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed, float4* ver
 
 int computeWalkSeeds(mfcl_ctx* ctx) {
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
-                ctx->d_seed};
+                ctx->d_seed, ctx->d_wrec};
   const uint32_t n = ctx->n_shapes;
-  walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed, ctx->d_verts);
+  walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed, ctx->d_verts, ctx->d_wrec);
   MFCL_HIP_TRY(ctx, hipGetLastError());
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return MFCL_OK;
@@ -673,7 +687,7 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   QEntry* queue = static_cast<QEntry*>(ctx->ws_queue.p);
 
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
-                ctx->d_seed};
+                ctx->d_seed, ctx->d_wrec};
   PairIO io{pa, pb, posea, poseb, guess, out};
   const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
