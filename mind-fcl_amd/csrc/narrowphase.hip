@@ -697,16 +697,17 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   bucketScatterKernel<<<nb, kBlock, 0, s>>>(key, n, offs, order);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
   {
-    const int G = ctx->tune.gjk_group;
-    const unsigned gb = static_cast<unsigned>((n * G + kBlock - 1) / kBlock);
     const int mode = P.mode == MFCL_MODE_SIGNED_DISTANCE ? kModeSd : (P.penetration ? kModePen : kModeBool);
+    // the lane kernel has no boolean mode: boolean queries take 16-lane groups
+    const int G = ctx->tune.gjk_group == 1 && mode == kModeBool ? 16 : ctx->tune.gjk_group;
+    const unsigned gb = static_cast<unsigned>((n * G + kBlock - 1) / kBlock);
 #define MFCL_NARROW(GG, MM) \
   narrowKernel<GG, MM><<<gb, kBlock, 0, s>>>(T, ctx->d_interior, io, order, n, P, queue, qcount, sc)
 #define MFCL_NARROW_MODES(GG)                 \
   if (mode == kModeSd) MFCL_NARROW(GG, kModeSd); \
   else if (mode == kModePen) MFCL_NARROW(GG, kModePen); \
   else MFCL_NARROW(GG, kModeBool)
-    if (G == 1 && mode != kModeBool) {
+    if (G == 1) {
       const unsigned lb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
       if (int rc = ensure(ctx, ctx->ws_lane, static_cast<size_t>(lb) * (kBlock / 64) * kLaneDirDoubles * sizeof(double)))
         return rc;

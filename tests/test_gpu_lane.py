@@ -60,7 +60,7 @@ def _run(ctx, w, req, group, guess):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode,pen,guess", [("sd", 0, False), ("sd", 0, True), ("collide", 1, False),
-                                            ("collide", 1, True)])
+                                            ("collide", 1, True), ("collide", 0, False)])
 def test_lane_kernel_equals_group_kernel(gpu_ctx, mode, pen, guess):
     w = _mixed(21)
     gpu_ctx.upload_shapes(w["types"], w["params"], w["vert_off"], w["vert_cnt"], w["verts"])
