@@ -285,22 +285,25 @@ def bench_warm(ctx, args, h, dpa, dpb, dqa, dqb, dout, n, out_plain, value_plain
 
 def bench_arm(ctx, args, threads, margin=0.05, cap=64):
     """Robot-arm configurations: 65,536 joint vectors of a 7-link chain of
-    convex links against 512 oriented boxes -- kinematics, AABB-tree
-    broadphase, signed distance of every candidate pair within the margin,
-    per-configuration collision flag and minimum distance.  The heightmap
-    of the full configuration is not part of this build."""
+    convex links against 512 oriented boxes and a 256 x 256 fBm heightmap
+    (1 cm pixels) -- kinematics, AABB-tree broadphase, signed distance of
+    every box candidate pair within the margin, heightmap candidate pixels
+    per link with a boolean narrowphase, per-configuration collision flag
+    and minimum distance (to the boxes)."""
     import mfcl
     from mfcl import workloads as W
 
-    scene = W.arm_scene(args.arm_configs, seed=6)
+    scene = W.arm_scene(args.arm_configs, seed=6, heightmap=True)
     L, nb = scene["n_links"], scene["n_boxes"]
     ctx.upload_shapes(scene["types"], scene["params"], scene["vert_off"], scene["vert_cnt"], scene["verts"],
                       faces=mfcl.faces_from_triangles(scene["tris"], scene["tri_off"], scene["tri_cnt"]))
     mn, mx = W.box_aabbs(scene["box_pose"], scene["box_half"])
     ctx.aabbtree_build(mn, mx, np.arange(nb))
+    ctx.upload_heightmap(scene["hm_heights"], scene["hm_res"], scene["hm_res"])
     n = len(scene["q"])
+    hm = dict(box_shape0=scene["hm_box_shape0"], pose7=scene["hm_pose7"], cap=4096)
     arm = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb, broadphase_cap=cap,
-                        max_pairs=16 * n * L)
+                        max_pairs=16 * n * L, heightmap=hm)
     dq, dbox = ctx.to_device(scene["q"]), ctx.to_device(scene["box_pose"])
     dany, dmin = ctx.alloc(4 * n), ctx.alloc(4 * n)
     for _ in range(max(1, args.warmup)):
@@ -316,11 +319,12 @@ def bench_arm(ctx, args, threads, margin=0.05, cap=64):
     dmin.download(min_d)
     value = n * args.steps / el
     res = {"value": value, "unit": "configurations/s", "ms_per_step": 1e3 * el / args.steps,
-           "pairs_per_step": npairs, "colliding_fraction": float(np.mean(any_hit & 1)),
+           "pairs_per_step": npairs, "heightmap_pairs_per_step": ctx.arm_last_heightmap_pairs(),
+           "colliding_fraction": float(np.mean(any_hit & 1)),
            "failed_fraction": float(np.mean((any_hit & 2) > 0)),
            "workload": f"{n} configurations of a {L}-link arm (hull links of 64..128 vertices with faces) vs {nb} "
-                       "oriented boxes in a 2 m cube 0.5 m in front of the base; candidates within 0.05 m; "
-                       "no heightmap"}
+                       "oriented boxes in a 2 m cube 0.5 m in front of the base (candidates within 0.05 m) and a "
+                       "256 x 256 fBm heightmap of 1 cm pixels (4 octaves, 0.1 m) 0.45 m below the base"}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import arm_ref
@@ -330,11 +334,17 @@ def bench_arm(ctx, args, threads, margin=0.05, cap=64):
         sub = dict(scene)
         S = arm_ref.shape_set(scene)
         tree = pyoracle.AABBTree(mn, mx, np.arange(nb))
-        arm_ref.pipeline(sub, scene["q"][:m], margin, cap, S, tree, threads=threads)  # warm-up
+        def cpu_pass():
+            pose = arm_ref.fk(scene["q"][:m], scene["link_len"])
+            fl, md, _ = arm_ref.pipeline(sub, scene["q"][:m], margin, cap, S, tree, threads=threads, pose=pose)
+            hit, _, _ = arm_ref.hm_collide(sub, S, pose, threads=threads)
+            return fl | hit.reshape(m, L).any(1).astype(np.int32), md
+
+        cpu_pass()  # warm-up
         reps, total = 0, 0.0
         while total < 5.0 and reps < 50:
             t0 = time.perf_counter()
-            flags, mind, _ = arm_ref.pipeline(sub, scene["q"][:m], margin, cap, S, tree, threads=threads)
+            flags, mind = cpu_pass()
             total += time.perf_counter() - t0
             reps += 1
         rate = reps * m / total

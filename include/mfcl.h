@@ -234,10 +234,18 @@ typedef struct mfcl_arm {
   int32_t broadphase_cap;                   /* candidates kept per link, 1..512 */
   int32_t gjk_max_iterations;               /* 0: the default */
   uint64_t max_pairs;                       /* candidate pair capacity, 0: 8 per link */
+  /* heightmap (mfcl_upload_heightmap), 0: none.  Pixel (x, y) of the bottom
+   * layer is shape hm_box_shape0 + y * full_x + x, a box of sides
+   * (res_x, res_y, height); the map's frame has world pose hm_pose7. */
+  int32_t use_heightmap;
+  uint32_t hm_box_shape0;
+  int32_t hm_cap;                           /* candidate pixels kept per link, 1..65536 */
+  float hm_pose7[7];
 } mfcl_arm;
 /* q: n_configs x n_links joint angles (device).  Per configuration:
- * any_hit bit 0 = some candidate pair has signed distance < 0, bit 1 = some
- * candidate was dropped (broadphase cap, pair capacity) or its query failed;
+ * any_hit bit 0 = some candidate pair has signed distance < 0 or some link
+ * collides with the heightmap, bit 1 = some candidate was dropped
+ * (broadphase cap, pair capacity) or its query failed;
  * min_dist = the minimum signed distance over the candidates (+inf when no
  * obstacle is within `margin` of a link's bounding sphere).  n_pairs: the
  * candidate pairs queried (host, may be NULL).  Synchronises the stream once
@@ -245,6 +253,24 @@ typedef struct mfcl_arm {
 int mfcl_arm_query_dev(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_configs, float margin,
                        const float* box_pose7, int32_t* any_hit, float* min_dist, uint64_t* n_pairs,
                        void* stream);
+
+/* Replaces heightmap::LayeredHeightMap (layered_heightmap.h:33-36, built
+ * from the bottom layer as layered_heightmap-inl.h:8-51 and :76-101: each
+ * upper pixel holds the maximum of its 2 x 2 children).  heights_mm: the
+ * bottom layer, (2 half_x) x (2 half_y) uint16 millimetres, pixel (x, y) at
+ * y * 2 half_x + x (flat_heightmap-inl.h pixelToFlatIndex); half_x and
+ * half_y powers of two.  A link of mfcl_arm_query_dev collides with the map
+ * when it collides with the box of some pixel of non-zero height
+ * (HeightMapCollisionSolver::HeightMapShapeIntersect, heightmap_solver_
+ * traverse-inl.h:17-120: candidate pixels are those whose box overlaps the
+ * link's AABB in the map frame, then ShapeIntersect(Box, Convex) per pixel,
+ * heightmap_solver_leaf-inl.h:6-28).  The reference has no distance to a
+ * heightmap, so min_dist covers the obstacle boxes only. */
+int mfcl_upload_heightmap(mfcl_ctx* ctx, const uint16_t* heights_mm, uint16_t half_x, uint16_t half_y, float res_x,
+                          float res_y);
+/* No counterpart in the reference: the heightmap candidate pairs (link, pixel
+ * box) that the last mfcl_arm_query_dev sent to the narrowphase. */
+int mfcl_arm_last_heightmap_pairs(mfcl_ctx* ctx, uint64_t* n_pairs);
 
 /* ---- learned GJK warm start --------------------------------------------- */
 /* No counterpart in the reference: a 39-128-128-128-4 GELU MLP whose output

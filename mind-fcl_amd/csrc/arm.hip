@@ -119,7 +119,7 @@ MFCL_D float keyFloat(int32_t k) { return __int_as_float(k >= 0 ? k : k ^ 0x7fff
 
 __global__ void armInitKernel(uint64_t n_cfg, int32_t* any_hit, int32_t* min_key, unsigned long long* n_pairs) {
   const uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c == 0) *n_pairs = 0;
+  if (c == 0) n_pairs[0] = n_pairs[1] = 0;
   if (c >= n_cfg) return;
   any_hit[c] = 0;
   min_key[c] = floatKey(INFINITY);
@@ -128,10 +128,10 @@ __global__ void armInitKernel(uint64_t n_cfg, int32_t* any_hit, int32_t* min_key
 // Per pair: collision when the signed distance is negative (EPA depth);
 // the configuration's minimum over its pairs with a valid result; a pair
 // whose query failed (ok != 1) marks its configuration with bit 1.
-__global__ void armReduceKernel(const mfcl_convex_result* res, const uint32_t* pcfg,
-                                const unsigned long long* n_pairs, int32_t* any_hit, int32_t* min_key) {
+__global__ void armReduceKernel(const mfcl_convex_result* res, const uint32_t* pcfg, uint64_t n_pairs,
+                                int32_t* any_hit, int32_t* min_key) {
   const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= *n_pairs) return;
+  if (p >= n_pairs) return;
   const mfcl_convex_result& r = res[p];
   const uint32_t c = pcfg[p];
   if (r.ok != 1) {
@@ -146,6 +146,177 @@ __global__ void armFinishKernel(uint64_t n_cfg, const int32_t* min_key, float* m
   const uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= n_cfg) return;
   min_dist[c] = keyFloat(min_key[c]);
+}
+
+// ---- heightmap -------------------------------------------------------------
+// The layers of the map (top 0 .. bottom n_layers - 1) and its pose.
+struct HmDev {
+  const uint16_t* h;
+  int n_layers;
+  uint64_t off[kMaxHmLayers];
+  int fx[kMaxHmLayers], fy[kMaxHmLayers];
+  float rx[kMaxHmLayers], ry[kMaxHmLayers];
+  double upper_m;
+  float pose7[7];
+};
+// The link hulls (vertex ranges in the shape table).
+struct ArmHulls {
+  uint64_t voff[MFCL_ARM_MAX_LINKS];
+  uint32_t vcnt[MFCL_ARM_MAX_LINKS];
+};
+
+// aabbToPixelROI's corner projection (flat_heightmap-inl.h:58-87): pixel
+// floor(p / res) + half clamped to [0, full - 1]; true when it was clamped.
+MFCL_D bool hmProject(double p, double res, int half, int full, int& out) {
+  const double v = floor(p / res) + half;
+  if (v <= 0) {
+    out = 0;
+    return v != 0;
+  }
+  if (v >= full - 1) {
+    out = full - 1;
+    return v != full - 1;
+  }
+  out = static_cast<int>(v);
+  return false;
+}
+
+// p strictly inside the box of its bottom-layer pixel by `slack` on every
+// side (pixel x spans [(x - half) res, (x - half + 1) res], z spans [0, h]).
+MFCL_D bool hmInside(const HmDev& H, d3 p, double slack) {
+  const int b = H.n_layers - 1;
+  const double rx = H.rx[b], ry = H.ry[b];
+  const int hx = H.fx[b] / 2, hy = H.fy[b] / 2;
+  const double vx = floor(p.x / rx) + hx, vy = floor(p.y / ry) + hy;
+  if (!(vx >= 0 && vx < H.fx[b] && vy >= 0 && vy < H.fy[b])) return false;
+  const int px = static_cast<int>(vx), py = static_cast<int>(vy);
+  const int hmm = H.h[H.off[b] + static_cast<uint64_t>(py) * H.fx[b] + px];
+  if (hmm == 0) return false;
+  const double top = static_cast<double>(hmm) * 0.001;
+  const double x0 = (px - hx) * rx, y0 = (py - hy) * ry;
+  return p.z > slack && p.z < top - slack && p.x > x0 + slack && p.x < x0 + rx - slack && p.y > y0 + slack &&
+         p.y < y0 + ry - slack;
+}
+
+// Heightmap candidates of one link (query k = configuration * n_links +
+// link), the flat algorithm of HeightMapShapeIntersect (heightmap_solver_
+// traverse-inl.h:17-98): the hull's AABB in the map frame from its
+// transformed vertices (computeBV<AABB, Convex>, utility-inl.h:160-173), no
+// candidate when it lies below the map's base or above its highest pixel,
+// else the pixels of the AABB's region (aabbToPixelROI) with non-zero height
+// at or above the AABB's bottom.  Tiles of 2^s x 2^s pixels whose upper-layer
+// maximum is below the AABB's bottom are skipped whole (the same set).
+//
+// Before any narrowphase work a link is decided colliding when its interior
+// point (the vertex mean, inside the hull) or, by 0.1 mm, one of its
+// vertices lies inside the box of the pixel under it: the point is then
+// common to the link and that box.  Links of configurations that already collide (with
+// an obstacle box) are not searched.  Candidates beyond `cap` mark the
+// configuration failed (bit 1).
+__global__ void __launch_bounds__(256) armHmKernel(ArmDev arm, ArmHulls hulls, HmDev H, const float4* verts,
+                                                   const double4* interior, const float* link_pose7, uint64_t nq,
+                                                   int cap, uint32_t hm_shape0, uint32_t* pa, uint32_t* pb,
+                                                   float* qa, float* qb, uint32_t* pcfg, uint64_t max_pairs,
+                                                   unsigned long long* n_pairs, int32_t* any_hit) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  const uint64_t cfg = k / arm.n_links;
+  const int link = static_cast<int>(k % arm.n_links);
+  if (any_hit[cfg] & 1) return;
+  const tf3 thm = poseFromQuat(H.pose7);
+  const tf3 m = relTf(thm, poseFromQuat(link_pose7 + 7 * k));  // link -> map frame
+  const float4* v = verts + hulls.voff[link];
+  const int nv = static_cast<int>(hulls.vcnt[link]);
+  d3 mn = {INFINITY, INFINITY, INFINITY}, mx = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < nv; ++i) {
+    const float4 q = v[i];
+    const d3 p = apply(m, d3{q.x, q.y, q.z});
+    mn.x = fmin(mn.x, p.x); mn.y = fmin(mn.y, p.y); mn.z = fmin(mn.z, p.z);
+    mx.x = fmax(mx.x, p.x); mx.y = fmax(mx.y, p.y); mx.z = fmax(mx.z, p.z);
+  }
+  // no vertex, a non-finite pose, the map's base or its top
+  if (!(mn.x <= mx.x && mn.y <= mx.y && mn.z <= mx.z)) return;
+  if (mx.z < 0 || mn.z > H.upper_m) return;
+  const int b = H.n_layers - 1;
+  const int fx = H.fx[b], fy = H.fy[b];
+  int x0, y0, x1, y1;
+  const bool c0 = hmProject(mn.x, H.rx[b], fx / 2, fx, x0) | hmProject(mn.y, H.ry[b], fy / 2, fy, y0);
+  const bool c1 = hmProject(mx.x, H.rx[b], fx / 2, fx, x1) | hmProject(mx.y, H.ry[b], fy / 2, fy, y1);
+  if (c0 && c1 && (x0 == x1 || y0 == y1)) return;
+  x0 = min(x0, x1);
+  y0 = min(y0, y1);
+  {
+    const double4 c = interior[arm.shape[link]];
+    bool in = hmInside(H, apply(m, d3{c.x, c.y, c.z}), 0.0);
+    for (int i = 0; i < nv && !in; ++i) {
+      const float4 q = v[i];
+      in = hmInside(H, apply(m, d3{q.x, q.y, q.z}), 1e-4);
+    }
+    if (in) {
+      atomicOr(&any_hit[cfg], 1);
+      return;
+    }
+  }
+  const int s = b >= 2 ? 2 : b;  // tile side 2^s, read from layer b - s
+  const int cl = b - s, fxc = fx >> s;
+  const uint16_t* hb = H.h + H.off[b];
+  const uint16_t* hc = H.h + H.off[cl];
+  const double zmin = mn.z;
+  int total = 0;
+  unsigned long long base = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    int j = 0;
+    for (int ty = y0 >> s; ty <= y1 >> s; ++ty)
+      for (int tx = x0 >> s; tx <= x1 >> s; ++tx) {
+        const int tmax = hc[static_cast<uint64_t>(ty) * fxc + tx];
+        if (tmax == 0 || zmin > static_cast<double>(tmax) * 0.001) continue;
+        const int ya = max(y0, ty << s), yb = min(y1, ((ty + 1) << s) - 1);
+        const int xa = max(x0, tx << s), xb = min(x1, ((tx + 1) << s) - 1);
+        for (int y = ya; y <= yb; ++y)
+          for (int x = xa; x <= xb; ++x) {
+            const int hmm = hb[static_cast<uint64_t>(y) * fx + x];
+            if (hmm == 0) continue;
+            const double top = static_cast<double>(hmm) * 0.001;
+            if (zmin > top) continue;
+            if (pass == 0 || j >= total) {
+              ++j;
+              continue;
+            }
+            const uint64_t p = base + j++;
+            if (p >= max_pairs) continue;
+            // the pixel's box first, as boxToShapeProcessLeafPair orders
+            // them (constructBox: about the pixel's centre, the map's
+            // rotation), then the link
+            pcfg[p] = static_cast<uint32_t>(cfg);
+            pa[p] = hm_shape0 + static_cast<uint32_t>(y * fx + x);
+            pb[p] = arm.shape[link];
+            const d3 c = apply(thm, d3{(x - fx / 2 + 0.5) * H.rx[b], (y - fy / 2 + 0.5) * H.ry[b], 0.5 * top});
+            qa[7 * p + 0] = static_cast<float>(c.x);
+            qa[7 * p + 1] = static_cast<float>(c.y);
+            qa[7 * p + 2] = static_cast<float>(c.z);
+            for (int t = 3; t < 7; ++t) qa[7 * p + t] = H.pose7[t];
+            for (int t = 0; t < 7; ++t) qb[7 * p + t] = link_pose7[7 * k + t];
+          }
+      }
+    if (pass == 0) {
+      if (j > cap) atomicOr(&any_hit[cfg], 2);
+      total = j < cap ? j : cap;
+      if (total == 0) return;
+      base = atomicAdd(n_pairs, static_cast<unsigned long long>(total));
+      if (base + total > max_pairs) atomicOr(&any_hit[cfg], 2);
+    }
+  }
+}
+
+// Collide-mode results of heightmap candidates: a hit marks the
+// configuration colliding, an invalid pair failed.
+__global__ void armHmReduceKernel(const mfcl_convex_result* res, const uint32_t* pcfg, uint64_t n_pairs,
+                                  int32_t* any_hit) {
+  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const mfcl_convex_result& r = res[p];
+  if (r.ok != 1) atomicOr(&any_hit[pcfg[p]], 2);
+  else if (r.hit) atomicOr(&any_hit[pcfg[p]], 1);
 }
 
 int aabbQueryLaunch(mfcl_ctx* ctx, const float* qmn, const float* qmx, uint64_t nq, int32_t cap, int32_t* counts,
@@ -177,6 +348,41 @@ int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t 
   if (static_cast<uint64_t>(arm->box_shape0) + arm->n_boxes > ctx->n_shapes) {
     ctx->err = "box shape ids out of range";
     return MFCL_E_INVALID;
+  }
+  ctx->arm_hm_pairs = 0;
+  if (arm->use_heightmap) {
+    if (ctx->hm.n_layers == 0 || !ctx->d_hm) {
+      ctx->err = "no heightmap uploaded";
+      return MFCL_E_STATE;
+    }
+    if (arm->hm_cap < 1 || arm->hm_cap > 65536) {
+      ctx->err = "hm_cap out of range [1, 65536]";
+      return MFCL_E_INVALID;
+    }
+    const int b = ctx->hm.n_layers - 1;
+    const uint64_t npix = static_cast<uint64_t>(ctx->hm.fx[b]) * ctx->hm.fy[b];
+    if (static_cast<uint64_t>(arm->hm_box_shape0) + npix > ctx->n_shapes) {
+      ctx->err = "heightmap pixel box shape ids out of range";
+      return MFCL_E_INVALID;
+    }
+    for (uint64_t i = 0; i < npix; ++i)
+      if (ctx->h_type[arm->hm_box_shape0 + i] != MFCL_SHAPE_BOX) {
+        ctx->err = "heightmap pixel shapes must be boxes";
+        return MFCL_E_INVALID;
+      }
+    for (int i = 0; i < arm->n_links; ++i)
+      if (ctx->h_type[arm->link_shape[i]] != MFCL_SHAPE_CONVEX) {
+        ctx->err = "a heightmap query needs convex-hull links";
+        return MFCL_E_INVALID;
+      }
+    float qn = 0;
+    for (int t = 3; t < 7; ++t) qn += arm->hm_pose7[t] * arm->hm_pose7[t];
+    for (int t = 0; t < 7; ++t)
+      if (!std::isfinite(arm->hm_pose7[t])) qn = 0;
+    if (!(qn > 0)) {
+      ctx->err = "hm_pose7 must be finite with a non-zero quaternion";
+      return MFCL_E_INVALID;
+    }
   }
   if (n_cfg == 0) return MFCL_OK;
   const uint64_t nq = n_cfg * static_cast<uint64_t>(arm->n_links);
@@ -235,7 +441,43 @@ int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t 
     mfcl_request_defaults(&req, MFCL_MODE_SIGNED_DISTANCE);
     req.gjk_max_iterations = arm->gjk_max_iterations ? arm->gjk_max_iterations : req.gjk_max_iterations;
     if (int rc = convexQueryLaunch(ctx, pa, pb, qa, qb, nullptr, np, &req, res, s)) return rc;
-    armReduceKernel<<<static_cast<unsigned>((np + 255) / 256), 256, 0, s>>>(res, pcfg, npairs, any_hit, min_key);
+    armReduceKernel<<<static_cast<unsigned>((np + 255) / 256), 256, 0, s>>>(res, pcfg, np, any_hit, min_key);
+  }
+  if (arm->use_heightmap) {
+    // the heightmap's candidates reuse the pair buffers (stream order)
+    HmDev H;
+    H.h = ctx->d_hm;
+    H.n_layers = ctx->hm.n_layers;
+    for (int l = 0; l < kMaxHmLayers; ++l) {
+      H.off[l] = ctx->hm.off[l];
+      H.fx[l] = ctx->hm.fx[l];
+      H.fy[l] = ctx->hm.fy[l];
+      H.rx[l] = ctx->hm.rx[l];
+      H.ry[l] = ctx->hm.ry[l];
+    }
+    H.upper_m = static_cast<double>(ctx->hm.upper_mm) * 0.001;
+    for (int t = 0; t < 7; ++t) H.pose7[t] = arm->hm_pose7[t];
+    ArmHulls hulls;
+    for (int i = 0; i < MFCL_ARM_MAX_LINKS; ++i) {
+      hulls.voff[i] = i < arm->n_links ? ctx->h_voff[arm->link_shape[i]] : 0;
+      hulls.vcnt[i] = i < arm->n_links ? ctx->h_vcnt[arm->link_shape[i]] : 0;
+    }
+    armHmKernel<<<gq, 256, 0, s>>>(a, hulls, H, ctx->d_verts, ctx->d_interior, pose7, nq, arm->hm_cap,
+                                   arm->hm_box_shape0, pa, pb, qa, qb, pcfg, max_pairs, npairs + 1, any_hit);
+    unsigned long long nh = 0;
+    MFCL_HIP_TRY(ctx, hipMemcpyAsync(&nh, npairs + 1, sizeof(nh), hipMemcpyDeviceToHost, s));
+    MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (nh > max_pairs) nh = max_pairs;
+    ctx->arm_hm_pairs = nh;
+    if (nh) {
+      mfcl_request req;
+      mfcl_request_defaults(&req, MFCL_MODE_COLLIDE);
+      req.max_contacts = 1;
+      req.penetration = 0;
+      req.gjk_max_iterations = arm->gjk_max_iterations ? arm->gjk_max_iterations : req.gjk_max_iterations;
+      if (int rc = convexQueryLaunch(ctx, pa, pb, qa, qb, nullptr, nh, &req, res, s)) return rc;
+      armHmReduceKernel<<<static_cast<unsigned>((nh + 255) / 256), 256, 0, s>>>(res, pcfg, nh, any_hit);
+    }
   }
   armFinishKernel<<<gc, 256, 0, s>>>(n_cfg, min_key, min_dist);
   MFCL_HIP_TRY(ctx, hipGetLastError());

@@ -64,9 +64,15 @@ void freeMeshes(mfcl_ctx* c) {
   c->mesh_host = MeshHost();
 }
 
+void freeHeightmap(mfcl_ctx* c) {
+  freePtr(reinterpret_cast<void*&>(c->d_hm));
+  c->hm = HeightmapHost();
+}
+
 void freeAll(mfcl_ctx* c) {
   freeShapes(c);
   freeMeshes(c);
+  freeHeightmap(c);
   freePtr(c->d_bp_nodes);
   freePtr(reinterpret_cast<void*&>(c->d_bp_ids));
   freePtr(c->ws_order.p);
@@ -632,6 +638,68 @@ int mfcl_mlp_train_step_dev(mfcl_ctx* ctx, const float* feat64, const float* tar
   const int rc = mlpTrainStep(ctx, feat64, target4, n, hyper6, apply_update, loss_host, s);
   if (rc) return rc;
   if (loss_host) MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return MFCL_OK;
+}
+
+// LayeredHeightMap(res_x, res_y, half_x, half_y) (layered_heightmap-inl.h:
+// 8-51): halve the shape (doubling the resolution) while both halves exceed
+// one, then build the layers from that top back down to the given bottom;
+// each upper pixel is the maximum of its 2 x 2 children (rebuildNextLayer,
+// :76-101).
+int mfcl_upload_heightmap(mfcl_ctx* ctx, const uint16_t* heights_mm, uint16_t half_x, uint16_t half_y, float res_x,
+                          float res_y) {
+  if (!ctx) return MFCL_E_INVALID;
+  auto pow2 = [](uint32_t v) { return v > 0 && (v & (v - 1)) == 0; };
+  if (!heights_mm || !pow2(half_x) || !pow2(half_y) || half_x > 16384 || half_y > 16384 || !(res_x > 0) ||
+      !(res_y > 0) || !std::isfinite(res_x) || !std::isfinite(res_y)) {
+    ctx->err = "heightmap: half shapes must be powers of two in [1, 16384] and resolutions positive";
+    return MFCL_E_INVALID;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  freeHeightmap(ctx);
+  int nx = half_x, ny = half_y, k = 0;
+  while (nx > 1 && ny > 1) {
+    nx /= 2;
+    ny /= 2;
+    ++k;
+  }
+  HeightmapHost h;
+  h.n_layers = k + 1;
+  std::vector<std::vector<uint16_t>> layer(h.n_layers);
+  uint64_t total = 0;
+  for (int l = 0; l < h.n_layers; ++l) {
+    const int up = k - l;  // halvings above the bottom
+    h.fx[l] = 2 * (half_x >> up);
+    h.fy[l] = 2 * (half_y >> up);
+    h.rx[l] = std::ldexp(res_x, up);
+    h.ry[l] = std::ldexp(res_y, up);
+    h.off[l] = total;
+    total += static_cast<uint64_t>(h.fx[l]) * h.fy[l];
+  }
+  const int b = h.n_layers - 1;
+  layer[b].assign(heights_mm, heights_mm + static_cast<size_t>(h.fx[b]) * h.fy[b]);
+  for (int l = b - 1; l >= 0; --l) {
+    layer[l].assign(static_cast<size_t>(h.fx[l]) * h.fy[l], 0);
+    for (int y = 0; y < h.fy[l + 1]; ++y)
+      for (int x = 0; x < h.fx[l + 1]; ++x) {
+        uint16_t& u = layer[l][static_cast<size_t>(y / 2) * h.fx[l] + x / 2];
+        u = std::max(u, layer[l + 1][static_cast<size_t>(y) * h.fx[l + 1] + x]);
+      }
+  }
+  for (uint16_t v : layer[b]) h.upper_mm = std::max<int>(h.upper_mm, v);
+  std::vector<uint16_t> all;
+  all.reserve(total);
+  for (auto& v : layer) all.insert(all.end(), v.begin(), v.end());
+  if (int rc = uploadVec(ctx, reinterpret_cast<void**>(&ctx->d_hm), all.data(), all.size() * sizeof(uint16_t)))
+    return rc;
+  ctx->hm = h;
+  return MFCL_OK;
+}
+
+int mfcl_arm_last_heightmap_pairs(mfcl_ctx* ctx, uint64_t* n_pairs) {
+  if (!ctx || !n_pairs) return MFCL_E_INVALID;
+  *n_pairs = ctx->arm_hm_pairs;
   return MFCL_OK;
 }
 

@@ -57,6 +57,19 @@ struct MeshHost {
   std::vector<uint32_t> node_count;
 };
 
+// Layers of a heightmap, top (0) to bottom (n_layers - 1), as
+// LayeredHeightMap builds them: layer l has full shape (fx, fy) = 2 x its half
+// shape and resolution (rx, ry); its pixels start at element off of the
+// device array.  upper_mm: the largest height.
+constexpr int kMaxHmLayers = 17;
+struct HeightmapHost {
+  int n_layers{0};
+  uint64_t off[kMaxHmLayers]{};
+  int fx[kMaxHmLayers]{}, fy[kMaxHmLayers]{};
+  float rx[kMaxHmLayers]{}, ry[kMaxHmLayers]{};
+  int upper_mm{0};
+};
+
 }  // namespace mfcl
 
 struct mfcl_ctx {
@@ -107,6 +120,12 @@ struct mfcl_ctx {
   void* d_bp_nodes{nullptr};
   int64_t* d_bp_ids{nullptr};
 
+  // heightmap (mfcl_upload_heightmap): every layer's uint16 mm heights in one
+  // array, top layer first
+  uint16_t* d_hm{nullptr};
+  mfcl::HeightmapHost hm;
+  uint64_t arm_hm_pairs{0};  // heightmap candidate pairs of the last arm query
+
   // growable device workspaces, keyed by purpose
   mfcl::DevBuf ws_order, ws_queue, ws_arena, ws_misc, ws_io, ws_oflow, ws_lane, ws_arm;
 
@@ -123,6 +142,8 @@ struct mfcl_ctx {
 namespace mfcl {
 
 int ensure(mfcl_ctx* ctx, DevBuf& b, size_t bytes);
+// capi.cpp: frees the heightmap
+void freeHeightmap(mfcl_ctx* ctx);
 void freeAll(mfcl_ctx* ctx);
 // hipMalloc + copy of `bytes` bytes (frees *dptr first).
 int uploadVec(mfcl_ctx* ctx, void** dptr, const void* src, size_t bytes);

@@ -54,10 +54,17 @@ class Arm(C.Structure):
         ("broadphase_cap", C.c_int32),
         ("gjk_max_iterations", C.c_int32),
         ("max_pairs", C.c_uint64),
+        ("use_heightmap", C.c_int32),
+        ("hm_box_shape0", C.c_uint32),
+        ("hm_cap", C.c_int32),
+        ("hm_pose7", C.c_float * 7),
     ]
 
 
-def make_arm(link_shape, link_len, link_radius, box_shape0, n_boxes, broadphase_cap=32, max_pairs=0) -> Arm:
+def make_arm(link_shape, link_len, link_radius, box_shape0, n_boxes, broadphase_cap=32, max_pairs=0,
+             heightmap=None) -> Arm:
+    """heightmap: None, or dict(box_shape0=first pixel box shape id, pose7=the
+    map frame's world pose, cap=candidate pixels kept per link)."""
     a = Arm()
     a.n_links = len(link_shape)
     for i in range(a.n_links):
@@ -66,6 +73,12 @@ def make_arm(link_shape, link_len, link_radius, box_shape0, n_boxes, broadphase_
         a.link_radius[i] = float(link_radius[i])
     a.box_shape0, a.n_boxes = int(box_shape0), int(n_boxes)
     a.broadphase_cap, a.gjk_max_iterations, a.max_pairs = int(broadphase_cap), 0, int(max_pairs)
+    if heightmap is not None:
+        a.use_heightmap = 1
+        a.hm_box_shape0 = int(heightmap["box_shape0"])
+        a.hm_cap = int(heightmap.get("cap", 4096))
+        for t in range(7):
+            a.hm_pose7[t] = float(heightmap["pose7"][t])
     return a
 
 
@@ -125,6 +138,8 @@ def load_library() -> C.CDLL:
     lib.mfcl_upload_shapes.argtypes = [p, p, p, p, p, u32, p, u64]
     lib.mfcl_upload_convex_faces.argtypes = [p, p, u64, p, p]
     lib.mfcl_arm_query_dev.argtypes = [p, C.POINTER(Arm), p, u64, C.c_float, p, p, p, C.POINTER(u64), p]
+    lib.mfcl_upload_heightmap.argtypes = [p, p, C.c_uint16, C.c_uint16, C.c_float, C.c_float]
+    lib.mfcl_arm_last_heightmap_pairs.argtypes = [p, C.POINTER(u64)]
     lib.mfcl_convex_query.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p]
     lib.mfcl_convex_query_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
@@ -364,6 +379,17 @@ class Context:
         self._check(self.lib.mfcl_arm_query_dev(self.h, C.byref(arm), q.ptr, n_configs, margin, box_pose7.ptr,
                                                 any_hit.ptr, min_dist.ptr, C.byref(n_pairs), None))
         return int(n_pairs.value)
+
+    def upload_heightmap(self, heights_mm, res_x: float, res_y: float):
+        """heights_mm: (2 half_y, 2 half_x) uint16, row y holds pixels (x, y)."""
+        h = np.ascontiguousarray(heights_mm, np.uint16)
+        fy, fx = h.shape
+        self._check(self.lib.mfcl_upload_heightmap(self.h, h.ctypes.data, fx // 2, fy // 2, res_x, res_y))
+
+    def arm_last_heightmap_pairs(self) -> int:
+        n = C.c_uint64(0)
+        self._check(self.lib.mfcl_arm_last_heightmap_pairs(self.h, C.byref(n)))
+        return int(n.value)
 
     # ---- learned warm start
     def mlp_num_params(self) -> int:

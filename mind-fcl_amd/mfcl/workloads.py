@@ -292,7 +292,40 @@ def mesh_pairs(pool: dict, n_pairs: int = 4096, seed: int = 30, spread: float = 
     return dict(pair_a=a, pair_b=b, pose_a=pa, pose_b=pb)
 
 
-def arm_scene(n_configs: int = 65536, seed: int = 6, n_links: int = 7, n_boxes: int = 512):
+def fbm_heightmap(rng, half: int = 128, octaves: int = 4, amplitude: float = 0.1) -> np.ndarray:
+    """(2 half, 2 half) uint16 millimetre heights: fractal Brownian motion of
+    value noise (octave o: a (4 2^o + 1)^2 lattice of U[0, 1] values,
+    smoothstep-interpolated, weight 2^-o), normalised to [0, 1] and scaled by
+    the amplitude in metres."""
+    n = 2 * half
+    acc = np.zeros((n, n))
+    wsum = 0.0
+    u = (np.arange(n) + 0.5) / n
+    for o in range(octaves):
+        g = 4 * 2 ** o
+        lat = rng.uniform(0.0, 1.0, size=(g + 1, g + 1))
+        x = u * g
+        i = np.minimum(x.astype(np.int64), g - 1)
+        f = x - i
+        f = f * f * (3 - 2 * f)
+        a = lat[i][:, i] * (1 - f)[None, :] + lat[i][:, i + 1] * f[None, :]
+        b = lat[i + 1][:, i] * (1 - f)[None, :] + lat[i + 1][:, i + 1] * f[None, :]
+        acc += (a * (1 - f)[:, None] + b * f[:, None]) * 0.5 ** o
+        wsum += 0.5 ** o
+    acc = (acc - acc.min()) / (acc.max() - acc.min())
+    return np.rint(acc * amplitude * 1000.0).astype(np.uint16)
+
+
+def heightmap_box_params(heights_mm: np.ndarray, res_x: float, res_y: float) -> np.ndarray:
+    """Box shape parameters (full sides res_x, res_y, height) of every pixel,
+    in pixel order y * full_x + x (the map's pixel boxes, constructBox)."""
+    h = np.asarray(heights_mm).reshape(-1).astype(np.float64) * 0.001
+    out = np.zeros((len(h), 4), np.float32)
+    out[:, 0], out[:, 1], out[:, 2] = np.float32(res_x), np.float32(res_y), h.astype(np.float32)
+    return out
+
+
+def arm_scene(n_configs: int = 65536, seed: int = 6, n_links: int = 7, n_boxes: int = 512, heightmap: bool = False):
     """Robot-arm configuration workload: a serial chain of n_links convex
     links (64..128 hull vertices each, with faces; link length U[0.1, 0.4] m,
     hull radius U[0.03, 0.08] m about the link's long axis), n_boxes
@@ -301,7 +334,13 @@ def arm_scene(n_configs: int = 65536, seed: int = 6, n_links: int = 7, n_boxes: 
     [0.5, 2.5] x [-1, 1] x [-1, 1] m, uniform rotations) and n_configs joint
     vectors U[-pi, pi].  With the cube around the base every configuration
     would collide; in front of it about a third do.  Shapes: the links first
-    (ids 0..n_links-1), then the boxes."""
+    (ids 0..n_links-1), then the boxes.
+
+    heightmap=True adds a 256 x 256 map of 1 cm pixels with fBm heights (4
+    octaves, 0.1 m amplitude) whose frame sits 0.45 m below the base, shifted
+    0.3 m along x and turned 0.3 rad about z, so the terrain's top lies
+    between 0.45 m and 0.35 m below the base; its 65,536 pixel boxes follow
+    the obstacle boxes in the shape table."""
     rng = np.random.default_rng(seed)
     lens = rng.uniform(0.1, 0.4, size=n_links).astype(np.float32)
     rad = rng.uniform(0.03, 0.08, size=n_links).astype(np.float32)
@@ -326,9 +365,24 @@ def arm_scene(n_configs: int = 65536, seed: int = 6, n_links: int = 7, n_boxes: 
     tri_off = np.concatenate([toff, np.zeros(n_boxes, np.int64)])
     tri_cnt = np.concatenate([tcnt, np.zeros(n_boxes, np.int64)])
     q = rng.uniform(-np.pi, np.pi, size=(n_configs, n_links)).astype(np.float32)
-    return dict(types=types, params=params, verts=verts, vert_off=vert_off, vert_cnt=vert_cnt, tris=tris,
-                tri_off=tri_off, tri_cnt=tri_cnt, link_len=lens, link_radius=link_radius, box_pose=box_pose,
-                box_half=half.astype(np.float32), q=q, n_links=n_links, n_boxes=n_boxes)
+    out = dict(types=types, params=params, verts=verts, vert_off=vert_off, vert_cnt=vert_cnt, tris=tris,
+               tri_off=tri_off, tri_cnt=tri_cnt, link_len=lens, link_radius=link_radius, box_pose=box_pose,
+               box_half=half.astype(np.float32), q=q, n_links=n_links, n_boxes=n_boxes)
+    if heightmap:
+        hm = fbm_heightmap(np.random.default_rng(seed + 1000))
+        res = 0.01
+        npix = hm.size
+        out["hm_heights"] = hm
+        out["hm_res"] = res
+        out["hm_box_shape0"] = n_links + n_boxes
+        out["hm_pose7"] = np.array([0.3, 0.0, -0.45, np.cos(0.15), 0.0, 0.0, np.sin(0.15)], np.float32)
+        out["types"] = np.concatenate([types, np.full(npix, BOX, np.int32)])
+        out["params"] = np.concatenate([params, heightmap_box_params(hm, res, res)])
+        out["vert_off"] = np.concatenate([vert_off, np.zeros(npix, np.uint64)])
+        out["vert_cnt"] = np.concatenate([vert_cnt, np.zeros(npix, np.uint32)])
+        out["tri_off"] = np.concatenate([tri_off, np.zeros(npix, np.int64)])
+        out["tri_cnt"] = np.concatenate([tri_cnt, np.zeros(npix, np.int64)])
+    return out
 
 
 def box_aabbs(box_pose: np.ndarray, half: np.ndarray):

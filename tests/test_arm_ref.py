@@ -70,3 +70,66 @@ def test_pipeline_hit_and_clear(oracle):
     flags, mind, npairs = A.pipeline(scene, q, 0.05, 8, S, tree)
     assert flags[0] & 1 and mind[0] < 0
     assert flags[1] == 0 and np.isinf(mind[1])
+
+
+# ---- heightmap ---------------------------------------------------------------
+def _hm_scene(n=256):
+    return W.arm_scene(n, seed=6, heightmap=True)
+
+
+def test_fbm_heightmap_shape_and_range():
+    h = W.fbm_heightmap(np.random.default_rng(0))
+    assert h.shape == (256, 256) and h.dtype == np.uint16
+    assert h.min() == 0 and h.max() == 100  # normalised to [0, 0.1] m
+    # smooth at the pixel scale: neighbouring pixels differ by a few mm
+    assert np.abs(np.diff(h.astype(np.int64), axis=1)).max() <= 10
+
+
+def test_hm_candidates_cover_every_colliding_pixel():
+    """The flat algorithm only culls: for links with candidates, every pixel
+    box of the whole map that the oracle finds colliding (brute force over
+    all 65,536 pixels) is a candidate."""
+    sc = _hm_scene(64)
+    pose = A.fk(sc["q"], sc["link_len"])
+    k, x, y, *_ = A.hm_candidates(sc, pose)
+    S = A.shape_set(sc)
+    fy, fx = sc["hm_heights"].shape
+    qs = np.unique(k)[:6]
+    assert len(qs) >= 3
+    for q in qs:
+        allk = np.full(fx * fy, q)
+        ax, ay = np.meshgrid(np.arange(fx), np.arange(fy))
+        ax, ay = ax.reshape(-1), ay.reshape(-1)
+        keep = sc["hm_heights"][ay, ax] > 0
+        pa, pb, qa, qb = A.hm_pairs(sc, allk[keep], ax[keep], ay[keep], pose)
+        ref, _ = S.collide(pa, pb, pyoracle.pose12(qa), pyoracle.pose12(qb), max_contacts=1, penetration=False,
+                           threads=8)
+        hit_pix = set(zip(ax[keep][ref["hit"] == 1].tolist(), ay[keep][ref["hit"] == 1].tolist()))
+        cand = set(zip(x[k == q].tolist(), y[k == q].tolist()))
+        assert hit_pix <= cand, (q, len(hit_pix - cand))
+
+
+def test_hm_pixel_box_and_roi_closed_forms():
+    """Pixel boxes: sides (res, res, h) and world centres on the map's
+    rotated grid; a link placed inside a tall pixel's box collides, one
+    raised above the map's highest pixel has no candidate."""
+    sc = _hm_scene(4)
+    H = sc["hm_heights"]
+    res = sc["hm_res"]
+    y, x = np.unravel_index(np.argmax(H), H.shape)
+    pa, pb, qa, qb = A.hm_pairs(sc, np.array([0]), np.array([x]), np.array([y]), A.fk(sc["q"], sc["link_len"]))
+    ang = 0.3
+    cx, cy = (x - 128 + 0.5) * res, (y - 128 + 0.5) * res
+    want = np.array([0.3 + np.cos(ang) * cx - np.sin(ang) * cy, np.sin(ang) * cx + np.cos(ang) * cy,
+                     -0.45 + 0.5 * H[y, x] * 1e-3])
+    np.testing.assert_allclose(qa[0, :3], want, atol=2e-7)
+    assert pa[0] == sc["hm_box_shape0"] + y * 256 + x
+    np.testing.assert_allclose(sc["params"][pa[0], :3], [res, res, H[y, x] * 1e-3], rtol=1e-7)
+    # link 0 of configuration 0 moved: centred inside that pixel box / far above the map
+    for z, expect in ((0.5 * H[y, x] * 1e-3, True), (0.5, False)):
+        p = A.fk(sc["q"][:1], sc["link_len"]).copy()
+        ctr = np.array([cx, cy, z])
+        p[0, 0, :3] = np.array([0.3 + np.cos(ang) * ctr[0] - np.sin(ang) * ctr[1],
+                                np.sin(ang) * ctr[0] + np.cos(ang) * ctr[1], -0.45 + ctr[2]], np.float32)
+        hit, cnt, _ = A.hm_collide(sc, A.shape_set(sc), p, threads=1)
+        assert hit[0] == expect and (cnt[0] > 0) == expect

@@ -51,7 +51,8 @@ def _c_layout(tmp_path):
           printf("sizeof.arm %zu\\n", sizeof(mfcl_arm));
           OFF(mfcl_arm, n_links); OFF(mfcl_arm, link_shape); OFF(mfcl_arm, link_len); OFF(mfcl_arm, link_radius);
           OFF(mfcl_arm, box_shape0); OFF(mfcl_arm, n_boxes); OFF(mfcl_arm, broadphase_cap);
-          OFF(mfcl_arm, gjk_max_iterations); OFF(mfcl_arm, max_pairs);
+          OFF(mfcl_arm, gjk_max_iterations); OFF(mfcl_arm, max_pairs); OFF(mfcl_arm, use_heightmap);
+          OFF(mfcl_arm, hm_box_shape0); OFF(mfcl_arm, hm_cap); OFF(mfcl_arm, hm_pose7);
           OFF(mfcl_request, mode); OFF(mfcl_request, max_contacts); OFF(mfcl_request, penetration);
           OFF(mfcl_request, gjk_max_iterations); OFF(mfcl_request, gjk_tolerance);
           OFF(mfcl_request, epa_max_faces); OFF(mfcl_request, epa_max_iterations);

@@ -70,3 +70,68 @@ def test_arm_query_rejects_bad_arguments(gpu_ctx):
     bad = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb, broadphase_cap=0)
     with pytest.raises(RuntimeError, match="broadphase_cap"):
         gpu_ctx.arm_query_dev(bad, dq, 8, MARGIN, dbox, dany, dmin)
+
+
+@pytest.mark.gpu
+def test_arm_heightmap_matches_cpu_pipeline(gpu_ctx):
+    """The arm query with the 256 x 256 fBm heightmap: any-collision per
+    configuration equals the CPU pipeline (box pairs by signed distance, map
+    candidates by ShapeIntersect(Box, Convex) per pixel box) wherever the
+    configuration is not touching (a mismatch is allowed only when some
+    candidate of that configuration has |signed distance| <= 1e-5); the
+    minimum distance is that of the boxes alone."""
+    scene = W.arm_scene(2048, seed=6, heightmap=True)
+    L, nb = scene["n_links"], scene["n_boxes"]
+    gpu_ctx.upload_shapes(scene["types"], scene["params"], scene["vert_off"], scene["vert_cnt"], scene["verts"],
+                          faces=mfcl.faces_from_triangles(scene["tris"], scene["tri_off"], scene["tri_cnt"]))
+    gpu_ctx.upload_heightmap(scene["hm_heights"], scene["hm_res"], scene["hm_res"])
+    mn, mx = W.box_aabbs(scene["box_pose"], scene["box_half"])
+    gpu_ctx.aabbtree_build(mn, mx, np.arange(nb))
+    hm = dict(box_shape0=scene["hm_box_shape0"], pose7=scene["hm_pose7"], cap=4096)
+    arm = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb, broadphase_cap=CAP,
+                        heightmap=hm)
+    n = len(scene["q"])
+    dq, dbox = gpu_ctx.to_device(scene["q"]), gpu_ctx.to_device(scene["box_pose"])
+    dany, dmin = gpu_ctx.alloc(4 * n), gpu_ctx.alloc(4 * n)
+    npairs = gpu_ctx.arm_query_dev(arm, dq, n, MARGIN, dbox, dany, dmin)
+    n_hm = gpu_ctx.arm_last_heightmap_pairs()
+    any_hit, min_d = np.zeros(n, np.int32), np.zeros(n, np.float32)
+    dany.download(any_hit)
+    dmin.download(min_d)
+
+    S = A.shape_set(scene)
+    tree = pyoracle.AABBTree(mn, mx, np.arange(nb))
+    pose = A.fk(scene["q"], scene["link_len"])
+    flags, mind, nref = A.pipeline(scene, scene["q"], MARGIN, CAP, S, tree, threads=8, pose=pose)
+    hm_hit, hm_cnt, (k, x, y) = A.hm_collide(scene, S, pose, threads=8)
+    want = (flags & 1).astype(bool) | hm_hit.reshape(n, L).any(1)
+    got = (any_hit & 1).astype(bool)
+    mism = np.nonzero(got != want)[0]
+    touching = []
+    for c in mism:
+        sel = (k // L) == c
+        if not sel.any():
+            touching.append(False)
+            continue
+        pa, pb, qa, qb = A.hm_pairs(scene, k[sel], x[sel], y[sel], pose)
+        ref, _ = S.signed_distance(pa, pb, pyoracle.pose12(qa), pyoracle.pose12(qb), threads=8)
+        touching.append(bool((np.abs(ref["distance"]) <= 1e-5).any()) or abs(mind[c]) <= 1e-5)
+    print(f"\n[arm+heightmap] configs={n} colliding cpu={int(want.sum())} gpu={int(got.sum())} "
+          f"map-only={int((want & ~(flags & 1).astype(bool)).sum())} box pairs={npairs} map pairs gpu={n_hm} "
+          f"map candidates cpu={len(k)} mismatches={len(mism)} (touching {sum(touching)}) {mism[:10].tolist()}")
+    assert all(touching)
+    assert int((any_hit & 2).sum()) == 0 and int((flags & 2).sum()) == 0
+    assert hm_cnt.max() <= 4096
+    inside = (mind < 0.5 * MARGIN) & (min_d < 0.5 * MARGIN)
+    derr = np.abs(min_d - mind)[inside]
+    assert derr.size == 0 or derr.max() <= 1e-5
+    # the shortcuts decide most colliding links before any narrowphase
+    assert n_hm < len(k)
+
+
+@pytest.mark.gpu
+def test_heightmap_upload_rejects_bad_shapes(gpu_ctx):
+    with pytest.raises(RuntimeError, match="powers of two"):
+        gpu_ctx.upload_heightmap(np.zeros((6, 6), np.uint16), 0.01, 0.01)
+    with pytest.raises(RuntimeError, match="powers of two"):
+        gpu_ctx.upload_heightmap(np.zeros((8, 8), np.uint16), 0.0, 0.01)
