@@ -572,29 +572,11 @@ __global__ void zeroKernel(uint32_t* p, int n) {
 // Seed vertices of the support walk (convex-inl.h:151-199): per walking
 // hull, the first vertex attaining the maximum of +x, -x, +y, -y, +z, -z
 // (convex-inl.h:133-149; the dot product with a unit axis is the coordinate).
-// It also stores each vertex's neighbour-record offset in the vertex's
-// fourth word (read together with the vertex by the walk) and, for hulls of
-// at most 65535 vertices, the vertex's walk record: (x, y, z, neighbour
-// count) and the first eight neighbour ids packed as 16-bit pairs.
-__global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed, float4* verts_rw, float4* wrec) {
+__global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || T.nbr_off[i] < 0) return;
   const float4* v = T.verts + T.vert_off[i];
   const int nv = static_cast<int>(T.vert_cnt[i]);
-  const int* tab = T.nbr + T.nbr_off[i];
-  for (int k = 0; k < nv; ++k) {
-    const uint64_t g = T.vert_off[i] + static_cast<uint64_t>(k);
-    verts_rw[g].w = __int_as_float(tab[k]);
-    if (nv > 65535) continue;
-    const int* rec = tab + tab[k];
-    const int cnt = rec[0];
-    uint32_t u[4] = {0, 0, 0, 0};
-    for (int j = 0; j < 8 && j < cnt; ++j) u[j >> 1] |= static_cast<uint32_t>(rec[1 + j]) << (16 * (j & 1));
-    const float4 q = v[k];
-    wrec[2 * g] = make_float4(q.x, q.y, q.z, __int_as_float(cnt));
-    wrec[2 * g + 1] = make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]), __uint_as_float(u[2]),
-                                  __uint_as_float(u[3]));
-  }
   int idx[6] = {0, 0, 0, 0, 0, 0};
   double best[6];
   {
@@ -619,11 +601,37 @@ __global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed, float4* ver
   seed[8 * static_cast<uint64_t>(i) + 6] = seed[8 * static_cast<uint64_t>(i) + 7] = 0;
 }
 
+// Per walking hull (one block each, its threads over the vertices): every
+// vertex's neighbour-record offset goes into the vertex's fourth word (read
+// together with the vertex by the walk) and, for hulls of at most 65535
+// vertices, its walk record: (x, y, z, neighbour count) and the first eight
+// neighbour ids packed as 16-bit pairs.
+__global__ void walkRecKernel(ShapeTables T, float4* verts_rw, float4* wrec) {
+  const uint32_t i = blockIdx.x;
+  if (T.nbr_off[i] < 0) return;
+  const int nv = static_cast<int>(T.vert_cnt[i]);
+  const int* tab = T.nbr + T.nbr_off[i];
+  for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+    const uint64_t g = T.vert_off[i] + static_cast<uint64_t>(k);
+    const float4 q = T.verts[g];
+    verts_rw[g].w = __int_as_float(tab[k]);
+    if (nv > 65535) continue;
+    const int* rec = tab + tab[k];
+    const int cnt = rec[0];
+    uint32_t u[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 8 && j < cnt; ++j) u[j >> 1] |= static_cast<uint32_t>(rec[1 + j]) << (16 * (j & 1));
+    wrec[2 * g] = make_float4(q.x, q.y, q.z, __int_as_float(cnt));
+    wrec[2 * g + 1] = make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]), __uint_as_float(u[2]),
+                                  __uint_as_float(u[3]));
+  }
+}
+
 int computeWalkSeeds(mfcl_ctx* ctx) {
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
                 ctx->d_seed, ctx->d_wrec};
   const uint32_t n = ctx->n_shapes;
-  walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed, ctx->d_verts, ctx->d_wrec);
+  walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed);
+  walkRecKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_verts, ctx->d_wrec);
   MFCL_HIP_TRY(ctx, hipGetLastError());
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return MFCL_OK;
