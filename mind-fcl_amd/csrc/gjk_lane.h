@@ -75,7 +75,7 @@ struct MinkLane {
     s.p0 = p.x; s.p1 = p.y; s.p2 = p.z;
     s.verts = nullptr; s.nv = 0; s.rbound = p.w;
     s.lv = nullptr; s.ls = 0;
-    s.nbr = nullptr; s.seed = nullptr; s.wrec = nullptr;
+    s.nbr = nullptr; s.seed = nullptr; s.wrec = nullptr; s.wblk = nullptr;
     return s;
   }
   MFCL_D const float4* verts(int k) const {

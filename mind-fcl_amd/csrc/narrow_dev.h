@@ -38,6 +38,9 @@ struct DevShape {
   // walking hull: per vertex two float4, (x, y, z, neighbour count) and its
   // first eight neighbour ids as 16-bit pairs; nullptr: not available
   const float4* wrec;
+  // walking hull: per vertex a 128-byte block, the walk record followed by
+  // the coordinates of its first eight neighbours; nullptr: not available
+  const float4* wblk;
 };
 
 // Vertex i of a hull, from the LDS copy when there is one (same values).
@@ -84,6 +87,7 @@ struct ShapeTables {
   const int64_t* nbr_off; // per shape, -1: naive scans; nullptr: no faces uploaded
   const int* seed;        // per shape: 8 ints, the walk's axis seed vertices
   const float4* wrec;     // walk records, two float4 per vertex (indexed like verts)
+  const float4* wblk;     // walk blocks, eight float4 per vertex (indexed like verts)
 };
 
 MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
@@ -97,6 +101,7 @@ MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
   s.nbr = nullptr;
   s.seed = nullptr;
   s.wrec = nullptr;
+  s.wblk = nullptr;
   if (s.type == kConvex) {
     s.verts = T.verts + T.vert_off[i];
     s.nv = static_cast<int>(T.vert_cnt[i]);
@@ -106,6 +111,7 @@ MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
         s.nbr = T.nbr + o;
         s.seed = T.seed + 8 * static_cast<uint64_t>(i);
         if (T.wrec && s.nv <= 65535) s.wrec = T.wrec + 2 * static_cast<uint64_t>(T.vert_off[i]);
+        if (T.wblk && s.nv <= 65535) s.wblk = T.wblk + 8 * static_cast<uint64_t>(T.vert_off[i]);
       }
     }
   } else {
@@ -388,10 +394,136 @@ MFCL_D void walkCompare(Walk& W, const DevShape& s) {
   W.start = next_start;
 }
 
-// allow_rec = false keeps to the adjacency table: the lane kernel's many
-// independent walks are bound by memory traffic rather than latency, and the
-// 32-byte records double the bytes a step touches.
+// The walk over 128-byte blocks (s.wblk): a vertex's block holds its
+// coordinates, its neighbour count, its first eight neighbour ids and those
+// neighbours' coordinates, so a step reads one cache line -- the chosen
+// neighbour's block -- instead of a record and up to eight vertices spread
+// over the hull.  The comparisons, their order and the parent rule are
+// walkCompare's; neighbours beyond the eighth come from the adjacency table.
+struct BlkWalk {
+  int cnt;
+  uint32_t ids[4];
+  float nx[8][3];
+};
+MFCL_D float3 blkLoad(const DevShape& s, int v, BlkWalk& B) {
+  const float4* p = s.wblk + 8 * static_cast<uint64_t>(v);
+  const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6], q7 = p[7];
+  B.cnt = __float_as_int(q0.w);
+  B.ids[0] = __float_as_uint(q1.x); B.ids[1] = __float_as_uint(q1.y);
+  B.ids[2] = __float_as_uint(q1.z); B.ids[3] = __float_as_uint(q1.w);
+  const float f[24] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q4.x, q4.y, q4.z, q4.w,
+                       q5.x, q5.y, q5.z, q5.w, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y, q7.z, q7.w};
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    B.nx[j][0] = f[3 * j];
+    B.nx[j][1] = f[3 * j + 1];
+    B.nx[j][2] = f[3 * j + 2];
+  }
+  return make_float3(q0.x, q0.y, q0.z);
+}
+
+struct BlkState {
+  BlkWalk B;
+  d3 d;
+  double ext_v;
+  int ext, parent0, parent1, visited;
+  bool keep;
+};
+
+MFCL_D void blkInit(BlkState& W, const DevShape& s, d3 d, bool active) {
+  W.d = d;
+  W.keep = active;
+  W.visited = 0;
+  W.ext = 0;
+  W.ext_v = 0;
+  W.B.cnt = 0;
+  if (!active) return;
+  int init = -1;
+  double max_dot = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    // the seed directions of convex-inl.h:151-199: +-UnitX, +-UnitY, +-UnitZ
+    const double sg = (i & 1) ? -1.0 : 1.0;
+    const double ax = (i >> 1) == 0 ? sg : sg * 0.0;
+    const double ay = (i >> 1) == 1 ? sg : sg * 0.0;
+    const double az = (i >> 1) == 2 ? sg : sg * 0.0;
+    const double dv = ax * d.x + ay * d.y + az * d.z;
+    if (init < 0 || dv > max_dot) {
+      init = s.seed[i];
+      max_dot = dv;
+    }
+  }
+  const float3 q = blkLoad(s, init, W.B);
+  W.ext_v = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
+  W.ext = W.parent0 = W.parent1 = init;
+  W.visited = 1;
+}
+
+MFCL_D void blkStep(BlkState& W, const DevShape& s) {
+  if (!W.keep) return;
+  W.keep = false;
+  const int old = W.ext;
+  const int count = W.B.cnt;
+  const d3 d = W.d;
+#pragma unroll
+  for (int j = 0; j < kWalkChunk; ++j) {
+    const int id = static_cast<int>((W.B.ids[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
+    if (j >= count || id == W.parent0 || id == W.parent1) continue;
+    const double v = d.x * static_cast<double>(W.B.nx[j][0]) + d.y * static_cast<double>(W.B.nx[j][1]) +
+                     d.z * static_cast<double>(W.B.nx[j][2]);
+    ++W.visited;
+    if (v > W.ext_v) {
+      W.parent1 = W.ext;
+      W.keep = true;
+      W.ext = id;
+      W.ext_v = v;
+    }
+  }
+  if (count > kWalkChunk) {
+    const int start = __float_as_int(s.verts[old].w);
+    for (int c0 = kWalkChunk; c0 < count; c0 += kWalkChunk) {
+      int id[kWalkChunk];
+      float3 w[kWalkChunk];
+#pragma unroll
+      for (int j = 0; j < kWalkChunk; ++j) {
+        const int nb = s.nbr[start + 1 + c0 + j];
+        id[j] = c0 + j < count ? nb : old;
+        const float4 t = s.verts[id[j]];
+        w[j] = make_float3(t.x, t.y, t.z);
+      }
+#pragma unroll
+      for (int j = 0; j < kWalkChunk; ++j) {
+        if (c0 + j >= count || id[j] == W.parent0 || id[j] == W.parent1) continue;
+        const double v = d.x * static_cast<double>(w[j].x) + d.y * static_cast<double>(w[j].y) +
+                         d.z * static_cast<double>(w[j].z);
+        ++W.visited;
+        if (v > W.ext_v) {
+          W.parent1 = W.ext;
+          W.keep = true;
+          W.ext = id[j];
+          W.ext_v = v;
+        }
+      }
+    }
+  }
+  W.parent0 = old;
+  if (W.keep) blkLoad(s, W.ext, W.B);
+}
+
+MFCL_D int hullWalkBlk(const DevShape& s, d3 d, int* scanned) {
+  BlkState W;
+  blkInit(W, s, d, true);
+  while (W.keep) blkStep(W, s);
+  *scanned += W.visited;
+  return W.ext;
+}
+
+// allow_rec = false keeps to the adjacency table (or the walk blocks when
+// there are): the lane kernel's many independent walks are bound by memory
+// traffic rather than latency, and the 32-byte records double the bytes a
+// step touches.
 MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned, bool allow_rec = true) {
+  if (!allow_rec && s.wblk && !s.lv) return hullWalkBlk(s, d, scanned);
   Walk W;
   const bool rec = allow_rec && s.wrec && !s.lv;
   walkInit(W, s, d, true, rec);
@@ -412,6 +544,19 @@ MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned, bool allow_rec = true
 // the extreme vertices of the active ones.
 MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevShape& sb, d3 db, bool act_b,
                       int& ib, int* scanned) {
+  if (sa.wblk && !sa.lv && sb.wblk && !sb.lv) {
+    BlkState A, B;
+    blkInit(A, sa, da, act_a);
+    blkInit(B, sb, db, act_b);
+    while (A.keep || B.keep) {
+      blkStep(A, sa);
+      blkStep(B, sb);
+    }
+    if (act_a) ia = A.ext;
+    if (act_b) ib = B.ext;
+    *scanned += A.visited + B.visited;
+    return;
+  }
   Walk A, B;
   const bool rec = sa.wrec && !sa.lv && sb.wrec && !sb.lv;
   walkInit(A, sa, da, act_a, rec);
@@ -535,6 +680,8 @@ struct MinkCfg {
   const int* seed1;
   const float4* wrec0;
   const float4* wrec1;
+  const float4* wblk0;
+  const float4* wblk1;
   int ls0, ls1;
   int type0, type1, nv0, nv1;
   float rb0, rb1;
@@ -571,6 +718,7 @@ MFCL_D void writeCfg(MFCL_LDS MinkCfg* c, const DevShape& a, const tf3& ta, cons
   c->nbr0 = a.nbr; c->nbr1 = b.nbr;
   c->seed0 = a.seed; c->seed1 = b.seed;
   c->wrec0 = a.wrec; c->wrec1 = b.wrec;
+  c->wblk0 = a.wblk; c->wblk1 = b.wblk;
   c->ls0 = a.ls; c->ls1 = b.ls;
   c->type0 = a.type; c->type1 = b.type;
   c->nv0 = a.nv; c->nv1 = b.nv;
@@ -593,7 +741,7 @@ struct MinkL {
     s.type = c->type0; s.p0 = c->q0[0]; s.p1 = c->q0[1]; s.p2 = c->q0[2];
     s.verts = c->v0; s.nv = c->nv0; s.rbound = c->rb0;
     s.lv = c->lv0; s.ls = c->ls0;
-    s.nbr = c->nbr0; s.seed = c->seed0; s.wrec = c->wrec0;
+    s.nbr = c->nbr0; s.seed = c->seed0; s.wrec = c->wrec0; s.wblk = c->wblk0;
     return s;
   }
   MFCL_D DevShape sh1() const {
@@ -601,7 +749,7 @@ struct MinkL {
     s.type = c->type1; s.p0 = c->q1[0]; s.p1 = c->q1[1]; s.p2 = c->q1[2];
     s.verts = c->v1; s.nv = c->nv1; s.rbound = c->rb1;
     s.lv = c->lv1; s.ls = c->ls1;
-    s.nbr = c->nbr1; s.seed = c->seed1; s.wrec = c->wrec1;
+    s.nbr = c->nbr1; s.seed = c->seed1; s.wrec = c->wrec1; s.wblk = c->wblk1;
     return s;
   }
   MFCL_D m33 rot1() const { return loadM33(c->r1); }

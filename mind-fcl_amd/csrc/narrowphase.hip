@@ -626,12 +626,52 @@ __global__ void walkRecKernel(ShapeTables T, float4* verts_rw, float4* wrec) {
   }
 }
 
+// Walk blocks (one block of threads per walking hull of at most 65535
+// vertices): per vertex 8 float4 = (x, y, z, neighbour count), the first
+// eight neighbour ids as 16-bit pairs, then those neighbours' x, y, z
+// (24 floats, zero past the count).  Runs after walkRecKernel has written
+// the vertices' fourth words; reads only their coordinates.
+__global__ void walkBlkKernel(ShapeTables T, float4* wblk) {
+  const uint32_t i = blockIdx.x;
+  if (T.nbr_off[i] < 0) return;
+  const int nv = static_cast<int>(T.vert_cnt[i]);
+  if (nv > 65535) return;
+  const int* tab = T.nbr + T.nbr_off[i];
+  const float4* v = T.verts + T.vert_off[i];
+  for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+    const uint64_t g = T.vert_off[i] + static_cast<uint64_t>(k);
+    const int* rec = tab + tab[k];
+    const int cnt = rec[0];
+    uint32_t u[4] = {0, 0, 0, 0};
+    float f[24];
+#pragma unroll
+    for (int j = 0; j < 24; ++j) f[j] = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (j >= cnt) break;
+      const int nb = rec[1 + j];
+      u[j >> 1] |= static_cast<uint32_t>(nb) << (16 * (j & 1));
+      const float4 q = v[nb];
+      f[3 * j] = q.x;
+      f[3 * j + 1] = q.y;
+      f[3 * j + 2] = q.z;
+    }
+    const float4 self = v[k];
+    float4* o = wblk + 8 * g;
+    o[0] = make_float4(self.x, self.y, self.z, __int_as_float(cnt));
+    o[1] = make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]), __uint_as_float(u[2]), __uint_as_float(u[3]));
+#pragma unroll
+    for (int j = 0; j < 6; ++j) o[2 + j] = make_float4(f[4 * j], f[4 * j + 1], f[4 * j + 2], f[4 * j + 3]);
+  }
+}
+
 int computeWalkSeeds(mfcl_ctx* ctx) {
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
-                ctx->d_seed, ctx->d_wrec};
+                ctx->d_seed, ctx->d_wrec, ctx->d_wblk};
   const uint32_t n = ctx->n_shapes;
   walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed);
   walkRecKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_verts, ctx->d_wrec);
+  walkBlkKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_wblk);
   MFCL_HIP_TRY(ctx, hipGetLastError());
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return MFCL_OK;
@@ -695,7 +735,7 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   QEntry* queue = static_cast<QEntry*>(ctx->ws_queue.p);
 
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
-                ctx->d_seed, ctx->d_wrec};
+                ctx->d_seed, ctx->d_wrec, ctx->d_wblk};
   PairIO io{pa, pb, posea, poseb, guess, out};
   const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
