@@ -49,7 +49,6 @@ void freeShapes(mfcl_ctx* c) {
   freePtr(reinterpret_cast<void*&>(c->d_nbr));
   freePtr(reinterpret_cast<void*&>(c->d_nbr_off));
   freePtr(reinterpret_cast<void*&>(c->d_seed));
-  freePtr(reinterpret_cast<void*&>(c->d_wrec));
   freePtr(reinterpret_cast<void*&>(c->d_wblk));
   c->n_shapes = 0;
   c->n_verts_alloc = 0;
@@ -388,7 +387,6 @@ int mfcl_upload_convex_faces(mfcl_ctx* ctx, const int32_t* faces, uint64_t faces
     freePtr(reinterpret_cast<void*&>(ctx->d_nbr));
     freePtr(reinterpret_cast<void*&>(ctx->d_nbr_off));
     freePtr(reinterpret_cast<void*&>(ctx->d_seed));
-    freePtr(reinterpret_cast<void*&>(ctx->d_wrec));
     freePtr(reinterpret_cast<void*&>(ctx->d_wblk));
   }
   return rc;

@@ -141,8 +141,6 @@ int uploadConvexFaces(mfcl_ctx* ctx, const int32_t* faces, const uint64_t* face_
   if ((rc = uploadVec(ctx, reinterpret_cast<void**>(&ctx->d_nbr_off), off.data(), off.size() * sizeof(int64_t))))
     return rc;
   if ((rc = uploadVec(ctx, reinterpret_cast<void**>(&ctx->d_seed), nullptr, 8 * sizeof(int32_t) * n))) return rc;
-  if ((rc = uploadVec(ctx, reinterpret_cast<void**>(&ctx->d_wrec), nullptr, 2 * sizeof(float4) * ctx->n_verts_alloc)))
-    return rc;
   if ((rc = uploadVec(ctx, reinterpret_cast<void**>(&ctx->d_wblk), nullptr, 8 * sizeof(float4) * ctx->n_verts_alloc)))
     return rc;
   return computeWalkSeeds(ctx);

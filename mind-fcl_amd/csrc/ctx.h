@@ -101,7 +101,6 @@ struct mfcl_ctx {
   int32_t* d_nbr{nullptr};      // adjacency tables in the reference's encoding
   int64_t* d_nbr_off{nullptr};  // per shape: table offset, -1 = naive scans; null = no faces
   int32_t* d_seed{nullptr};     // per shape: 8 ints, the walk's six axis seed vertices
-  float4* d_wrec{nullptr};      // per vertex: 2 float4 walk records (x, y, z, count | 8 x u16 ids)
   float4* d_wblk{nullptr};      // per vertex: 8 float4 walk blocks (record + 8 neighbours' x, y, z)
   uint64_t n_verts_alloc{0};    // float4 entries of d_verts
   std::vector<uint32_t> h_vcnt, h_voff;

@@ -75,7 +75,7 @@ struct MinkLane {
     s.p0 = p.x; s.p1 = p.y; s.p2 = p.z;
     s.verts = nullptr; s.nv = 0; s.rbound = p.w;
     s.lv = nullptr; s.ls = 0;
-    s.nbr = nullptr; s.seed = nullptr; s.wrec = nullptr; s.wblk = nullptr;
+    s.nbr = nullptr; s.seed = nullptr; s.wblk = nullptr;
     return s;
   }
   MFCL_D const float4* verts(int k) const {
@@ -280,8 +280,8 @@ MFCL_D void laneSupport(bool need, const MinkLane& md, d3 d, int id, MV& out, ui
       const DevShape s0 = loadShape(md.T, md.ia), s1 = loadShape(md.T, md.ib);
       // one walk after the other: the lanes' independent walks already keep
       // enough memory requests in flight, and lock-step pairs cost registers
-      if (w0) idx0 = hullWalk(s0, d, &walked, false);
-      if (w1) idx1 = hullWalk(s1, d1, &walked, false);
+      if (w0) idx0 = hullWalk(s0, d, &walked);
+      if (w1) idx1 = hullWalk(s1, d1, &walked);
     }
   }
   if (need) {

@@ -35,9 +35,6 @@ struct DevShape {
   // vertex walk); nullptr: support by the naive scan
   const int* nbr;
   const int* seed;
-  // walking hull: per vertex two float4, (x, y, z, neighbour count) and its
-  // first eight neighbour ids as 16-bit pairs; nullptr: not available
-  const float4* wrec;
   // walking hull: per vertex a 128-byte block, the walk record followed by
   // the coordinates of its first eight neighbours; nullptr: not available
   const float4* wblk;
@@ -86,7 +83,6 @@ struct ShapeTables {
   const int* nbr;         // adjacency tables (mfcl_upload_convex_faces)
   const int64_t* nbr_off; // per shape, -1: naive scans; nullptr: no faces uploaded
   const int* seed;        // per shape: 8 ints, the walk's axis seed vertices
-  const float4* wrec;     // walk records, two float4 per vertex (indexed like verts)
   const float4* wblk;     // walk blocks, eight float4 per vertex (indexed like verts)
 };
 
@@ -100,7 +96,6 @@ MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
   s.rbound = p.w;
   s.nbr = nullptr;
   s.seed = nullptr;
-  s.wrec = nullptr;
   s.wblk = nullptr;
   if (s.type == kConvex) {
     s.verts = T.verts + T.vert_off[i];
@@ -110,7 +105,6 @@ MFCL_D DevShape loadShape(const ShapeTables& T, uint32_t i) {
       if (o >= 0) {
         s.nbr = T.nbr + o;
         s.seed = T.seed + 8 * static_cast<uint64_t>(i);
-        if (T.wrec && s.nv <= 65535) s.wrec = T.wrec + 2 * static_cast<uint64_t>(T.vert_off[i]);
         if (T.wblk && s.nv <= 65535) s.wblk = T.wblk + 8 * static_cast<uint64_t>(T.vert_off[i]);
       }
     }
@@ -236,24 +230,7 @@ struct Walk {
   float3 w[kWalkChunk];
 };
 
-// With walk records (s.wrec, hull not staged in LDS) a step needs one
-// round trip to device memory: the current vertex's count and first eight
-// neighbour ids are already in registers, the step loads the neighbours'
-// (x, y, z, count) and, after the comparisons, the chosen one's ids from the
-// same 32-byte record (a cache hit).  Vertices of degree above eight read the
-// rest of their neighbour list from the adjacency table.
-MFCL_D void wrecIds(const float4& b, int* id) {
-  const uint32_t u[4] = {__float_as_uint(b.x), __float_as_uint(b.y), __float_as_uint(b.z), __float_as_uint(b.w)};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    id[2 * k] = static_cast<int>(u[k] & 0xFFFFu);
-    id[2 * k + 1] = static_cast<int>(u[k] >> 16);
-  }
-}
-
-// rec: the walk will step with the walk records (s.wrec), else with the
-// adjacency table.
-MFCL_D void walkInit(Walk& W, const DevShape& s, d3 d, bool active, bool rec) {
+MFCL_D void walkInit(Walk& W, const DevShape& s, d3 d, bool active) {
   W.d = d;
   W.keep = active;
   W.visited = 0;
@@ -278,69 +255,12 @@ MFCL_D void walkInit(Walk& W, const DevShape& s, d3 d, bool active, bool rec) {
     }
   }
   W.ext = init;
-  float3 q;
-  if (rec) {
-    const float4 a = s.wrec[2 * init];
-    wrecIds(s.wrec[2 * init + 1], W.id);
-    W.count = __float_as_int(a.w);
-    q = make_float3(a.x, a.y, a.z);
-  } else {
-    q = hullVertRec(s, init, W.start);
-  }
+  const float3 q = hullVertRec(s, init, W.start);
   W.ext_v = d.x * static_cast<double>(q.x) + d.y * static_cast<double>(q.y) + d.z * static_cast<double>(q.z);
   W.parent0 = W.parent1 = init;
   W.visited = 1;
 }
 
-// One step with walk records (see above).
-MFCL_D void walkStepRec(Walk& W, const DevShape& s) {
-  if (!W.keep) return;
-  W.keep = false;
-  const int old = W.ext;
-  const int count = W.count;
-  int next_count = count;
-#pragma unroll
-  for (int j = 0; j < kWalkChunk; ++j) {
-    const int v = j < count ? W.id[j] : old;  // in range for the loads
-    W.id[j] = v;
-    const float4 a = s.wrec[2 * v];
-    W.w[j] = make_float3(a.x, a.y, a.z);
-    W.st[j] = __float_as_int(a.w);
-  }
-  for (int c0 = 0; c0 < count; c0 += kWalkChunk) {
-    if (c0 > 0) {
-      const int start = __float_as_int(s.verts[old].w);
-#pragma unroll
-      for (int j = 0; j < kWalkChunk; ++j) {
-        const int nb = s.nbr[start + 1 + c0 + j];
-        const int v = c0 + j < count ? nb : old;
-        W.id[j] = v;
-        const float4 a = s.wrec[2 * v];
-        W.w[j] = make_float3(a.x, a.y, a.z);
-        W.st[j] = __float_as_int(a.w);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kWalkChunk; ++j) {
-      if (c0 + j >= count || W.id[j] == W.parent0 || W.id[j] == W.parent1) continue;
-      const double v = W.d.x * static_cast<double>(W.w[j].x) + W.d.y * static_cast<double>(W.w[j].y) +
-                       W.d.z * static_cast<double>(W.w[j].z);
-      ++W.visited;
-      if (v > W.ext_v) {
-        W.parent1 = W.ext;
-        W.keep = true;
-        W.ext = W.id[j];
-        W.ext_v = v;
-        next_count = W.st[j];
-      }
-    }
-  }
-  W.parent0 = old;
-  if (W.keep) {
-    W.count = next_count;
-    wrecIds(s.wrec[2 * W.ext + 1], W.id);
-  }
-}
 // first batch: the neighbour record of the current vertex
 MFCL_D void walkFetchRecord(Walk& W, const DevShape& s) {
   if (!W.keep) return;
@@ -518,23 +438,16 @@ MFCL_D int hullWalkBlk(const DevShape& s, d3 d, int* scanned) {
   return W.ext;
 }
 
-// allow_rec = false keeps to the adjacency table (or the walk blocks when
-// there are): the lane kernel's many independent walks are bound by memory
-// traffic rather than latency, and the 32-byte records double the bytes a
-// step touches.
-MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned, bool allow_rec = true) {
-  if (!allow_rec && s.wblk && !s.lv) return hullWalkBlk(s, d, scanned);
+// The walk over blocks when the hull has them and is not staged in LDS,
+// else over the adjacency table.
+MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
+  if (s.wblk && !s.lv) return hullWalkBlk(s, d, scanned);
   Walk W;
-  const bool rec = allow_rec && s.wrec && !s.lv;
-  walkInit(W, s, d, true, rec);
-  if (rec) {
-    while (W.keep) walkStepRec(W, s);
-  } else {
-    while (W.keep) {
-      walkFetchRecord(W, s);
-      walkFetchNeighbours(W, s);
-      walkCompare(W, s);
-    }
+  walkInit(W, s, d, true);
+  while (W.keep) {
+    walkFetchRecord(W, s);
+    walkFetchNeighbours(W, s);
+    walkCompare(W, s);
   }
   *scanned += W.visited;
   return W.ext;
@@ -558,23 +471,15 @@ MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevS
     return;
   }
   Walk A, B;
-  const bool rec = sa.wrec && !sa.lv && sb.wrec && !sb.lv;
-  walkInit(A, sa, da, act_a, rec);
-  walkInit(B, sb, db, act_b, rec);
-  if (rec) {
-    while (A.keep || B.keep) {
-      walkStepRec(A, sa);
-      walkStepRec(B, sb);
-    }
-  } else {
-    while (A.keep || B.keep) {
-      walkFetchRecord(A, sa);
-      walkFetchRecord(B, sb);
-      walkFetchNeighbours(A, sa);
-      walkFetchNeighbours(B, sb);
-      walkCompare(A, sa);
-      walkCompare(B, sb);
-    }
+  walkInit(A, sa, da, act_a);
+  walkInit(B, sb, db, act_b);
+  while (A.keep || B.keep) {
+    walkFetchRecord(A, sa);
+    walkFetchRecord(B, sb);
+    walkFetchNeighbours(A, sa);
+    walkFetchNeighbours(B, sb);
+    walkCompare(A, sa);
+    walkCompare(B, sb);
   }
   if (act_a) ia = A.ext;
   if (act_b) ib = B.ext;
@@ -678,8 +583,6 @@ struct MinkCfg {
   const int* nbr1;
   const int* seed0;
   const int* seed1;
-  const float4* wrec0;
-  const float4* wrec1;
   const float4* wblk0;
   const float4* wblk1;
   int ls0, ls1;
@@ -717,7 +620,6 @@ MFCL_D void writeCfg(MFCL_LDS MinkCfg* c, const DevShape& a, const tf3& ta, cons
   c->lv0 = a.lv; c->lv1 = b.lv;
   c->nbr0 = a.nbr; c->nbr1 = b.nbr;
   c->seed0 = a.seed; c->seed1 = b.seed;
-  c->wrec0 = a.wrec; c->wrec1 = b.wrec;
   c->wblk0 = a.wblk; c->wblk1 = b.wblk;
   c->ls0 = a.ls; c->ls1 = b.ls;
   c->type0 = a.type; c->type1 = b.type;
@@ -741,7 +643,7 @@ struct MinkL {
     s.type = c->type0; s.p0 = c->q0[0]; s.p1 = c->q0[1]; s.p2 = c->q0[2];
     s.verts = c->v0; s.nv = c->nv0; s.rbound = c->rb0;
     s.lv = c->lv0; s.ls = c->ls0;
-    s.nbr = c->nbr0; s.seed = c->seed0; s.wrec = c->wrec0; s.wblk = c->wblk0;
+    s.nbr = c->nbr0; s.seed = c->seed0; s.wblk = c->wblk0;
     return s;
   }
   MFCL_D DevShape sh1() const {
@@ -749,7 +651,7 @@ struct MinkL {
     s.type = c->type1; s.p0 = c->q1[0]; s.p1 = c->q1[1]; s.p2 = c->q1[2];
     s.verts = c->v1; s.nv = c->nv1; s.rbound = c->rb1;
     s.lv = c->lv1; s.ls = c->ls1;
-    s.nbr = c->nbr1; s.seed = c->seed1; s.wrec = c->wrec1; s.wblk = c->wblk1;
+    s.nbr = c->nbr1; s.seed = c->seed1; s.wblk = c->wblk1;
     return s;
   }
   MFCL_D m33 rot1() const { return loadM33(c->r1); }

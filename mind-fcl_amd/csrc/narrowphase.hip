@@ -603,26 +603,15 @@ __global__ void walkSeedKernel(ShapeTables T, uint32_t n, int* seed) {
 
 // Per walking hull (one block each, its threads over the vertices): every
 // vertex's neighbour-record offset goes into the vertex's fourth word (read
-// together with the vertex by the walk) and, for hulls of at most 65535
-// vertices, its walk record: (x, y, z, neighbour count) and the first eight
-// neighbour ids packed as 16-bit pairs.
-__global__ void walkRecKernel(ShapeTables T, float4* verts_rw, float4* wrec) {
+// together with the vertex by the table walk).
+__global__ void walkRecKernel(ShapeTables T, float4* verts_rw) {
   const uint32_t i = blockIdx.x;
   if (T.nbr_off[i] < 0) return;
   const int nv = static_cast<int>(T.vert_cnt[i]);
   const int* tab = T.nbr + T.nbr_off[i];
   for (int k = threadIdx.x; k < nv; k += blockDim.x) {
     const uint64_t g = T.vert_off[i] + static_cast<uint64_t>(k);
-    const float4 q = T.verts[g];
     verts_rw[g].w = __int_as_float(tab[k]);
-    if (nv > 65535) continue;
-    const int* rec = tab + tab[k];
-    const int cnt = rec[0];
-    uint32_t u[4] = {0, 0, 0, 0};
-    for (int j = 0; j < 8 && j < cnt; ++j) u[j >> 1] |= static_cast<uint32_t>(rec[1 + j]) << (16 * (j & 1));
-    wrec[2 * g] = make_float4(q.x, q.y, q.z, __int_as_float(cnt));
-    wrec[2 * g + 1] = make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]), __uint_as_float(u[2]),
-                                  __uint_as_float(u[3]));
   }
 }
 
@@ -667,10 +656,10 @@ __global__ void walkBlkKernel(ShapeTables T, float4* wblk) {
 
 int computeWalkSeeds(mfcl_ctx* ctx) {
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
-                ctx->d_seed, ctx->d_wrec, ctx->d_wblk};
+                ctx->d_seed, ctx->d_wblk};
   const uint32_t n = ctx->n_shapes;
   walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed);
-  walkRecKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_verts, ctx->d_wrec);
+  walkRecKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_verts);
   walkBlkKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_wblk);
   MFCL_HIP_TRY(ctx, hipGetLastError());
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -735,7 +724,7 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   QEntry* queue = static_cast<QEntry*>(ctx->ws_queue.p);
 
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
-                ctx->d_seed, ctx->d_wrec, ctx->d_wblk};
+                ctx->d_seed, ctx->d_wblk};
   PairIO io{pa, pb, posea, poseb, guess, out};
   const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
