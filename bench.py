@@ -9,10 +9,10 @@ intersecting).  Inputs are resident in HBM before the timed region; one
 step = one batched query over all pairs.
 
 Prints one JSON line.  Also reports the roofline of the GJK stage (HBM
-accounting: 16 B per hull vertex record the support searches read plus 4 B
-of neighbour id, 184 B of ids, poses and result per pair) from HIP events on
-the query stream, and the oracle's FP64 multithreaded CPU throughput on a
-bounded sample of the same workload.
+accounting per pair: each FP32 hull vertex read once, 12 (N_a + N_b) B, plus
+2 x 28 B of poses, 2 x 8 B of shape offset and count and a 48 B result)
+from HIP events on the query stream, and the oracle's FP64 multithreaded CPU
+throughput on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -154,12 +154,16 @@ def main():
     value = n * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
     nv = h["vert_cnt"].astype(np.int64)
-    # GJK stage, per launch: every hull vertex record the support searches
-    # read (16 B; counted by the library, EPA's share excluded below), one
-    # 4 B neighbour id per walked vertex, and per pair 8 B of ids + 56 B of
-    # poses read and the 120 B result written
+    # GJK stage, per launch: every FP32 hull vertex of both shapes read once
+    # (12 B each) and per pair 56 B of poses, 16 B of shape offsets and
+    # counts and a 48 B result
+    pair_verts = float(nv[h["pair_a"]].sum() + nv[h["pair_b"]].sum())
+    alg_bytes = 12.0 * pair_verts + 120.0 * n
+    # secondary: the bytes of the vertex records the support searches
+    # actually request (16 B per vertex read, counted by the library, EPA's
+    # share excluded, plus a 4 B neighbour id) and 184 B of pair I/O
     gjk_vertices = float(counters[1]) - float(counters[4])
-    alg_bytes = 20.0 * gjk_vertices + 184.0 * n
+    support_bytes = 20.0 * gjk_vertices + 184.0 * n
     t_gjk = float(kern[:, 1].mean()) * 1e-3
     achieved = alg_bytes / t_gjk / 1e9
     log(f"[gpu] {ms_step:.3f} ms/step; kernels ms (order, gjk, epa) = {kern.mean(0)}; "
@@ -204,6 +208,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(tuned),
             "algorithmic_bytes_per_launch": alg_bytes,
+            "support_read_bytes_per_launch": support_bytes,
             "kernel_ms": {"order": float(kern[:, 0].mean()), "gjk": float(kern[:, 1].mean()),
                           "epa": float(kern[:, 2].mean())},
         },
