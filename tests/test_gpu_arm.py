@@ -135,3 +135,103 @@ def test_heightmap_upload_rejects_bad_shapes(gpu_ctx):
         gpu_ctx.upload_heightmap(np.zeros((6, 6), np.uint16), 0.01, 0.01)
     with pytest.raises(RuntimeError, match="powers of two"):
         gpu_ctx.upload_heightmap(np.zeros((8, 8), np.uint16), 0.0, 0.01)
+
+
+def _scene_with_map(n, heights, res, pose7, seed=6):
+    """arm_scene with a caller-chosen heightmap (any power-of-two halves,
+    resolution and frame) appended as pixel box shapes."""
+    sc = W.arm_scene(n, seed=seed)
+    npix = heights.size
+    sc["hm_heights"], sc["hm_res"], sc["hm_pose7"] = heights, res, np.asarray(pose7, np.float32)
+    sc["hm_box_shape0"] = len(sc["types"])
+    sc["types"] = np.concatenate([sc["types"], np.full(npix, W.BOX, np.int32)])
+    sc["params"] = np.concatenate([sc["params"], W.heightmap_box_params(heights, res, res)])
+    sc["vert_off"] = np.concatenate([sc["vert_off"], np.zeros(npix, np.uint64)])
+    sc["vert_cnt"] = np.concatenate([sc["vert_cnt"], np.zeros(npix, np.uint32)])
+    sc["tri_off"] = np.concatenate([sc["tri_off"], np.zeros(npix, np.int64)])
+    sc["tri_cnt"] = np.concatenate([sc["tri_cnt"], np.zeros(npix, np.int64)])
+    return sc
+
+
+@pytest.mark.gpu
+def test_arm_heightmap_non_square_tilted(gpu_ctx):
+    """A 128 x 512 map of 2 cm pixels (halves 64 and 256: a 1 x 4 top layer,
+    8 layers) with holes (zero-height pixels), its frame tilted 0.2 rad about
+    x: collision flags equal the CPU pipeline's outside the touching band."""
+    rng = np.random.default_rng(11)
+    h = W.fbm_heightmap(rng, half=256)[:, :128].copy()  # rows = y (512), columns = x (128)
+    h[rng.random(h.shape) < 0.05] = 0
+    ang = 0.2
+    pose7 = [0.2, -0.1, -0.5, np.cos(ang / 2), np.sin(ang / 2), 0.0, 0.0]
+    scene = _scene_with_map(1024, h, 0.02, pose7)
+    L, nb = scene["n_links"], scene["n_boxes"]
+    gpu_ctx.upload_shapes(scene["types"], scene["params"], scene["vert_off"], scene["vert_cnt"], scene["verts"],
+                          faces=mfcl.faces_from_triangles(scene["tris"], scene["tri_off"], scene["tri_cnt"]))
+    gpu_ctx.upload_heightmap(h, 0.02, 0.02)
+    mn, mx = W.box_aabbs(scene["box_pose"], scene["box_half"])
+    gpu_ctx.aabbtree_build(mn, mx, np.arange(nb))
+    arm = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb, broadphase_cap=CAP,
+                        heightmap=dict(box_shape0=scene["hm_box_shape0"], pose7=pose7, cap=8192))
+    n = len(scene["q"])
+    dq, dbox = gpu_ctx.to_device(scene["q"]), gpu_ctx.to_device(scene["box_pose"])
+    dany, dmin = gpu_ctx.alloc(4 * n), gpu_ctx.alloc(4 * n)
+    gpu_ctx.arm_query_dev(arm, dq, n, MARGIN, dbox, dany, dmin)
+    any_hit = np.zeros(n, np.int32)
+    dany.download(any_hit)
+    S = A.shape_set(scene)
+    tree = pyoracle.AABBTree(mn, mx, np.arange(nb))
+    pose = A.fk(scene["q"], scene["link_len"])
+    flags, mind, _ = A.pipeline(scene, scene["q"], MARGIN, CAP, S, tree, threads=8, pose=pose)
+    hm_hit, hm_cnt, (k, x, y) = A.hm_collide(scene, S, pose, threads=8)
+    want = (flags & 1).astype(bool) | hm_hit.reshape(n, L).any(1)
+    got = (any_hit & 1).astype(bool)
+    mism = np.nonzero(got != want)[0]
+    touching = []
+    for c in mism:
+        sel = (k // L) == c
+        ok = abs(mind[c]) <= 1e-5
+        if sel.any():
+            pa, pb, qa, qb = A.hm_pairs(scene, k[sel], x[sel], y[sel], pose)
+            ref, _ = S.signed_distance(pa, pb, pyoracle.pose12(qa), pyoracle.pose12(qb), threads=8)
+            ok = ok or bool((np.abs(ref["distance"]) <= 1e-5).any())
+        touching.append(ok)
+    map_only = want & ~(flags & 1).astype(bool)
+    print(f"\n[arm+tilted map] configs={n} colliding={int(want.sum())} map-only={int(map_only.sum())} "
+          f"map candidates cpu={len(k)} mismatches={len(mism)} {mism[:10].tolist()}")
+    assert map_only.sum() > 0
+    assert all(touching)
+    assert int((any_hit & 2).sum()) == 0 and hm_cnt.max() <= 8192
+
+
+@pytest.mark.gpu
+def test_arm_heightmap_rejects_bad_arguments():
+    ctx = mfcl.Context(0)  # a context without a heightmap
+    try:
+        _bad_heightmap_arguments(ctx)
+    finally:
+        ctx.close()
+
+
+def _bad_heightmap_arguments(gpu_ctx):
+    scene = W.arm_scene(8, seed=6, heightmap=True)
+    L, nb = scene["n_links"], scene["n_boxes"]
+    gpu_ctx.upload_shapes(scene["types"], scene["params"], scene["vert_off"], scene["vert_cnt"], scene["verts"])
+    mn, mx = W.box_aabbs(scene["box_pose"], scene["box_half"])
+    gpu_ctx.aabbtree_build(mn, mx, np.arange(nb))
+    dq, dbox = gpu_ctx.to_device(scene["q"]), gpu_ctx.to_device(scene["box_pose"])
+    dany, dmin = gpu_ctx.alloc(32), gpu_ctx.alloc(32)
+    hm = dict(box_shape0=scene["hm_box_shape0"], pose7=scene["hm_pose7"], cap=64)
+    arm = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb, heightmap=hm)
+    with pytest.raises(RuntimeError, match="no heightmap"):
+        gpu_ctx.arm_query_dev(arm, dq, 8, MARGIN, dbox, dany, dmin)
+    gpu_ctx.upload_heightmap(scene["hm_heights"], scene["hm_res"], scene["hm_res"])
+    bad = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb, heightmap=dict(hm, cap=0))
+    with pytest.raises(RuntimeError, match="hm_cap"):
+        gpu_ctx.arm_query_dev(bad, dq, 8, MARGIN, dbox, dany, dmin)
+    bad = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb, heightmap=dict(hm, box_shape0=0))
+    with pytest.raises(RuntimeError, match="must be boxes"):
+        gpu_ctx.arm_query_dev(bad, dq, 8, MARGIN, dbox, dany, dmin)
+    bad = mfcl.make_arm(np.arange(L), scene["link_len"], scene["link_radius"], L, nb,
+                        heightmap=dict(hm, pose7=[0, 0, 0, 0, 0, 0, 0]))
+    with pytest.raises(RuntimeError, match="quaternion"):
+        gpu_ctx.arm_query_dev(bad, dq, 8, MARGIN, dbox, dany, dmin)
