@@ -141,8 +141,15 @@ int uploadConvexFaces(mfcl_ctx* ctx, const int32_t* faces, const uint64_t* face_
   if ((rc = uploadVec(ctx, reinterpret_cast<void**>(&ctx->d_nbr_off), off.data(), off.size() * sizeof(int64_t))))
     return rc;
   if ((rc = uploadVec(ctx, reinterpret_cast<void**>(&ctx->d_seed), nullptr, 8 * sizeof(int32_t) * n))) return rc;
-  if ((rc = uploadVec(ctx, reinterpret_cast<void**>(&ctx->d_wblk), nullptr, 8 * sizeof(float4) * ctx->n_verts_alloc)))
-    return rc;
+  // walk blocks: 128 B per vertex; when the device cannot hold them the
+  // walks read the adjacency table instead (same results, slower)
+  if (ctx->d_wblk) (void)hipFree(ctx->d_wblk);
+  ctx->d_wblk = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&ctx->d_wblk), 8 * sizeof(float4) * std::max<uint64_t>(ctx->n_verts_alloc, 1)) !=
+      hipSuccess) {
+    ctx->d_wblk = nullptr;
+    (void)hipGetLastError();
+  }
   return computeWalkSeeds(ctx);
 }
 

@@ -80,7 +80,7 @@ struct mfcl_ctx {
     int gjk_group{1};           // lanes per pair in the GJK stage (1, 16, 32, 64)
     int epa_threads{131072};    // persistent EPA lanes (multiple of 256): 64 per wavefront
     int guess_policy{1};        // warm start: 0 MLP direction for every pair, 1 not for predicted penetration
-    int epa_lds_faces{64};      // face capacity of the LDS polytope (overflow reruns at full size)
+    int epa_lds_faces{48};      // face capacity of the LDS polytope (overflow reruns at full size)
     int epa_overflow_waves{512};   // persistent wavefronts (one per block) for overflow pairs
     int mesh_blocks_per_cu{4};  // persistent mesh-traversal blocks (4 waves) per CU
     int mesh_stack{16384};      // node-pair stack entries per wavefront

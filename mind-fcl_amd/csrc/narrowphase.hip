@@ -660,7 +660,7 @@ int computeWalkSeeds(mfcl_ctx* ctx) {
   const uint32_t n = ctx->n_shapes;
   walkSeedKernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(T, n, ctx->d_seed);
   walkRecKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_verts);
-  walkBlkKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_wblk);
+  if (ctx->d_wblk) walkBlkKernel<<<n, 64, 0, ctx->stream>>>(T, ctx->d_wblk);
   MFCL_HIP_TRY(ctx, hipGetLastError());
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return MFCL_OK;
