@@ -502,13 +502,17 @@ __global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO
   int scanned = 0;
   // pairs are taken one at a time from a shared counter: EPA iteration
   // counts vary widely (p99 / mean ~ 3), so a static share per wavefront
-  // would leave the grid waiting on its most loaded wavefronts
+  // would leave the grid waiting on its most loaded wavefronts.  They are
+  // taken from the back of the queue: the GJK stage runs the pairs in
+  // ascending cost order (bucketed by hull sizes), so the largest pairs,
+  // which tend to need the most EPA iterations, start first and the tail
+  // shortens.
   for (;;) {
     uint32_t q = 0;
     if (lane == 0) q = atomicAdd(next, 1u);
     q = __shfl(q, 0);
     if (q >= count) break;
-    const QEntry e = queue[q];
+    const QEntry e = queue[count - 1 - q];
     const uint32_t i = e.pair;
     DevShape A, B;
     tf3 ta, tb;
