@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
   MFCL_LDS MinkCfg* cfg = (MFCL_LDS MinkCfg*)(&cfgs[threadIdx.x / G]);
   int scanned = 0;
   if (slot < n) {
-    const uint32_t i = order[slot];
+    const uint32_t i = order[n - 1 - slot];  // costly pairs first, as laneKernel
     mfcl_convex_result r;
     clearResult(r);
     ResAcc acc;
@@ -365,7 +365,9 @@ __global__ void __launch_bounds__(kBlock) laneKernel(ShapeTables T, PairIO io, c
   bool run_gjk = false;
   d3 guess = {1, 0, 0};
   if (slot < n) {
-    i = order[slot];
+    // the bucketed order runs from cheap to costly pairs: the grid takes it
+    // from the costly end so that the largest pairs start first
+    i = order[n - 1 - slot];
     mfcl_convex_result r;
     clearResult(r);
     ResAcc acc;
@@ -502,17 +504,16 @@ __global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO
   int scanned = 0;
   // pairs are taken one at a time from a shared counter: EPA iteration
   // counts vary widely (p99 / mean ~ 3), so a static share per wavefront
-  // would leave the grid waiting on its most loaded wavefronts.  They are
-  // taken from the back of the queue: the GJK stage runs the pairs in
-  // ascending cost order (bucketed by hull sizes), so the largest pairs,
-  // which tend to need the most EPA iterations, start first and the tail
-  // shortens.
+  // would leave the grid waiting on its most loaded wavefronts.  The GJK
+  // stage queues the pairs roughly in descending cost order (bucketed by
+  // hull sizes), so the largest pairs, which tend to need the most EPA
+  // iterations, start first and the tail shortens.
   for (;;) {
     uint32_t q = 0;
     if (lane == 0) q = atomicAdd(next, 1u);
     q = __shfl(q, 0);
     if (q >= count) break;
-    const QEntry e = queue[count - 1 - q];
+    const QEntry e = queue[q];
     const uint32_t i = e.pair;
     DevShape A, B;
     tf3 ta, tb;
