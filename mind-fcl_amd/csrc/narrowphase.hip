@@ -52,14 +52,20 @@ MFCL_D bool closedForm(int ta, int tb) {
   return ta == kBox && tb == kBox;
 }
 
-// 4 buckets per octave of the summed vertex count; closed-form pairs first.
+// Closed-form pairs first (4 buckets per octave of the summed vertex
+// count), then the others by how many of the two shapes walk (0, 1, 2: a
+// wavefront's walks and cooperative scans then rarely mix) and, within that,
+// 3 buckets per octave of the summed vertex count.
 MFCL_D uint8_t pairKey(const ShapeTables& T, const PairIO& io, uint64_t i, const Params& P) {
   const uint32_t a = io.pa[i], b = io.pb[i];
   if (a >= P.n_shapes || b >= P.n_shapes) return 0;
   const bool cf = P.mode == MFCL_MODE_COLLIDE && closedForm(T.type[a], T.type[b]);
   const int cost = shapeCost(T, a) + shapeCost(T, b);
-  const int lg = min(63, static_cast<int>(4.0f * log2f(static_cast<float>(cost))));
-  return static_cast<uint8_t>((cf ? 0 : 64) + lg);
+  if (cf) return static_cast<uint8_t>(min(63, static_cast<int>(4.0f * log2f(static_cast<float>(cost)))));
+  auto walks = [&](uint32_t k) { return T.type[k] == kConvex && T.nbr_off && T.nbr_off[k] >= 0; };
+  const int c = (walks(a) ? 1 : 0) + (walks(b) ? 1 : 0);
+  const int lvl = max(0, min(20, static_cast<int>(3.0f * log2f(static_cast<float>(cost))) - 10));
+  return static_cast<uint8_t>(64 + 21 * c + lvl);
 }
 
 // Block-local histogram in LDS, one global atomic per bin per block.
