@@ -32,11 +32,44 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu_info():
+    """Threads the CPU baselines use: every CPU this process may run on
+    (sched_getaffinity), capped by the cgroup CPU quota when one is set (a
+    quota of q CPUs gives no more throughput to more than q threads).  The
+    host's hardware thread count is reported beside it."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(np.ceil(quota))))
+    return {"threads": threads, "affinity_cpus": aff, "cgroup_cpu_quota": quota,
+            "host_hardware_threads": os.cpu_count()}
+
+
+def timed_median(fn, n_units, target_s=10.0, runs=5):
+    """Warm-up call, then `runs` timed runs of k calls each (k sized so the
+    runs total about target_s); returns the median rate in units/s, k and
+    the total seconds.  fn() returns its own wall seconds."""
+    first = fn()
+    k = max(1, int(np.ceil(target_s / runs / max(first, 1e-6))))
+    rates, total = [], 0.0
+    for _ in range(runs):
+        secs = sum(fn() for _ in range(k))
+        total += secs
+        rates.append(k * n_units / secs)
+    return float(np.median(rates)), k, total
+
+
 def cpu_baseline(h, n_sample, threads, target_cpu_s=10.0):
     """Oracle FP64 shapeSignedDistance over the first n_sample pairs, with
     the same hull faces as the device (vertex-adjacency walk above 32
-    vertices, as fcl::Convex); repeated until ~target_cpu_s of CPU work has
-    been timed."""
+    vertices, as fcl::Convex): one warm-up pass, then the median of 5 timed
+    runs totalling about target_cpu_s of wall time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -47,15 +80,12 @@ def cpu_baseline(h, n_sample, threads, target_cpu_s=10.0):
     ta = pyoracle.pose12(h["pose_a"][:n_sample])
     tb = pyoracle.pose12(h["pose_b"][:n_sample])
     pa, pb = h["pair_a"][:n_sample], h["pair_b"][:n_sample]
-    ref, secs = S.signed_distance(pa, pb, ta, tb, threads=threads)  # warm-up pass
-    reps = max(1, int(np.ceil(target_cpu_s / max(1e-6, secs * threads))))
-    total = 0.0
-    for _ in range(reps):
-        _, s = S.signed_distance(pa, pb, ta, tb, threads=threads)
-        total += s
-    rate = reps * n_sample / total
-    return rate, ref, dict(sample=f"first {n_sample} pairs of the same workload, {reps} timed passes "
-                                  f"({reps * n_sample} pair queries, {total:.2f}s wall on {threads} threads)")
+    ref, _ = S.signed_distance(pa, pb, ta, tb, threads=threads)
+    rate, k, total = timed_median(lambda: S.signed_distance(pa, pb, ta, tb, threads=threads)[1], n_sample,
+                                  target_cpu_s)
+    return rate, ref, dict(sample=f"first {n_sample} pairs of the same workload; median of 5 runs of {k} passes "
+                                  f"each after a warm-up ({5 * k * n_sample} pair queries, {total:.2f}s wall on "
+                                  f"{threads} threads)")
 
 
 def measured_traffic(tuned):
@@ -216,21 +246,24 @@ def main():
     params_path = os.path.join(ROOT, "mind-fcl_amd", "mfcl", "data", "warmstart_mlp.npz")
     if not args.no_warm and os.path.exists(params_path):
         result["warm_start"] = bench_warm(ctx, args, h, dpa, dpb, dqa, dqb, dout, n, out, value, params_path)
+    cpu = host_cpu_info()
+    if args.cpu_threads:
+        cpu["threads"] = args.cpu_threads
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu["threads"]
         rate, ref, info = cpu_baseline(h, min(args.cpu_pairs, n), threads)
         m = min(args.cpu_pairs, n)
         sign_mism = int(np.sum((ref["distance"] < 0) != (out["distance"][:m] < 0)))
         not_exact = int(np.sum((ref["distance"] != out["distance"][:m]) | (ref["ok"] != out["ok"][:m])))
         result["cpu_baseline"] = {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
-                                  "sample": info["sample"], "sample_sign_mismatches_vs_gpu": sign_mism,
+                                  "host": cpu, "sample": info["sample"], "sample_sign_mismatches_vs_gpu": sign_mism,
                                   "sample_pairs_not_bit_identical_to_gpu": not_exact}
         result["speedup_vs_cpu"] = value / rate
     del h
     if not args.no_mesh:
-        result["mesh"] = bench_mesh(ctx, args, threads=args.cpu_threads or min(16, os.cpu_count() or 1))
+        result["mesh"] = bench_mesh(ctx, args, threads=cpu["threads"])
     if not args.no_arm:
-        result["arm"] = bench_arm(ctx, args, threads=args.cpu_threads or min(16, os.cpu_count() or 1))
+        result["arm"] = bench_arm(ctx, args, threads=cpu["threads"])
     ctx.close()
     print(json.dumps(result), flush=True)
 
@@ -311,6 +344,7 @@ def bench_arm(ctx, args, threads, margin=0.05, cap=64):
                         max_pairs=16 * n * L, heightmap=hm)
     dq, dbox = ctx.to_device(scene["q"]), ctx.to_device(scene["box_pose"])
     dany, dmin = ctx.alloc(4 * n), ctx.alloc(4 * n)
+    ctx.reserve_arm(arm, n)
     for _ in range(max(1, args.warmup)):
         npairs = ctx.arm_query_dev(arm, dq, n, margin, dbox, dany, dmin)
     ctx.sync()
@@ -345,17 +379,19 @@ def bench_arm(ctx, args, threads, margin=0.05, cap=64):
             hit, _, _ = arm_ref.hm_collide(sub, S, pose, threads=threads)
             return fl | hit.reshape(m, L).any(1).astype(np.int32), md
 
-        cpu_pass()  # warm-up
-        reps, total = 0, 0.0
-        while total < 5.0 and reps < 50:
+        last = {}
+
+        def timed_pass():
             t0 = time.perf_counter()
-            flags, mind = cpu_pass()
-            total += time.perf_counter() - t0
-            reps += 1
-        rate = reps * m / total
+            last["r"] = cpu_pass()
+            return time.perf_counter() - t0
+
+        rate, k, total = timed_median(timed_pass, m, 10.0)
+        flags, mind = last["r"]
         band = np.abs(mind) <= 1e-5
         res["cpu_baseline"] = {"value": rate, "unit": "configurations/s", "cores": threads, "kind": "port",
-                               "sample": f"first {m} configurations, {reps} timed passes ({total:.2f}s wall)",
+                               "sample": f"first {m} configurations; median of 5 runs of {k} passes each after a "
+                                         f"warm-up ({total:.2f}s wall)",
                                "sample_hit_mismatches_vs_gpu": int(np.sum(((flags & 1) != (any_hit[:m] & 1)) & ~band))}
         res["speedup_vs_cpu"] = value / rate
     log(f"[arm] {json.dumps(res)}")
@@ -411,15 +447,12 @@ def bench_mesh(ctx, args, threads):
                              pool["tri_cnt"], threads=threads)
         m = min(args.mesh_cpu_pairs, n)
         ta, tb = pyoracle.pose12(pairs["pose_a"][:m]), pyoracle.pose12(pairs["pose_b"][:m])
-        ref, secs = M.collide(pairs["pair_a"][:m], pairs["pair_b"][:m], ta, tb, 16, threads=threads)
-        reps = max(1, int(np.ceil(10.0 / max(1e-6, secs * threads))))
-        tot = 0.0
-        for _ in range(reps):
-            _, s = M.collide(pairs["pair_a"][:m], pairs["pair_b"][:m], ta, tb, 16, threads=threads)
-            tot += s
-        rate = reps * m / tot
+        ref, _ = M.collide(pairs["pair_a"][:m], pairs["pair_b"][:m], ta, tb, 16, threads=threads)
+        rate, k, tot = timed_median(lambda: M.collide(pairs["pair_a"][:m], pairs["pair_b"][:m], ta, tb, 16,
+                                                      threads=threads)[1], m, 10.0)
         out["cpu_baseline"] = {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
-                               "sample": f"first {m} pairs, {reps} timed passes ({tot:.2f}s wall)",
+                               "sample": f"first {m} pairs; median of 5 runs of {k} passes each after a warm-up "
+                                         f"({tot:.2f}s wall)",
                                "sample_bool_mismatches_vs_gpu": int(np.sum(ref["hit"] != res["hit"][:m]))}
         out["speedup_vs_cpu"] = out["value"] / rate
     log(f"[mesh] {json.dumps(out)}")

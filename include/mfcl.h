@@ -10,8 +10,11 @@
  *  - every function returns 0 (MFCL_OK) or a negative MFCL_E_* code; the
  *    message of the last failure is available from mfcl_last_error();
  *  - "_dev" entry points take device pointers and enqueue on the given
- *    stream without synchronising; the others take host pointers, copy in,
- *    run, copy out and synchronise;
+ *    stream without synchronising and without allocating: their workspace
+ *    is sized beforehand by mfcl_reserve / mfcl_reserve_arm (or at upload),
+ *    and a query larger than it fails with MFCL_E_STATE; the others take
+ *    host pointers, size their workspace themselves, copy in, run, copy out
+ *    and synchronise;
  *  - a context owns its geometry and workspace; one context per stream;
  *    distinct contexts may be used from distinct threads.
  *  - poses are FP32 (tx, ty, tz, qw, qx, qy, qz); the quaternion is
@@ -91,6 +94,19 @@ typedef struct mfcl_convex_result {
   double pb[3];
 } mfcl_convex_result;
 
+/* One contact of a collide query (mfcl_convex_collide_contacts): the
+ * reference's Contact fields normal, pos and penetration_depth
+ * (include/fcl/narrowphase/contact.h:47-98), world frame, normal from shape
+ * a to shape b.  56 bytes. */
+typedef struct mfcl_contact {
+  double normal[3];
+  double pos[3];
+  double depth;
+} mfcl_contact;
+/* Most contacts one convex pair produces (boxBox2's maxc,
+ * box_box-inl.h:823-841; every other pair type produces one). */
+#define MFCL_MAX_PAIR_CONTACTS 4
+
 /* Mesh query result header, one per pair; contacts go to a separate
  * (n_pairs x max_contacts) array of (b1, b2) triangle index pairs. */
 typedef struct mfcl_mesh_result {
@@ -130,6 +146,22 @@ int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n);
 #define MFCL_TUNE_GUESS_POLICY 6      /* mfcl_mlp_guess_dev: 0 the MLP direction for every pair; 1 (default)
                                          the default guess where the MLP predicts penetration */
 int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value);
+
+/* Workspace sizing (no counterpart in the reference, whose calls allocate
+ * per call).  Grows the context's workspace of the given kind so that later
+ * "_dev" queries of up to `count` items run without allocating; never
+ * shrinks it.  Synchronises the device when it grows.
+ *   MFCL_WS_CONVEX: pairs per mfcl_convex_query_dev /
+ *                   mfcl_convex_collide_contacts_dev call;
+ *   MFCL_WS_MLP:    rows per mfcl_mlp_forward_dev / mfcl_mlp_train_step_dev
+ *                   call (mfcl_mlp_guess_dev needs none).
+ * The mesh traversal's stacks are sized at mfcl_upload_meshes and by
+ * mfcl_tune's mesh knobs; the broadphase query needs no workspace. */
+#define MFCL_WS_CONVEX 1
+#define MFCL_WS_MLP 2
+int mfcl_reserve(mfcl_ctx* ctx, int32_t kind, uint64_t count);
+/* Pairs the convex workspace holds now (0 before any reservation). */
+int mfcl_reserved_pairs(mfcl_ctx* ctx, uint64_t* pairs);
 
 /* ---- device memory helpers (so callers need no HIP runtime) ------------- */
 int mfcl_dev_alloc(mfcl_ctx* ctx, uint64_t bytes, void** dptr);
@@ -172,11 +204,28 @@ int mfcl_upload_convex_faces(mfcl_ctx* ctx, const int32_t* faces, uint64_t faces
 int mfcl_convex_query(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
                       const float* pose_a7, const float* pose_b7, const float* guess3, uint64_t n,
                       const mfcl_request* req, mfcl_convex_result* out);
-/* Same over device pointers, enqueued on stream (hipStream_t or NULL). */
+/* Same over device pointers, enqueued on stream (hipStream_t or NULL).
+ * Needs mfcl_reserve(ctx, MFCL_WS_CONVEX, >= n). */
 int mfcl_convex_query_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
                           const float* pose_a7, const float* pose_b7, const float* guess3,
                           uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
                           void* stream);
+/* fcl::collide with every contact the CollisionResult would receive
+ * (collide mode with penetration only): up to min(max_contacts, 4) per pair
+ * -- boxBox2 yields up to 4 (box_box-inl.h:823-841), the other pair types
+ * one -- truncated to the deepest by the reference's partial sort when they
+ * do not all fit (shape_pair_intersect-inl.h:91-113), in CollisionResult
+ * order.  contacts: n * min(max_contacts, 4) records, pair i's k-th contact
+ * at contacts[i * min(max_contacts, 4) + k] for k < out[i].n_contacts
+ * (other records untouched); out[i] holds contact 0 as in
+ * mfcl_convex_query.  Host pointers. */
+int mfcl_convex_collide_contacts(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                                 const float* pose_a7, const float* pose_b7, const float* guess3, uint64_t n,
+                                 const mfcl_request* req, mfcl_convex_result* out, mfcl_contact* contacts);
+int mfcl_convex_collide_contacts_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                                     const float* pose_a7, const float* pose_b7, const float* guess3,
+                                     uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
+                                     mfcl_contact* contacts, void* stream);
 
 /* ---- OBBRSS mesh-mesh --------------------------------------------------- */
 /* Builds one OBB tree per mesh on the host and uploads node and triangle
@@ -249,10 +298,16 @@ typedef struct mfcl_arm {
  * min_dist = the minimum signed distance over the candidates (+inf when no
  * obstacle is within `margin` of a link's bounding sphere).  n_pairs: the
  * candidate pairs queried (host, may be NULL).  Synchronises the stream once
- * (the pair count sizes the narrowphase launch). */
+ * (the pair count sizes the narrowphase launch).  Needs
+ * mfcl_reserve_arm(ctx, arm, >= n_configs). */
 int mfcl_arm_query_dev(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_configs, float margin,
                        const float* box_pose7, int32_t* any_hit, float* min_dist, uint64_t* n_pairs,
                        void* stream);
+
+/* Sizes the arm query's workspace (link poses, broadphase hits, candidate
+ * pairs and their convex-query workspace) for up to n_configs
+ * configurations of this arm description. */
+int mfcl_reserve_arm(mfcl_ctx* ctx, const mfcl_arm* arm, uint64_t n_configs);
 
 /* Replaces heightmap::LayeredHeightMap (layered_heightmap.h:33-36, built
  * from the bottom layer as layered_heightmap-inl.h:8-51 and :76-101: each

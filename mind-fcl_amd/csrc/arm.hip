@@ -322,6 +322,19 @@ __global__ void armHmReduceKernel(const mfcl_convex_result* res, const uint32_t*
 int aabbQueryLaunch(mfcl_ctx* ctx, const float* qmn, const float* qmx, uint64_t nq, int32_t cap, int32_t* counts,
                     int64_t* ids, hipStream_t s);
 
+// Bytes of the arm query's own workspace for n_cfg configurations (link
+// poses, boxes, hit counts and ids, candidate pairs, their results, the
+// minimum keys and counters; every piece rounded up to 256 bytes) and the
+// candidate pair capacity.
+size_t armWsBytes(const mfcl_arm* arm, uint64_t n_cfg, uint64_t* max_pairs_out) {
+  const uint64_t nq = n_cfg * static_cast<uint64_t>(arm->n_links);
+  const uint64_t max_pairs = arm->max_pairs ? arm->max_pairs : nq * 8;
+  const size_t b_pose = nq * 28, b_box = nq * 24, b_cnt = nq * 4, b_ids = nq * arm->broadphase_cap * 8;
+  const size_t b_pairs = max_pairs * (4 + 4 + 28 + 28 + 4), b_res = max_pairs * sizeof(mfcl_convex_result);
+  if (max_pairs_out) *max_pairs_out = max_pairs;
+  return b_pose + b_box + b_cnt + b_ids + b_pairs + b_res + n_cfg * 4 + 16 * 8 + 16 * 256;
+}
+
 int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_cfg, float margin,
                    const float* box_pose7, int32_t* any_hit, float* min_dist, uint64_t* n_pairs_out, hipStream_t s) {
   if (arm->n_links < 1 || arm->n_links > MFCL_ARM_MAX_LINKS) {
@@ -387,13 +400,19 @@ int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t 
   if (n_cfg == 0) return MFCL_OK;
   const uint64_t nq = n_cfg * static_cast<uint64_t>(arm->n_links);
   const int cap = arm->broadphase_cap;
-  const uint64_t max_pairs = arm->max_pairs ? arm->max_pairs : nq * 8;
-  // workspace: link poses, boxes, hit counts and ids, pairs, results, keys
-  const size_t b_pose = nq * 28, b_box = nq * 24, b_cnt = nq * 4, b_ids = nq * cap * 8;
-  const size_t b_pairs = max_pairs * (4 + 4 + 28 + 28 + 4), b_res = max_pairs * sizeof(mfcl_convex_result);
-  // every piece below is rounded up to 256 bytes: 16 pieces of slack
-  const size_t total = b_pose + b_box + b_cnt + b_ids + b_pairs + b_res + n_cfg * 4 + 16 * 8 + 16 * 256;
-  if (int rc = ensure(ctx, ctx->ws_arm, total)) return rc;
+  uint64_t max_pairs = 0;
+  const size_t total = armWsBytes(arm, n_cfg, &max_pairs);
+  if (total > ctx->ws_arm.bytes || max_pairs > ctx->ws_convex_pairs) {
+    ctx->err = "arm workspace too small for " + std::to_string(n_cfg) +
+               " configurations of this arm: call mfcl_reserve_arm first";
+    return MFCL_E_STATE;
+  }
+  ConvexWs cws;
+  if (!carveConvexWs(ctx->ws_convex.p, ctx->ws_convex.bytes, max_pairs, cws)) {
+    ctx->err = "convex workspace layout exceeds its allocation";
+    return MFCL_E_STATE;
+  }
+  const size_t b_pose = nq * 28, b_cnt = nq * 4, b_ids = nq * cap * 8, b_res = max_pairs * sizeof(mfcl_convex_result);
   char* w = static_cast<char*>(ctx->ws_arm.p);
   auto take = [&](size_t bytes) {
     char* r = w;
@@ -440,7 +459,7 @@ int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t 
     mfcl_request req;
     mfcl_request_defaults(&req, MFCL_MODE_SIGNED_DISTANCE);
     req.gjk_max_iterations = arm->gjk_max_iterations ? arm->gjk_max_iterations : req.gjk_max_iterations;
-    if (int rc = convexQueryLaunch(ctx, pa, pb, qa, qb, nullptr, np, &req, res, s)) return rc;
+    if (int rc = convexQueryLaunch(ctx, pa, pb, qa, qb, nullptr, np, &req, res, nullptr, cws, s)) return rc;
     armReduceKernel<<<static_cast<unsigned>((np + 255) / 256), 256, 0, s>>>(res, pcfg, np, any_hit, min_key);
   }
   if (arm->use_heightmap) {
@@ -475,7 +494,7 @@ int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t 
       req.max_contacts = 1;
       req.penetration = 0;
       req.gjk_max_iterations = arm->gjk_max_iterations ? arm->gjk_max_iterations : req.gjk_max_iterations;
-      if (int rc = convexQueryLaunch(ctx, pa, pb, qa, qb, nullptr, nh, &req, res, s)) return rc;
+      if (int rc = convexQueryLaunch(ctx, pa, pb, qa, qb, nullptr, nh, &req, res, nullptr, cws, s)) return rc;
       armHmReduceKernel<<<static_cast<unsigned>((nh + 255) / 256), 256, 0, s>>>(res, pcfg, nh, any_hit);
     }
   }

@@ -535,6 +535,20 @@ __global__ void zeroU32(uint32_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// The traversal's workspace: 2 KB of counters, then one stack of
+// mesh_stack entries per persistent wavefront (mesh_blocks_per_cu blocks
+// of 4 wavefronts per CU).
+int reserveMesh(mfcl_ctx* ctx) {
+  const uint32_t waves = static_cast<uint32_t>(ctx->n_cu * ctx->tune.mesh_blocks_per_cu * 4);
+  const uint32_t cap = static_cast<uint32_t>(ctx->tune.mesh_stack);
+  if (ctx->ws_arena.p && ctx->mesh_ws_waves == waves && ctx->mesh_ws_cap == cap) return MFCL_OK;
+  ctx->mesh_ws_waves = ctx->mesh_ws_cap = 0;
+  if (int rc = ensure(ctx, ctx->ws_arena, 2048 + static_cast<size_t>(waves) * cap * sizeof(uint64_t))) return rc;
+  ctx->mesh_ws_waves = waves;
+  ctx->mesh_ws_cap = cap;
+  return MFCL_OK;
+}
+
 int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
                       int32_t* contacts, hipStream_t s) {
@@ -543,15 +557,16 @@ int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
     ctx->err = "no meshes uploaded";
     return MFCL_E_STATE;
   }
-  hipDeviceProp_t prop;
-  MFCL_HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
-  const int blocks = prop.multiProcessorCount * ctx->tune.mesh_blocks_per_cu;
-  const int waves = blocks * 4;
-  const uint32_t cap = static_cast<uint32_t>(ctx->tune.mesh_stack);
-  if (int rc = ensure(ctx, ctx->ws_arena, static_cast<size_t>(waves) * cap * sizeof(uint64_t))) return rc;
-  if (int rc = ensure(ctx, ctx->ws_misc, 2048)) return rc;
-  uint32_t* next = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->ws_misc.p) + 1024);
-  unsigned long long* totals = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->ws_misc.p) + 1536);
+  const uint32_t waves = ctx->mesh_ws_waves, cap = ctx->mesh_ws_cap;
+  if (!ctx->ws_arena.p || waves == 0 || cap != static_cast<uint32_t>(ctx->tune.mesh_stack) ||
+      waves != static_cast<uint32_t>(ctx->n_cu * ctx->tune.mesh_blocks_per_cu * 4)) {
+    ctx->err = "mesh traversal stacks not sized for the current knobs";
+    return MFCL_E_STATE;
+  }
+  const int blocks = static_cast<int>(waves / 4);
+  char* misc = static_cast<char*>(ctx->ws_arena.p);
+  uint32_t* next = reinterpret_cast<uint32_t*>(misc + 1024);
+  unsigned long long* totals = reinterpret_cast<unsigned long long*>(misc + 1536);
   MeshArgs A;
   A.nodes = static_cast<const ObbNode*>(ctx->d_nodes);
   A.tris = static_cast<const float4*>(ctx->d_tris);
@@ -567,11 +582,11 @@ int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   A.max_contacts = max_contacts;
   A.out = out;
   A.contacts = contacts;
-  A.stacks = static_cast<uint64_t*>(ctx->ws_arena.p);
+  A.stacks = reinterpret_cast<uint64_t*>(misc + 2048);
   A.stack_cap = cap;
   A.next_query = next;
   A.totals = totals;
-  zeroU32<<<1, 512, 0, s>>>(static_cast<uint32_t*>(ctx->ws_misc.p), 512);
+  zeroU32<<<1, 512, 0, s>>>(reinterpret_cast<uint32_t*>(misc), 512);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
   meshKernel<<<blocks, 256, 0, s>>>(A);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));

@@ -75,15 +75,26 @@ void freeAll(mfcl_ctx* c) {
   freeHeightmap(c);
   freePtr(c->d_bp_nodes);
   freePtr(reinterpret_cast<void*&>(c->d_bp_ids));
-  freePtr(c->ws_order.p);
-  freePtr(c->ws_queue.p);
+  freePtr(c->ws_convex.p);
   freePtr(c->ws_arena.p);
-  freePtr(c->ws_misc.p);
   freePtr(c->ws_io.p);
-  freePtr(c->ws_oflow.p);
-  freePtr(c->ws_lane.p);
   freePtr(c->ws_arm.p);
+  c->ws_convex = c->ws_arena = c->ws_io = c->ws_arm = DevBuf();
+  c->ws_convex_pairs = c->ws_arm_configs = 0;
+  c->mesh_ws_waves = c->mesh_ws_cap = 0;
+  c->d_counters = nullptr;
+  c->d_qcounts = nullptr;
   mlpFree(c);
+}
+
+int reserveConvex(mfcl_ctx* ctx, uint64_t n) {
+  if (n <= ctx->ws_convex_pairs && ctx->ws_convex.p) return MFCL_OK;
+  // the counters of the last query live in this buffer
+  ctx->d_counters = nullptr;
+  ctx->d_qcounts = nullptr;
+  if (int rc = ensure(ctx, ctx->ws_convex, convexWsBytes(n))) return rc;
+  ctx->ws_convex_pairs = convexWsMaxPairs(ctx->ws_convex.bytes);
+  return MFCL_OK;
 }
 
 // Host builders (bvh_build.cpp, broadphase.hip).
@@ -146,6 +157,9 @@ int mfcl_create(mfcl_ctx** out, int device) {
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
     c->max_lds_per_block = lds;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->n_cu = cus;
   *out = c;
   return MFCL_OK;
 }
@@ -183,12 +197,12 @@ int mfcl_last_timing(mfcl_ctx* ctx, float* ms6) {
 }
 
 static int refreshCounters(mfcl_ctx* ctx) {
-  if (ctx->d_counters) {
+  if (ctx->d_counters && ctx->d_qcounts) {
     MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     uint64_t dev[2] = {0, 0};
     uint32_t q[2] = {0, 0};  // EPA queue count, overflow count
     MFCL_HIP_TRY(ctx, hipMemcpy(dev, ctx->d_counters, sizeof(dev), hipMemcpyDeviceToHost));
-    MFCL_HIP_TRY(ctx, hipMemcpy(q, static_cast<char*>(ctx->ws_misc.p) + 1024, sizeof(q), hipMemcpyDeviceToHost));
+    MFCL_HIP_TRY(ctx, hipMemcpy(q, ctx->d_qcounts, sizeof(q), hipMemcpyDeviceToHost));
     ctx->counters[1] = dev[0] + dev[1];
     ctx->counters[2] = q[0];
     ctx->counters[3] = q[1];
@@ -225,7 +239,7 @@ int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
     case MFCL_TUNE_MESH_BLOCKS_PER_CU:
       if (value < 1 || value > 16) break;
       ctx->tune.mesh_blocks_per_cu = value;
-      return MFCL_OK;
+      return ctx->n_meshes ? reserveMesh(ctx) : MFCL_OK;
     case MFCL_TUNE_EPA_LDS_FACES:
       if (value < 16 || value > 128) break;
       ctx->tune.epa_lds_faces = value;
@@ -237,12 +251,36 @@ int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
     case MFCL_TUNE_MESH_STACK:
       if (value < 1024 || value > (1 << 18)) break;
       ctx->tune.mesh_stack = value;
-      return MFCL_OK;
+      return ctx->n_meshes ? reserveMesh(ctx) : MFCL_OK;
     default:
       break;
   }
   ctx->err = "invalid tuning knob/value";
   return MFCL_E_INVALID;
+}
+
+int mfcl_reserve(mfcl_ctx* ctx, int32_t kind, uint64_t count) {
+  if (!ctx) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  switch (kind) {
+    case MFCL_WS_CONVEX:
+      if (count > 0xFFFFFFFFull) {
+        ctx->err = "too many pairs for one query";
+        return MFCL_E_INVALID;
+      }
+      return reserveConvex(ctx, count);
+    case MFCL_WS_MLP:
+      return mlpReserve(ctx, count);
+    default:
+      ctx->err = "unknown workspace kind";
+      return MFCL_E_INVALID;
+  }
+}
+
+int mfcl_reserved_pairs(mfcl_ctx* ctx, uint64_t* pairs) {
+  if (!ctx || !pairs) return MFCL_E_INVALID;
+  *pairs = ctx->ws_convex_pairs;
+  return MFCL_OK;
 }
 
 int mfcl_dev_alloc(mfcl_ctx* ctx, uint64_t bytes, void** dptr) {
@@ -403,37 +441,71 @@ static int finishTiming(mfcl_ctx* ctx, int first_ev, int n_ev, int slot0) {
   return MFCL_OK;
 }
 
-int mfcl_convex_query_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
-                          const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
-                          mfcl_convex_result* out, void* stream) {
+int mfcl_convex_collide_contacts_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                                     const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
+                                     mfcl_convex_result* out, mfcl_contact* contacts, void* stream) {
   if (!ctx || !req) return MFCL_E_INVALID;
   if (n && (!pa || !pb || !posea || !poseb || !out)) {
     ctx->err = "null device pointer";
     return MFCL_E_INVALID;
   }
+  if (n == 0) return MFCL_OK;
+  if (n > ctx->ws_convex_pairs) {
+    ctx->err = "convex workspace holds " + std::to_string(ctx->ws_convex_pairs) + " pairs, query has " +
+               std::to_string(n) + ": call mfcl_reserve(ctx, MFCL_WS_CONVEX, n) first";
+    return MFCL_E_STATE;
+  }
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   ctx->counters[0] = n;
-  const int rc = convexQueryLaunch(ctx, pa, pb, posea, poseb, guess, n, req, out, s);
+  ConvexWs ws;
+  if (!carveConvexWs(ctx->ws_convex.p, ctx->ws_convex.bytes, n, ws)) {
+    ctx->err = "convex workspace layout exceeds its allocation";
+    return MFCL_E_STATE;
+  }
+  const int rc = convexQueryLaunch(ctx, pa, pb, posea, poseb, guess, n, req, out, contacts, ws, s);
   if (rc) return rc;
-  if (ctx->timing && n) return finishTiming(ctx, 0, 4, 0);
+  if (ctx->timing) return finishTiming(ctx, 0, 4, 0);
   return MFCL_OK;
+}
+
+int mfcl_convex_query_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                          const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
+                          mfcl_convex_result* out, void* stream) {
+  return mfcl_convex_collide_contacts_dev(ctx, pa, pb, posea, poseb, guess, n, req, out, nullptr, stream);
 }
 
 int mfcl_convex_query(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
                       mfcl_convex_result* out) {
+  return mfcl_convex_collide_contacts(ctx, pa, pb, posea, poseb, guess, n, req, out, nullptr);
+}
+
+int mfcl_convex_collide_contacts(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                                 const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
+                                 mfcl_convex_result* out, mfcl_contact* contacts) {
   if (!ctx || !req) return MFCL_E_INVALID;
   if (n == 0) return MFCL_OK;
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (n > 0xFFFFFFFFull) {
+    ctx->err = "too many pairs in one query";
+    return MFCL_E_INVALID;
+  }
+  if (contacts && !(req->mode == MFCL_MODE_COLLIDE && req->penetration)) {
+    ctx->err = "a contact array needs a collide request with penetration";
+    return MFCL_E_INVALID;
+  }
+  if (int rc = reserveConvex(ctx, n)) return rc;
   for (uint64_t i = 0; i < n; ++i)
     if (pa[i] >= ctx->n_shapes || pb[i] >= ctx->n_shapes) {
       ctx->err = "pair " + std::to_string(i) + " references a shape id out of range";
       return MFCL_E_INVALID;
     }
+  const int cstride = req->max_contacts < MFCL_MAX_PAIR_CONTACTS ? req->max_contacts : MFCL_MAX_PAIR_CONTACTS;
   const size_t b_ids = 4 * n, b_pose = 28 * n, b_guess = req->use_guess ? 12 * n : 0,
-               b_out = sizeof(mfcl_convex_result) * n;
-  const size_t total = 2 * b_ids + 2 * b_pose + b_guess + b_out + 6 * 256;
+               b_out = sizeof(mfcl_convex_result) * n,
+               b_con = contacts && cstride > 0 ? sizeof(mfcl_contact) * n * cstride : 0;
+  const size_t total = 2 * b_ids + 2 * b_pose + b_guess + b_out + b_con + 7 * 256;
   if (int rc = ensure(ctx, ctx->ws_io, total)) return rc;
   char* base = static_cast<char*>(ctx->ws_io.p);
   auto carve = [&](size_t bytes) {
@@ -447,15 +519,19 @@ int mfcl_convex_query(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   float* dqb = reinterpret_cast<float*>(carve(b_pose));
   float* dg = b_guess ? reinterpret_cast<float*>(carve(b_guess)) : nullptr;
   mfcl_convex_result* dout = reinterpret_cast<mfcl_convex_result*>(carve(b_out));
+  mfcl_contact* dcon = b_con ? reinterpret_cast<mfcl_contact*>(carve(b_con)) : nullptr;
   hipStream_t s = ctx->stream;
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpa, pa, b_ids, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpb, pb, b_ids, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqa, posea, b_pose, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqb, poseb, b_pose, hipMemcpyHostToDevice, s));
   if (dg) MFCL_HIP_TRY(ctx, hipMemcpyAsync(dg, guess, b_guess, hipMemcpyHostToDevice, s));
-  int rc = mfcl_convex_query_dev(ctx, dpa, dpb, dqa, dqb, dg, n, req, dout, s);
+  // the device pass writes only the contacts each pair keeps: start from the caller's records
+  if (dcon) MFCL_HIP_TRY(ctx, hipMemcpyAsync(dcon, contacts, b_con, hipMemcpyHostToDevice, s));
+  int rc = mfcl_convex_collide_contacts_dev(ctx, dpa, dpb, dqa, dqb, dg, n, req, dout, dcon, s);
   if (rc) return rc;
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(out, dout, b_out, hipMemcpyDeviceToHost, s));
+  if (dcon) MFCL_HIP_TRY(ctx, hipMemcpyAsync(contacts, dcon, b_con, hipMemcpyDeviceToHost, s));
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
   return MFCL_OK;
 }
@@ -468,7 +544,8 @@ int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* ve
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   freeMeshes(ctx);
-  return buildAndUploadMeshes(ctx, verts_xyz, vert_off, vert_cnt, tris, tri_off, tri_cnt, n_meshes);
+  if (int rc = buildAndUploadMeshes(ctx, verts_xyz, vert_off, vert_cnt, tris, tri_off, tri_cnt, n_meshes)) return rc;
+  return reserveMesh(ctx);
 }
 
 int mfcl_mesh_num_nodes(mfcl_ctx* ctx, uint32_t mesh, uint32_t* n_nodes) {
@@ -714,6 +791,24 @@ int mfcl_arm_query_dev(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint6
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   if (n_pairs) *n_pairs = 0;
   return armQueryLaunch(ctx, arm, q, n_configs, margin, box_pose7, any_hit, min_dist, n_pairs, s);
+}
+
+int mfcl_reserve_arm(mfcl_ctx* ctx, const mfcl_arm* arm, uint64_t n_configs) {
+  if (!ctx || !arm) return MFCL_E_INVALID;
+  if (arm->n_links < 1 || arm->n_links > MFCL_ARM_MAX_LINKS || arm->broadphase_cap < 1 ||
+      arm->broadphase_cap > 512) {
+    ctx->err = "n_links or broadphase_cap out of range";
+    return MFCL_E_INVALID;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  uint64_t max_pairs = 0;
+  const size_t bytes = armWsBytes(arm, n_configs ? n_configs : 1, &max_pairs);
+  if (max_pairs > 0xFFFFFFFFull) {
+    ctx->err = "candidate pair capacity too large for one query";
+    return MFCL_E_INVALID;
+  }
+  if (int rc = ensure(ctx, ctx->ws_arm, bytes)) return rc;
+  return reserveConvex(ctx, max_pairs);
 }
 
 }  // extern "C"

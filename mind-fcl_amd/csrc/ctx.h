@@ -75,6 +75,7 @@ struct HeightmapHost {
 struct mfcl_ctx {
   int device{0};
   int max_lds_per_block{65536};  // device limit on dynamic + static LDS per workgroup
+  int n_cu{256};                 // compute units
   // launch-shape knobs (set by mfcl_tune / the autotuner)
   struct {
     int gjk_group{1};           // lanes per pair in the GJK stage (1, 16, 32, 64)
@@ -126,8 +127,15 @@ struct mfcl_ctx {
   mfcl::HeightmapHost hm;
   uint64_t arm_hm_pairs{0};  // heightmap candidate pairs of the last arm query
 
-  // growable device workspaces, keyed by purpose
-  mfcl::DevBuf ws_order, ws_queue, ws_arena, ws_misc, ws_io, ws_oflow, ws_lane, ws_arm;
+  // device workspaces, sized by mfcl_reserve / mfcl_reserve_arm / the mesh
+  // upload (query entry points never grow them) and, for the host-pointer
+  // wrappers, ws_io
+  mfcl::DevBuf ws_convex, ws_arena, ws_io, ws_arm;
+  uint64_t ws_convex_pairs{0};  // pairs ws_convex holds
+  uint64_t ws_arm_configs{0};   // configurations ws_arm holds (for ws_arm_key's arm)
+  uint64_t ws_arm_key[4]{};     // links, broadphase cap, max pairs per config, pair capacity
+  uint32_t mesh_ws_waves{0}, mesh_ws_cap{0};  // stacks ws_arena holds
+  uint32_t* d_qcounts{nullptr};  // EPA queue / overflow counts of the last convex query
 
   mfcl::MlpState mlp;
 
@@ -153,13 +161,31 @@ int uploadConvexFaces(mfcl_ctx* ctx, const int32_t* faces, const uint64_t* face_
                       uint64_t faces_len);
 int computeWalkSeeds(mfcl_ctx* ctx);
 // arm.hip
+size_t armWsBytes(const mfcl_arm* arm, uint64_t n_cfg, uint64_t* max_pairs_out);
 int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_cfg, float margin,
                    const float* box_pose7, int32_t* any_hit, float* min_dist, uint64_t* n_pairs_out, hipStream_t s);
+
+// Convex-query workspace (narrowphase.hip): pieces carved from one buffer.
+struct ConvexWs {
+  uint32_t* misc;      // 2 KB of counters
+  uint32_t* order;     // bucketed pair order
+  uint8_t* key;        // bucket keys
+  void* queue;         // EPA queue
+  void* oflow;         // EPA overflow queue
+  double* lane_dirs;   // lane kernel direction slots
+};
+size_t convexWsBytes(uint64_t n);
+uint64_t convexWsMaxPairs(size_t bytes);
+bool carveConvexWs(void* base, size_t bytes, uint64_t n, ConvexWs& w);
+// Grows ctx->ws_convex to hold n pairs (host call; may synchronise).
+int reserveConvex(mfcl_ctx* ctx, uint64_t n);
+// Mesh traversal stacks for the current tune knobs (bvh.hip; host call).
+int reserveMesh(mfcl_ctx* ctx);
 
 // Launchers (narrowphase.hip, bvh.hip, broadphase.hip).
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
-                      mfcl_convex_result* out, hipStream_t s);
+                      mfcl_convex_result* out, mfcl_contact* contacts, const ConvexWs& ws, hipStream_t s);
 int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
                       int32_t* contacts, hipStream_t s);
@@ -175,6 +201,7 @@ int mlpFeatures(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const flo
 int mlpGuess(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
              uint64_t n, float* guess3, float* dist, hipStream_t s);
 int mlpForwardFeat(mfcl_ctx* ctx, const float* xf, uint64_t n, float* y4_out, hipStream_t s);
+int mlpReserve(mfcl_ctx* ctx, uint64_t rows);
 int mlpTrainStep(mfcl_ctx* ctx, const float* xf, const float* target4, uint64_t n, const float* hyper,
                  int apply_update, float* loss_out, hipStream_t s);
 void mlpFree(mfcl_ctx* ctx);

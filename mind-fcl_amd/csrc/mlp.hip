@@ -458,12 +458,22 @@ struct Acts {
 
 // Carves the activation workspace for n_pad rows (training needs all of it,
 // inference only x, h1..h3 and y).
-int carveActs(mfcl_ctx* ctx, uint64_t np, bool train, Acts& a) {
+size_t actsBytes(uint64_t np, bool train) {
   const size_t bf_in = np * kMlpInPad * 2, bf_h = np * kHid * 2, bf_o = np * kMlpOutPad * 2;
   const size_t f_h = np * kHid * 4, f_o = np * kMlpOutPad * 4, f_in = np * kMlpInPad * 4;
   size_t total = bf_in + 3 * bf_h + f_o + 256 * 8;
-  if (train) total += bf_in + 3 * bf_h + 2 * bf_o + 6 * bf_h + 3 * f_h + f_in;
-  if (int rc = ensure(ctx, ctx->mlp.ws, total)) return rc;
+  if (train) total += bf_in + 3 * bf_h + 2 * bf_o + 6 * bf_h + 3 * f_h + f_in + 256 * 16;
+  return total;
+}
+
+int carveActs(mfcl_ctx* ctx, uint64_t np, bool train, Acts& a) {
+  const size_t bf_in = np * kMlpInPad * 2, bf_h = np * kHid * 2, bf_o = np * kMlpOutPad * 2;
+  const size_t f_h = np * kHid * 4, f_o = np * kMlpOutPad * 4, f_in = np * kMlpInPad * 4;
+  if (actsBytes(np, train) > ctx->mlp.ws.bytes) {
+    ctx->err = "MLP workspace too small for " + std::to_string(np) +
+               " rows: call mfcl_reserve(ctx, MFCL_WS_MLP, rows) first";
+    return MFCL_E_STATE;
+  }
   char* p = static_cast<char*>(ctx->mlp.ws.p);
   auto take = [&](size_t b) {
     char* q = p;
@@ -563,6 +573,14 @@ __global__ void featCastKernel(const float* xf, uint64_t n, uint64_t np, __bf16*
 }
 
 }  // namespace
+
+int mlpReserve(mfcl_ctx* ctx, uint64_t rows) {
+  if (rows > (1ull << 26)) {
+    ctx->err = "MLP rows out of range (<= 2^26)";
+    return MFCL_E_INVALID;
+  }
+  return ensure(ctx, ctx->mlp.ws, actsBytes(pad32(rows ? rows : 1), true));
+}
 
 int mlpInit(mfcl_ctx* ctx, uint64_t seed) {
   std::vector<float> p(kMlpParams, 0.f);

@@ -28,6 +28,8 @@ struct PairIO {
   const float* poseb;
   const float* guess;
   mfcl_convex_result* out;
+  mfcl_contact* contacts;  // optional: cstride records per pair
+  int cstride;             // min(max_contacts, 4) when contacts is set
 };
 
 struct QEntry {
@@ -172,6 +174,36 @@ MFCL_D bool runClosedForm(const DevShape& A, const tf3& ta, const DevShape& B, c
   return hit;
 }
 
+MFCL_D void storeContact(mfcl_contact* dst, const d3& n, const d3& p, double depth) {
+  mfcl_contact c;
+  c.normal[0] = n.x; c.normal[1] = n.y; c.normal[2] = n.z;
+  c.pos[0] = p.x; c.pos[1] = p.y; c.pos[2] = p.z;
+  c.depth = depth;
+  *dst = c;
+}
+
+// Contacts of a colliding closed-form pair into the result: the deepest
+// max_contacts when they do not all fit (shape_pair_intersect-inl.h:91-113),
+// contact 0 into the record, all kept ones into the contact array.
+MFCL_D void finishClosedForm(PContacts& pc, const Params& P, const PairIO& io, uint32_t i, mfcl_convex_result& r,
+                             ResAcc& acc, bool writer) {
+  int kept = pc.n;
+  if (pc.n > P.max_contacts) {
+    partialSortDeepest(pc.c, P.max_contacts, pc.n);
+    kept = P.max_contacts;
+  }
+  r.n_contacts = kept;
+  if (kept > 0) {
+    r.depth = pc.c[0].depth;
+    r.distance = -r.depth;
+    acc.normal = pc.c[0].n;
+    acc.pos = pc.c[0].p;
+  }
+  if (io.contacts && writer)
+    for (int k = 0; k < kept && k < io.cstride; ++k)
+      storeContact(io.contacts + static_cast<uint64_t>(i) * io.cstride + k, pc.c[k].n, pc.c[k].p, pc.c[k].depth);
+}
+
 MFCL_D void loadPair(const ShapeTables& T, const double4* interior, const PairIO& io, uint32_t i,
                      DevShape& A, DevShape& B, tf3& ta, tf3& tb, uint32_t& ia, uint32_t& ib) {
   ia = io.pa[i];
@@ -250,19 +282,7 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
             if constexpr (MODE == kModeBool) {
               r.n_contacts = 1;
             } else {
-              // shape_pair_intersect-inl.h:91-113: deepest contacts first when truncated
-              int k0 = 0;
-              if (pc.n > P.max_contacts) {
-                for (int k = 1; k < pc.n; ++k)
-                  if (pc.c[k].depth > pc.c[k0].depth) k0 = k;
-              }
-              r.n_contacts = pc.n < P.max_contacts ? pc.n : P.max_contacts;
-              if (pc.n > 0) {
-                r.depth = pc.c[k0].depth;
-                r.distance = -r.depth;
-                acc.normal = pc.c[k0].n;
-                acc.pos = pc.c[k0].p;
-              }
+              finishClosedForm(pc, P, io, i, r, acc, lg == 0);
             }
           }
         } else {
@@ -393,21 +413,7 @@ __global__ void __launch_bounds__(kBlock) laneKernel(ShapeTables T, PairIO io, c
         const bool hit = runClosedForm(loadShape(T, ia), ta, loadShape(T, ib), tb, &pc);
         r.hit = hit;
         r.ok = 1;
-        if (hit) {
-          // shape_pair_intersect-inl.h:91-113: deepest contacts first when truncated
-          int k0 = 0;
-          if (pc.n > P.max_contacts) {
-            for (int k = 1; k < pc.n; ++k)
-              if (pc.c[k].depth > pc.c[k0].depth) k0 = k;
-          }
-          r.n_contacts = pc.n < P.max_contacts ? pc.n : P.max_contacts;
-          if (pc.n > 0) {
-            r.depth = pc.c[k0].depth;
-            r.distance = -r.depth;
-            acc.normal = pc.c[k0].n;
-            acc.pos = pc.c[k0].p;
-          }
-        }
+        if (hit) finishClosedForm(pc, P, io, i, r, acc, true);
       } else {
         md.ia = ia;
         md.ib = ib;
@@ -559,6 +565,7 @@ __global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO
       const d3 nw = mul(ta.R, n1), pw = apply(ta, pt);
       r.normal[0] = nw.x; r.normal[1] = nw.y; r.normal[2] = nw.z;
       r.pos[0] = pw.x; r.pos[1] = pw.y; r.pos[2] = pw.z;
+      if (io.contacts) storeContact(io.contacts + static_cast<uint64_t>(i) * io.cstride, nw, pw, eo.depth);
     } else {
       const d3 wa = apply(ta, eo.p0), wb = apply(ta, eo.p1);
       r.pa[0] = wa.x; r.pa[1] = wa.y; r.pa[2] = wa.z;
@@ -677,9 +684,53 @@ int computeWalkSeeds(mfcl_ctx* ctx) {
   return MFCL_OK;
 }
 
+// Workspace of one convex query over n pairs, carved from one device buffer
+// (256-byte aligned pieces): the counters block (2 KB: bucket histogram and
+// offsets, EPA queue and overflow counts at byte 1024, EPA work cursors at
+// 1032 / 1036, 64-bit work counters at 1536), the bucket order (4n B) and
+// keys (n B), the EPA queue and overflow queue (one QEntry per pair each,
+// the most that can be queued) and the lane kernel's direction slots
+// (kLaneDirDoubles doubles per wavefront of its grid).
+static size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+static size_t laneDirBytes(uint64_t n) {
+  const uint64_t waves = (n + kBlock - 1) / kBlock * (kBlock / 64);
+  return static_cast<size_t>(waves) * kLaneDirDoubles * sizeof(double);
+}
+size_t convexWsBytes(uint64_t n) {
+  if (n == 0) n = 1;
+  return align256(2048) + align256(4 * n) + align256(n) + 2 * align256(n * sizeof(QEntry)) +
+         align256(laneDirBytes(n));
+}
+uint64_t convexWsMaxPairs(size_t bytes) {
+  // convexWsBytes grows by at least sizeof(QEntry) per pair: bisect
+  uint64_t lo = 0, hi = bytes / (2 * sizeof(QEntry)) + 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) / 2;
+    if (convexWsBytes(mid) <= bytes) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+bool carveConvexWs(void* base, size_t bytes, uint64_t n, ConvexWs& w) {
+  if (!base || convexWsBytes(n) > bytes) return false;
+  char* p = static_cast<char*>(base);
+  auto take = [&](size_t b) {
+    char* r = p;
+    p += align256(b);
+    return r;
+  };
+  w.misc = reinterpret_cast<uint32_t*>(take(2048));
+  w.order = reinterpret_cast<uint32_t*>(take(4 * (n ? n : 1)));
+  w.key = reinterpret_cast<uint8_t*>(take(n ? n : 1));
+  w.queue = take((n ? n : 1) * sizeof(QEntry));
+  w.oflow = take((n ? n : 1) * sizeof(QEntry));
+  w.lane_dirs = reinterpret_cast<double*>(take(laneDirBytes(n ? n : 1)));
+  return true;
+}
+
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
-                      mfcl_convex_result* out, hipStream_t s) {
+                      mfcl_convex_result* out, mfcl_contact* contacts, const ConvexWs& ws, hipStream_t s) {
   if (n == 0) return MFCL_OK;
   if (ctx->n_shapes == 0) {
     ctx->err = "no shapes uploaded";
@@ -709,34 +760,31 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   P.use_guess = req->use_guess;
   P.n_shapes = ctx->n_shapes;
 
-  // workspace: hist + offs (2 x 128 x 4 B), EPA queue count at byte 1024,
-  // overflow count at 1028, EPA work cursors at 1032 / 1036, work counters
-  // at byte 1536 (all cleared by zeroKernel); keys (n B) after the order array (4n B); EPA queue; EPA arenas
-  const size_t misc_bytes = 2048;
-  if (int rc = ensure(ctx, ctx->ws_misc, misc_bytes)) return rc;
-  if (int rc = ensure(ctx, ctx->ws_order, n * (sizeof(uint32_t) + 1) + 256)) return rc;
-  if (int rc = ensure(ctx, ctx->ws_queue, n * sizeof(QEntry))) return rc;
-  if (int rc = ensure(ctx, ctx->ws_oflow, n * sizeof(QEntry))) return rc;
+  if (contacts && !(P.mode == MFCL_MODE_COLLIDE && P.penetration)) {
+    ctx->err = "a contact array needs a collide request with penetration";
+    return MFCL_E_INVALID;
+  }
   const size_t full_slice = epaSliceBytes(req->epa_max_faces);
   if (full_slice > static_cast<size_t>(ctx->max_lds_per_block)) {
     ctx->err = "epa_max_faces = " + std::to_string(req->epa_max_faces) + " needs " + std::to_string(full_slice) +
                " bytes of LDS per wavefront; this device allows " + std::to_string(ctx->max_lds_per_block);
     return MFCL_E_UNSUPPORTED;
   }
-  uint32_t* hist = static_cast<uint32_t*>(ctx->ws_misc.p);
+  uint32_t* hist = ws.misc;
   uint32_t* offs = hist + kBuckets;
   uint32_t* qcount = offs + kBuckets;
   uint32_t* ocount = qcount + 1;
-  QEntry* oflow = static_cast<QEntry*>(ctx->ws_oflow.p);
-  unsigned long long* sc = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->ws_misc.p) + 1536);
+  QEntry* oflow = static_cast<QEntry*>(ws.oflow);
+  unsigned long long* sc = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(ws.misc) + 1536);
   ctx->d_counters = reinterpret_cast<uint64_t*>(sc);
-  uint32_t* order = static_cast<uint32_t*>(ctx->ws_order.p);
-  uint8_t* key = reinterpret_cast<uint8_t*>(order + n);
-  QEntry* queue = static_cast<QEntry*>(ctx->ws_queue.p);
+  ctx->d_qcounts = qcount;
+  uint32_t* order = ws.order;
+  uint8_t* key = ws.key;
+  QEntry* queue = static_cast<QEntry*>(ws.queue);
 
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
                 ctx->d_seed, ctx->d_wblk};
-  PairIO io{pa, pb, posea, poseb, guess, out};
+  PairIO io{pa, pb, posea, poseb, guess, out, contacts, contacts ? (P.max_contacts < 4 ? P.max_contacts : 4) : 0};
   const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
   zeroKernel<<<1, 512, 0, s>>>(hist, 512);
@@ -757,9 +805,7 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   else MFCL_NARROW(GG, kModeBool)
     if (G == 1) {
       const unsigned lb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-      if (int rc = ensure(ctx, ctx->ws_lane, static_cast<size_t>(lb) * (kBlock / 64) * kLaneDirDoubles * sizeof(double)))
-        return rc;
-      double* dws = static_cast<double*>(ctx->ws_lane.p);
+      double* dws = ws.lane_dirs;
       if (mode == kModeSd)
         laneKernel<kModeSd><<<lb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws);
       else

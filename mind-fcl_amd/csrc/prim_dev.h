@@ -19,6 +19,58 @@ struct PContacts {
 
 constexpr double kDblEps = 2.220446049250313e-16;
 
+// std::partial_sort(c, c + k, c + n, deeper-first) as libstdc++ performs it
+// (heap select over the first k, then a heap sort of them), which is what
+// the reference's truncation to the free contact slots runs
+// (shape_pair_intersect-inl.h:91-113): equal depths keep the same relative
+// order as there.  n <= 4.
+MFCL_D bool deeperThan(const PContact& a, const PContact& b) { return a.depth > b.depth; }
+MFCL_D void heapPush(PContact* f, int hole, int top, const PContact& v) {
+  int parent = (hole - 1) / 2;
+  while (hole > top && deeperThan(f[parent], v)) {
+    f[hole] = f[parent];
+    hole = parent;
+    parent = (hole - 1) / 2;
+  }
+  f[hole] = v;
+}
+MFCL_D void heapAdjust(PContact* f, int hole, int len, const PContact& v) {
+  const int top = hole;
+  int child = hole;
+  while (child < (len - 1) / 2) {
+    child = 2 * (child + 1);
+    if (deeperThan(f[child], f[child - 1])) --child;
+    f[hole] = f[child];
+    hole = child;
+  }
+  if ((len & 1) == 0 && child == (len - 2) / 2) {
+    child = 2 * (child + 1);
+    f[hole] = f[child - 1];
+    hole = child - 1;
+  }
+  heapPush(f, hole, top, v);
+}
+MFCL_D void heapPop(PContact* f, int len, int result) {
+  const PContact v = f[result];
+  f[result] = f[0];
+  heapAdjust(f, 0, len, v);
+}
+MFCL_D void partialSortDeepest(PContact* f, int k, int n) {
+  if (k >= 2) {
+    for (int parent = (k - 2) / 2;; --parent) {
+      const PContact v = f[parent];
+      heapAdjust(f, parent, k, v);
+      if (parent == 0) break;
+    }
+  }
+  for (int i = k; i < n; ++i)
+    if (deeperThan(f[i], f[0])) heapPop(f, k, i);
+  for (int len = k; len > 1;) {
+    --len;
+    heapPop(f, len, len);
+  }
+}
+
 // sphere_sphere-inl.h:45-68
 MFCL_D bool primSphereSphere(double r1, const tf3& t1, double r2, const tf3& t2, PContacts* out) {
   const d3 diff = t2.t - t1.t;

@@ -22,6 +22,9 @@ MFCL_SHAPE_CYLINDER = 5
 MFCL_SHAPE_CONVEX = 100
 MODE_COLLIDE = 0
 MODE_SIGNED_DISTANCE = 1
+WS_CONVEX = 1
+WS_MLP = 2
+MAX_PAIR_CONTACTS = 4
 
 
 class Request(C.Structure):
@@ -102,6 +105,9 @@ CONVEX_RESULT_DTYPE = np.dtype(
 )
 assert CONVEX_RESULT_DTYPE.itemsize == 128
 
+CONTACT_DTYPE = np.dtype([("normal", np.float64, 3), ("pos", np.float64, 3), ("depth", np.float64)])
+assert CONTACT_DTYPE.itemsize == 56
+
 MESH_RESULT_DTYPE = np.dtype(
     [("hit", np.int32), ("n_contacts", np.int32), ("bv_tests", np.int32), ("leaf_tests", np.int32)]
 )
@@ -142,6 +148,11 @@ def load_library() -> C.CDLL:
     lib.mfcl_arm_last_heightmap_pairs.argtypes = [p, C.POINTER(u64)]
     lib.mfcl_convex_query.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p]
     lib.mfcl_convex_query_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
+    lib.mfcl_convex_collide_contacts.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
+    lib.mfcl_convex_collide_contacts_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p, p]
+    lib.mfcl_reserve.argtypes = [p, i32, u64]
+    lib.mfcl_reserved_pairs.argtypes = [p, C.POINTER(u64)]
+    lib.mfcl_reserve_arm.argtypes = [p, C.POINTER(Arm), u64]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
     lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
     lib.mfcl_mesh_collide.argtypes = [p, p, p, p, p, u64, i32, p, p]
@@ -217,6 +228,7 @@ class DeviceArray:
 class Context:
     def __init__(self, device: int = 0):
         self.lib = load_library()
+        self._reserved = {WS_CONVEX: 0, WS_MLP: 0}
         self.h = C.c_void_p()
         rc = self.lib.mfcl_create(C.byref(self.h), device)
         if rc != 0:
@@ -254,6 +266,22 @@ class Context:
         c = np.zeros(5, np.uint64)
         self._check(self.lib.mfcl_last_counters_n(self.h, _ptr(c), 5))
         return c
+
+    def reserve(self, kind: int, count: int):
+        """mfcl_reserve: size the workspace of the "_dev" queries of that kind
+        for `count` items (the "_dev" wrappers below call it when a query
+        is larger than what was reserved)."""
+        if count > self._reserved[kind]:
+            self._check(self.lib.mfcl_reserve(self.h, kind, int(count)))
+            self._reserved[kind] = int(count)
+
+    def reserved_pairs(self) -> int:
+        n = C.c_uint64(0)
+        self._check(self.lib.mfcl_reserved_pairs(self.h, C.byref(n)))
+        return int(n.value)
+
+    def reserve_arm(self, arm: "Arm", n_configs: int):
+        self._check(self.lib.mfcl_reserve_arm(self.h, C.byref(arm), int(n_configs)))
 
     def alloc(self, nbytes: int) -> DeviceArray:
         return DeviceArray(self, nbytes)
@@ -309,8 +337,38 @@ class Context:
         )
         return out
 
+    def convex_collide_contacts(self, pair_a, pair_b, pose_a, pose_b, req: Request):
+        """Every kept contact per pair (collide mode with penetration):
+        returns (results, contacts[n, min(max_contacts, 4)] of CONTACT_DTYPE;
+        records past results['n_contacts'] are zero)."""
+        pair_a = np.ascontiguousarray(pair_a, np.uint32)
+        pair_b = np.ascontiguousarray(pair_b, np.uint32)
+        pose_a = np.ascontiguousarray(pose_a, np.float32).reshape(-1, 7)
+        pose_b = np.ascontiguousarray(pose_b, np.float32).reshape(-1, 7)
+        n = len(pair_a)
+        out = np.zeros(n, CONVEX_RESULT_DTYPE)
+        cont = np.zeros((n, max(1, min(req.max_contacts, MAX_PAIR_CONTACTS))), CONTACT_DTYPE)
+        self._check(
+            self.lib.mfcl_convex_collide_contacts(
+                self.h, _ptr(pair_a), _ptr(pair_b), _ptr(pose_a), _ptr(pose_b), None, n, C.byref(req), _ptr(out),
+                _ptr(cont)
+            )
+        )
+        return out, cont
+
+    def convex_collide_contacts_dev(self, pa, pb, qa, qb, n: int, req: Request, out: DeviceArray,
+                                    contacts: DeviceArray, guess: Optional[DeviceArray] = None):
+        self.reserve(WS_CONVEX, n)
+        self._check(
+            self.lib.mfcl_convex_collide_contacts_dev(
+                self.h, pa.ptr, pb.ptr, qa.ptr, qb.ptr, None if guess is None else guess.ptr, n, C.byref(req),
+                out.ptr, contacts.ptr, None
+            )
+        )
+
     def convex_query_dev(self, pa: DeviceArray, pb: DeviceArray, qa: DeviceArray, qb: DeviceArray, n: int,
                          req: Request, out: DeviceArray, guess: Optional[DeviceArray] = None):
+        self.reserve(WS_CONVEX, n)
         self._check(
             self.lib.mfcl_convex_query_dev(
                 self.h, pa.ptr, pb.ptr, qa.ptr, qb.ptr, None if guess is None else guess.ptr, n, C.byref(req),
@@ -419,10 +477,12 @@ class Context:
                                                 None if dist is None else dist.ptr, None))
 
     def mlp_forward_dev(self, feat: DeviceArray, n: int, out4: DeviceArray):
+        self.reserve(WS_MLP, n)
         self._check(self.lib.mfcl_mlp_forward_dev(self.h, feat.ptr, n, out4.ptr, None))
 
     def mlp_train_step_dev(self, feat: DeviceArray, target: DeviceArray, n: int, hyper=None,
                            apply_update: bool = True) -> float:
+        self.reserve(WS_MLP, n)
         loss = np.zeros(1, np.float32)
         hp = None if hyper is None else np.ascontiguousarray(hyper, np.float32)
         self._check(self.lib.mfcl_mlp_train_step_dev(self.h, feat.ptr, target.ptr, n, _ptr(hp),

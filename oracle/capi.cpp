@@ -137,10 +137,12 @@ void orc_support(void* hs, const int* type, const double* params, const int* hul
 }
 
 // fcl::collide over a batch of pairs.  Returns wall seconds of the query loop.
-double orc_collide_batch(void* hs, const int* type, const double* params, const int* hull_idx,
-                         const int* pa, const int* pb, const double* tfa12, const double* tfb12,
-                         int64_t n, int max_contacts, int penetration, const double* sp6,
-                         const double* guess3, int n_threads, OrcCollideRec* out) {
+// contacts7 (may be NULL): every contact the call keeps, 4 slots of
+// (normal xyz, pos xyz, depth) per pair in CollisionResult order.
+double orc_collide_contacts_batch(void* hs, const int* type, const double* params, const int* hull_idx,
+                                  const int* pa, const int* pb, const double* tfa12, const double* tfb12,
+                                  int64_t n, int max_contacts, int penetration, const double* sp6,
+                                  const double* guess3, int n_threads, OrcCollideRec* out, double* contacts7) {
   ShapeTable st{static_cast<HullSet*>(hs), type, params, hull_idx};
   const SolverParams base = solverFrom(sp6);
   const auto t0 = std::chrono::steady_clock::now();
@@ -167,8 +169,27 @@ double orc_collide_batch(void* hs, const int* type, const double* params, const 
       r.normal[0] = o.c[0].normal.x; r.normal[1] = o.c[0].normal.y; r.normal[2] = o.c[0].normal.z;
       r.pos[0] = o.c[0].pos.x; r.pos[1] = o.c[0].pos.y; r.pos[2] = o.c[0].pos.z;
     }
+    if (contacts7) {
+      double* c = contacts7 + 28 * i;
+      for (int k = 0; k < 28; ++k) c[k] = 0;
+      for (int k = 0; k < o.n && k < 4; ++k) {
+        const ContactPt& x = o.c[k];
+        double* d = c + 7 * k;
+        d[0] = x.normal.x; d[1] = x.normal.y; d[2] = x.normal.z;
+        d[3] = x.pos.x; d[4] = x.pos.y; d[5] = x.pos.z;
+        d[6] = x.depth;
+      }
+    }
   });
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+double orc_collide_batch(void* hs, const int* type, const double* params, const int* hull_idx,
+                         const int* pa, const int* pb, const double* tfa12, const double* tfb12,
+                         int64_t n, int max_contacts, int penetration, const double* sp6,
+                         const double* guess3, int n_threads, OrcCollideRec* out) {
+  return orc_collide_contacts_batch(hs, type, params, hull_idx, pa, pb, tfa12, tfb12, n, max_contacts,
+                                    penetration, sp6, guess3, n_threads, out, nullptr);
 }
 
 // GJKSolver::shapeSignedDistance over a batch.  Returns wall seconds.

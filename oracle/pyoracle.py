@@ -21,6 +21,7 @@ COLLIDE_REC = np.dtype(
      ("gjk_iters", np.int32), ("epa_iters", np.int32), ("pad", np.int32), ("depth", np.float64),
      ("normal", np.float64, 3), ("pos", np.float64, 3)]
 )
+CONTACT_REC = np.dtype([("normal", np.float64, 3), ("pos", np.float64, 3), ("depth", np.float64)])
 SD_REC = np.dtype(
     [("ok", np.int32), ("gjk_status", np.int32), ("epa_status", np.int32), ("gjk_iters", np.int32),
      ("epa_iters", np.int32), ("support_calls", np.int32), ("distance", np.float64), ("pa", np.float64, 3),
@@ -50,6 +51,8 @@ def lib() -> C.CDLL:
         L.orc_support.argtypes = [p, p, p, p, i32, p, p]
         L.orc_collide_batch.argtypes = [p, p, p, p, p, p, p, p, i64, i32, i32, p, p, i32, p]
         L.orc_collide_batch.restype = d
+        L.orc_collide_contacts_batch.argtypes = [p, p, p, p, p, p, p, p, i64, i32, i32, p, p, i32, p, p]
+        L.orc_collide_contacts_batch.restype = d
         L.orc_signed_distance_batch.argtypes = [p, p, p, p, p, p, p, p, i64, p, p, i32, p]
         L.orc_signed_distance_batch.restype = d
         L.orc_pair_algorithm.argtypes = [p, p, p, p, i32, i32, p, p, i32, i32, d, p, p]
@@ -157,6 +160,21 @@ class ShapeSet:
                                        _p(g), threads, _p(out))
         return out, secs
 
+    def collide_contacts(self, pa, pb, tfa12, tfb12, max_contacts=4, penetration=True, solver=None, threads=1):
+        """As collide(), plus every kept contact: (n, 4) records of
+        CONTACT_REC in CollisionResult order (entries past out['n'] zero)."""
+        pa = np.ascontiguousarray(pa, np.int32)
+        pb = np.ascontiguousarray(pb, np.int32)
+        tfa12 = np.ascontiguousarray(tfa12, np.float64).reshape(-1, 12)
+        tfb12 = np.ascontiguousarray(tfb12, np.float64).reshape(-1, 12)
+        sp = None if solver is None else np.ascontiguousarray(solver, np.float64)
+        out = np.zeros(len(pa), COLLIDE_REC)
+        cont = np.zeros((len(pa), 4), CONTACT_REC)
+        lib().orc_collide_contacts_batch(self.hs, _p(self.types), _p(self.params), _p(self.hull_idx), _p(pa),
+                                         _p(pb), _p(tfa12), _p(tfb12), len(pa), max_contacts,
+                                         1 if penetration else 0, _p(sp), None, threads, _p(out), _p(cont))
+        return out, cont
+
     def signed_distance(self, pa, pb, tfa12, tfb12, solver=None, guess=None, threads=1):
         pa = np.ascontiguousarray(pa, np.int32)
         pb = np.ascontiguousarray(pb, np.int32)
@@ -179,6 +197,13 @@ class ShapeSet:
         st = lib().orc_pair_algorithm(self.hs, _p(self.types), _p(self.params), _p(self.hull_idx), ia, ib,
                                       _p(tfa12), _p(tfb12), mode, max_iter, tol, _p(g), _p(d))
         return st, d[0], d[1:4], d[4:7]
+
+
+def collide_contacts(S: "ShapeSet", tfa12, tfb12, max_contacts: int):
+    """Every contact CollisionRequest(max_contacts) keeps on one pair
+    (shapes 0 and 1 of S)."""
+    out, cont = S.collide_contacts([0], [1], np.asarray(tfa12)[None], np.asarray(tfb12)[None], max_contacts)
+    return cont[0, :out["n"][0]]
 
 
 def capsule_capsule_distance(r1, lz1, tf1, r2, lz2, tf2) -> float:

@@ -33,6 +33,7 @@ def test_arm_query_matches_cpu_pipeline(gpu_ctx):
     n = len(scene["q"])
     dq, dbox = gpu_ctx.to_device(scene["q"]), gpu_ctx.to_device(scene["box_pose"])
     dany, dmin = gpu_ctx.alloc(4 * n), gpu_ctx.alloc(4 * n)
+    gpu_ctx.reserve_arm(arm, n)
     npairs = gpu_ctx.arm_query_dev(arm, dq, n, MARGIN, dbox, dany, dmin)
     any_hit, min_d = np.zeros(n, np.int32), np.zeros(n, np.float32)
     dany.download(any_hit)
@@ -44,7 +45,7 @@ def test_arm_query_matches_cpu_pipeline(gpu_ctx):
     band = np.abs(mind) <= 1e-5
     hit_mism = ((any_hit & 1) != (flags & 1)) & ~band
     inside = (mind < 0.5 * MARGIN) & (min_d < 0.5 * MARGIN)
-    derr = np.abs(min_d - mind)[inside]
+    derr = np.abs(min_d[inside] - mind[inside])
     print(f"\n[arm] configs={n} pairs gpu={npairs} cpu={nref} colliding={int((flags & 1).sum())} "
           f"failed={int((flags & 2).sum())} hit mismatches={int(hit_mism.sum())} "
           f"max |dist err| {derr.max() if derr.size else 0:.2e}")
@@ -93,6 +94,7 @@ def test_arm_heightmap_matches_cpu_pipeline(gpu_ctx):
     n = len(scene["q"])
     dq, dbox = gpu_ctx.to_device(scene["q"]), gpu_ctx.to_device(scene["box_pose"])
     dany, dmin = gpu_ctx.alloc(4 * n), gpu_ctx.alloc(4 * n)
+    gpu_ctx.reserve_arm(arm, n)
     npairs = gpu_ctx.arm_query_dev(arm, dq, n, MARGIN, dbox, dany, dmin)
     n_hm = gpu_ctx.arm_last_heightmap_pairs()
     any_hit, min_d = np.zeros(n, np.int32), np.zeros(n, np.float32)
@@ -123,7 +125,7 @@ def test_arm_heightmap_matches_cpu_pipeline(gpu_ctx):
     assert int((any_hit & 2).sum()) == 0 and int((flags & 2).sum()) == 0
     assert hm_cnt.max() <= 4096
     inside = (mind < 0.5 * MARGIN) & (min_d < 0.5 * MARGIN)
-    derr = np.abs(min_d - mind)[inside]
+    derr = np.abs(min_d[inside] - mind[inside])
     assert derr.size == 0 or derr.max() <= 1e-5
     # the shortcuts decide most colliding links before any narrowphase
     assert n_hm < len(k)
@@ -175,6 +177,7 @@ def test_arm_heightmap_non_square_tilted(gpu_ctx):
     n = len(scene["q"])
     dq, dbox = gpu_ctx.to_device(scene["q"]), gpu_ctx.to_device(scene["box_pose"])
     dany, dmin = gpu_ctx.alloc(4 * n), gpu_ctx.alloc(4 * n)
+    gpu_ctx.reserve_arm(arm, n)
     gpu_ctx.arm_query_dev(arm, dq, n, MARGIN, dbox, dany, dmin)
     any_hit = np.zeros(n, np.int32)
     dany.download(any_hit)
