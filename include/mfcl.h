@@ -210,6 +210,20 @@ int mfcl_convex_query_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t*
                           const float* pose_a7, const float* pose_b7, const float* guess3,
                           uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
                           void* stream);
+/* No counterpart in the reference: mfcl_convex_query_dev for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results; needs no mfcl_reserve: the workspace is a static 64 MiB device
+ * array per GPU, so one call takes at most
+ * mfcl_convex_query_static_dev_max_pairs() pairs (larger calls fail with
+ * MFCL_E_INVALID).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (concurrent streams, threads or contexts) race; order
+ * them on one stream. */
+int mfcl_convex_query_static_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                                 const float* pose_a7, const float* pose_b7, const float* guess3,
+                                 uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
+                                 void* stream);
+int mfcl_convex_query_static_dev_max_pairs(uint64_t* max_pairs);
 /* fcl::collide with every contact the CollisionResult would receive
  * (collide mode with penetration only): up to min(max_contacts, 4) per pair
  * -- boxBox2 yields up to 4 (box_box-inl.h:823-841), the other pair types

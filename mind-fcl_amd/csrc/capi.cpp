@@ -160,6 +160,9 @@ int mfcl_create(mfcl_ctx** out, int device) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     c->n_cu = cus;
+  // the static workspace's address on this device (left null on failure:
+  // mfcl_convex_query_static_dev then reports it)
+  if (staticScratch(device, &c->d_static_scratch) != MFCL_OK) c->d_static_scratch = nullptr;
   *out = c;
   return MFCL_OK;
 }
@@ -466,6 +469,45 @@ int mfcl_convex_collide_contacts_dev(mfcl_ctx* ctx, const uint32_t* pa, const ui
   const int rc = convexQueryLaunch(ctx, pa, pb, posea, poseb, guess, n, req, out, contacts, ws, s);
   if (rc) return rc;
   if (ctx->timing) return finishTiming(ctx, 0, 4, 0);
+  return MFCL_OK;
+}
+
+int mfcl_convex_query_static_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                                 const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
+                                 mfcl_convex_result* out, void* stream) {
+  if (!ctx || !req) return MFCL_E_INVALID;
+  if (n && (!pa || !pb || !posea || !poseb || !out)) {
+    ctx->err = "null device pointer";
+    return MFCL_E_INVALID;
+  }
+  if (n == 0) return MFCL_OK;
+  const uint64_t max_pairs = staticScratchMaxPairs();
+  if (n > max_pairs) {
+    ctx->err = "mfcl_convex_query_static_dev takes at most " + std::to_string(max_pairs) + " pairs per call (" +
+               std::to_string(kStaticScratchBytes >> 20) + " MiB static workspace), got " + std::to_string(n);
+    return MFCL_E_INVALID;
+  }
+  if (!ctx->d_static_scratch) {
+    ctx->err = "the static workspace's device address could not be looked up at mfcl_create";
+    return MFCL_E_STATE;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  ctx->counters[0] = n;
+  ConvexWs ws;
+  if (!carveConvexWs(ctx->d_static_scratch, kStaticScratchBytes, n, ws)) {
+    ctx->err = "static workspace layout exceeds its size";
+    return MFCL_E_STATE;
+  }
+  const int rc = convexQueryLaunch(ctx, pa, pb, posea, poseb, guess, n, req, out, nullptr, ws, s);
+  if (rc) return rc;
+  if (ctx->timing) return finishTiming(ctx, 0, 4, 0);
+  return MFCL_OK;
+}
+
+int mfcl_convex_query_static_dev_max_pairs(uint64_t* max_pairs) {
+  if (!max_pairs) return MFCL_E_INVALID;
+  *max_pairs = staticScratchMaxPairs();
   return MFCL_OK;
 }
 

@@ -136,6 +136,7 @@ struct mfcl_ctx {
   uint64_t ws_arm_key[4]{};     // links, broadphase cap, max pairs per config, pair capacity
   uint32_t mesh_ws_waves{0}, mesh_ws_cap{0};  // stacks ws_arena holds
   uint32_t* d_qcounts{nullptr};  // EPA queue / overflow counts of the last convex query
+  void* d_static_scratch{nullptr};  // this device's static workspace (mfcl_convex_query_static_dev)
 
   mfcl::MlpState mlp;
 
@@ -177,6 +178,12 @@ struct ConvexWs {
 size_t convexWsBytes(uint64_t n);
 uint64_t convexWsMaxPairs(size_t bytes);
 bool carveConvexWs(void* base, size_t bytes, uint64_t n, ConvexWs& w);
+// The static workspace of mfcl_convex_query_static_dev (narrowphase.hip):
+// 64 MiB per GPU; staticScratch() returns its device address on `device`
+// (looked up once per device, with that device current).
+constexpr size_t kStaticScratchBytes = size_t{64} << 20;
+int staticScratch(int device, void** out);
+uint64_t staticScratchMaxPairs();
 // Grows ctx->ws_convex to hold n pairs (host call; may synchronise).
 int reserveConvex(mfcl_ctx* ctx, uint64_t n);
 // Mesh traversal stacks for the current tune knobs (bvh.hip; host call).

@@ -11,6 +11,8 @@
 //      in device memory.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "ctx.h"
 #include "epa_dev.h"
 #include "gjk_lane.h"
@@ -727,6 +729,35 @@ bool carveConvexWs(void* base, size_t bytes, uint64_t n, ConvexWs& w) {
   w.lane_dirs = reinterpret_cast<double*>(take(laneDirBytes(n ? n : 1)));
   return true;
 }
+
+// Workspace of mfcl_convex_query_static_dev, for callers that cannot
+// allocate: a file-scope device array, one copy per GPU (its own address on
+// each), held until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): it
+// uses the device address that staticScratch() looked up once per device.
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+int staticScratch(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return MFCL_E_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    // the caller has made `device` current
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return MFCL_E_HIP;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return MFCL_OK;
+}
+
+uint64_t staticScratchMaxPairs() { return convexWsMaxPairs(kStaticScratchBytes); }
 
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,

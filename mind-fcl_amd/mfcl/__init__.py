@@ -150,6 +150,8 @@ def load_library() -> C.CDLL:
     lib.mfcl_convex_query_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
     lib.mfcl_convex_collide_contacts.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
     lib.mfcl_convex_collide_contacts_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p, p]
+    lib.mfcl_convex_query_static_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
+    lib.mfcl_convex_query_static_dev_max_pairs.argtypes = [C.POINTER(u64)]
     lib.mfcl_reserve.argtypes = [p, i32, u64]
     lib.mfcl_reserved_pairs.argtypes = [p, C.POINTER(u64)]
     lib.mfcl_reserve_arm.argtypes = [p, C.POINTER(Arm), u64]
