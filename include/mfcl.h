@@ -132,7 +132,9 @@ int mfcl_last_timing(mfcl_ctx* ctx, float* ms6);
  * polytope capacity after exhausting the on-chip polytope. */
 int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4);
 /* The same counters and more, the first n of: [0..3] as above, [4] hull
- * vertices read by the EPA stage alone (entries beyond 4 read 0). */
+ * vertices read by the EPA stage alone, [5] EPA pairs that outgrew the
+ * on-chip polytope and continued in a device-memory polytope of full
+ * capacity (entries beyond 5 read 0). */
 int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n);
 
 /* Launch-shape knobs (no counterpart in the reference: GPU scheduling
@@ -145,6 +147,8 @@ int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n);
 #define MFCL_TUNE_EPA_LDS_FACES 5     /* face capacity of the LDS polytope, 16..128 */
 #define MFCL_TUNE_GUESS_POLICY 6      /* mfcl_mlp_guess_dev: 0 the MLP direction for every pair; 1 (default)
                                          the default guess where the MLP predicts penetration */
+#define MFCL_TUNE_EPA_SPILL 7         /* 1 (default): a pair that outgrows the LDS polytope continues in a
+                                         device-memory polytope of full capacity; 0: it reruns afterwards */
 int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value);
 
 /* Workspace sizing (no counterpart in the reference, whose calls allocate

@@ -408,7 +408,8 @@ int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t 
     return MFCL_E_STATE;
   }
   ConvexWs cws;
-  if (!carveConvexWs(ctx->ws_convex.p, ctx->ws_convex.bytes, max_pairs, cws)) {
+  if (!carveConvexWs(ctx->ws_convex.p, ctx->ws_convex.bytes, max_pairs, epaArenaBytes(kReservedArenaSlices, 256),
+                     cws)) {
     ctx->err = "convex workspace layout exceeds its allocation";
     return MFCL_E_STATE;
   }

@@ -92,8 +92,9 @@ int reserveConvex(mfcl_ctx* ctx, uint64_t n) {
   // the counters of the last query live in this buffer
   ctx->d_counters = nullptr;
   ctx->d_qcounts = nullptr;
-  if (int rc = ensure(ctx, ctx->ws_convex, convexWsBytes(n))) return rc;
-  ctx->ws_convex_pairs = convexWsMaxPairs(ctx->ws_convex.bytes);
+  const size_t arena = epaArenaBytes(kReservedArenaSlices, 256);
+  if (int rc = ensure(ctx, ctx->ws_convex, convexWsBytes(n, arena))) return rc;
+  ctx->ws_convex_pairs = convexWsMaxPairs(ctx->ws_convex.bytes, arena);
   return MFCL_OK;
 }
 
@@ -203,9 +204,10 @@ static int refreshCounters(mfcl_ctx* ctx) {
   if (ctx->d_counters && ctx->d_qcounts) {
     MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     uint64_t dev[2] = {0, 0};
-    uint32_t q[2] = {0, 0};  // EPA queue count, overflow count
+    uint32_t q[5] = {0, 0, 0, 0, 0};  // EPA queue count, overflow count, 2 cursors, continued in device memory
     MFCL_HIP_TRY(ctx, hipMemcpy(dev, ctx->d_counters, sizeof(dev), hipMemcpyDeviceToHost));
     MFCL_HIP_TRY(ctx, hipMemcpy(q, ctx->d_qcounts, sizeof(q), hipMemcpyDeviceToHost));
+    ctx->counters[5] = q[4];
     ctx->counters[1] = dev[0] + dev[1];
     ctx->counters[2] = q[0];
     ctx->counters[3] = q[1];
@@ -224,7 +226,7 @@ int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4) {
 int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n) {
   if (!ctx || !c || n < 0) return MFCL_E_INVALID;
   if (int rc = refreshCounters(ctx)) return rc;
-  for (int32_t k = 0; k < n; ++k) c[k] = k < 5 ? ctx->counters[k] : 0;
+  for (int32_t k = 0; k < n; ++k) c[k] = k < 6 ? ctx->counters[k] : 0;
   return MFCL_OK;
 }
 
@@ -250,6 +252,10 @@ int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
     case MFCL_TUNE_GUESS_POLICY:
       if (value < 0 || value > 1) break;
       ctx->tune.guess_policy = value;
+      return MFCL_OK;
+    case MFCL_TUNE_EPA_SPILL:
+      if (value < 0 || value > 1) break;
+      ctx->tune.epa_spill = value;
       return MFCL_OK;
     case MFCL_TUNE_MESH_STACK:
       if (value < 1024 || value > (1 << 18)) break;
@@ -462,7 +468,7 @@ int mfcl_convex_collide_contacts_dev(mfcl_ctx* ctx, const uint32_t* pa, const ui
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   ctx->counters[0] = n;
   ConvexWs ws;
-  if (!carveConvexWs(ctx->ws_convex.p, ctx->ws_convex.bytes, n, ws)) {
+  if (!carveConvexWs(ctx->ws_convex.p, ctx->ws_convex.bytes, n, epaArenaBytes(kReservedArenaSlices, 256), ws)) {
     ctx->err = "convex workspace layout exceeds its allocation";
     return MFCL_E_STATE;
   }
@@ -495,7 +501,7 @@ int mfcl_convex_query_static_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   ctx->counters[0] = n;
   ConvexWs ws;
-  if (!carveConvexWs(ctx->d_static_scratch, kStaticScratchBytes, n, ws)) {
+  if (!carveConvexWs(ctx->d_static_scratch, kStaticScratchBytes, n, epaArenaBytes(kStaticArenaSlices, 256), ws)) {
     ctx->err = "static workspace layout exceeds its size";
     return MFCL_E_STATE;
   }

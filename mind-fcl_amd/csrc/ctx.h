@@ -81,7 +81,8 @@ struct mfcl_ctx {
     int gjk_group{1};           // lanes per pair in the GJK stage (1, 16, 32, 64)
     int epa_threads{131072};    // persistent EPA lanes (multiple of 256): 64 per wavefront
     int guess_policy{1};        // warm start: 0 MLP direction for every pair, 1 not for predicted penetration
-    int epa_lds_faces{48};      // face capacity of the LDS polytope (overflow reruns at full size)
+    int epa_lds_faces{64};      // face capacity of the LDS polytope (overflow continues or reruns at full size)
+    int epa_spill{1};           // overflow continues in a device-memory polytope (1) or reruns (0)
     int epa_overflow_waves{512};   // persistent wavefronts (one per block) for overflow pairs
     int mesh_blocks_per_cu{4};  // persistent mesh-traversal blocks (4 waves) per CU
     int mesh_stack{16384};      // node-pair stack entries per wavefront
@@ -144,7 +145,7 @@ struct mfcl_ctx {
   bool timing{false};
   hipEvent_t ev[16]{};
   float last_ms[6]{0, 0, 0, 0, 0, 0};
-  uint64_t counters[5]{0, 0, 0, 0, 0};
+  uint64_t counters[6]{0, 0, 0, 0, 0, 0};
   uint64_t* d_counters{nullptr};  // device-side work counters
 };
 
@@ -174,10 +175,16 @@ struct ConvexWs {
   void* queue;         // EPA queue
   void* oflow;         // EPA overflow queue
   double* lane_dirs;   // lane kernel direction slots
+  void* arena;         // EPA continuation polytopes (device memory), one per wavefront that may need one
+  size_t arena_bytes;
 };
-size_t convexWsBytes(uint64_t n);
-uint64_t convexWsMaxPairs(size_t bytes);
-bool carveConvexWs(void* base, size_t bytes, uint64_t n, ConvexWs& w);
+// EPA continuation arena of the reserved workspace (2048 polytopes of 256
+// faces) and of the static one (256 such polytopes).
+size_t epaArenaBytes(int slices, int max_faces);
+constexpr int kReservedArenaSlices = 2048, kStaticArenaSlices = 256;
+size_t convexWsBytes(uint64_t n, size_t arena_bytes);
+uint64_t convexWsMaxPairs(size_t bytes, size_t arena_bytes);
+bool carveConvexWs(void* base, size_t bytes, uint64_t n, size_t arena_bytes, ConvexWs& w);
 // The static workspace of mfcl_convex_query_static_dev (narrowphase.hip):
 // 64 MiB per GPU; staticScratch() returns its device address on `device`
 // (looked up once per device, with that device current).

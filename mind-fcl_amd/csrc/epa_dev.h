@@ -7,6 +7,8 @@
 // stamps stand in for the reference's newest-first element lists.
 #pragma once
 
+#include <type_traits>
+
 #include "narrow_dev.h"
 
 namespace mfcl {
@@ -19,6 +21,7 @@ struct EVert {
   int32_t stamp;      // creation order; < 0: free slot
   int16_t new_edge;
   uint8_t remove, pad;
+  int32_t h0, h1;     // the support vertices of the two hulls (-1: primitive or unknown)
 };
 struct EEdge {
   double d2, wx, wy, wz;  // squared distance to origin, witness
@@ -136,6 +139,23 @@ MFCL_D void triDist0(d3 a, d3 b, d3 c, MinD& o) {
   o.w = best.w;
 }
 
+// Where a polytope's slot arrays live: the wavefront's LDS slice (the
+// normal case) or a device-memory arena (a polytope that outgrew the LDS
+// slice and continues at full capacity).  sync() orders one lane's writes
+// before other lanes' reads of the same wavefront: LDS executes a
+// wavefront's operations in order, so a compiler barrier suffices there;
+// device memory needs a workgroup-scope fence.
+struct LdsMem {
+  template <class T>
+  using P = MFCL_LDS T*;
+  static MFCL_D void sync() { __builtin_amdgcn_wave_barrier(); }
+};
+struct GlbMem {
+  template <class T>
+  using P = T*;
+  static MFCL_D void sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
+};
+
 // The polytope, processed by the 64 lanes of one wavefront together.
 // Elements live in fixed slot arrays (slot s is visited by lane s % 64);
 // a slot is free when its stamp is negative.  Every element gets a creation
@@ -146,11 +166,12 @@ MFCL_D void triDist0(d3 a, d3 b, d3 c, MinD& o) {
 // which reproduces its tie-breaking exactly.  Slot numbers themselves never
 // influence a result.  Every member function must be called by all 64 lanes
 // with identical arguments.
-struct Poly {
-  MFCL_LDS EVert* V;
-  MFCL_LDS EEdge* E;
-  MFCL_LDS EFace* F;
-  MFCL_LDS BorderRec* BL;  // nv records
+template <class M>
+struct PolyT {
+  typename M::template P<EVert> V;
+  typename M::template P<EEdge> E;
+  typename M::template P<EFace> F;
+  typename M::template P<BorderRec> BL;  // nv records
   int nv, nf;   // slot capacities: vertices and edges nv, faces nf
   int stamp;    // creation counter
 
@@ -166,12 +187,12 @@ struct Poly {
     }
     for (int s = l; s < nf; s += 64) F[s].stamp = -1;
     stamp = 0;
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
   }
   // lowest free slot (the reference pops its free list; which slot is used
   // does not matter, only the capacity does)
-  template <class R>
-  MFCL_D int16_t allocSlot(MFCL_LDS R* arr, int cap) const {
+  template <class A>
+  MFCL_D int16_t allocSlot(A arr, int cap) const {
     const int l = __lane_id();
     for (int base = 0; base < cap; base += 64) {
       const int s = base + l;
@@ -180,21 +201,22 @@ struct Poly {
     }
     return kNil;
   }
-  MFCL_D int16_t addVertex(d3 v, d3 d) {
+  MFCL_D int16_t addVertex(d3 v, d3 d, int h0 = -1, int h1 = -1) {
     const int16_t i = allocSlot(V, nv);
     if (i == kNil) return kNil;
-    MFCL_LDS EVert& x = V[i];
+    auto& x = V[i];
     x.vx = v.x; x.vy = v.y; x.vz = v.z;
     x.dx = d.x; x.dy = d.y; x.dz = d.z;
+    x.h0 = h0; x.h1 = h1;
     x.stamp = stamp++;
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     return i;
   }
   MFCL_D int16_t addEdge(int16_t a, int16_t b) {
     if (a == kNil || b == kNil) return kNil;
     const int16_t i = allocSlot(E, nv);
     if (i == kNil) return kNil;
-    MFCL_LDS EEdge& e = E[i];
+    auto& e = E[i];
     e.v0 = a;
     e.v1 = b;
     e.f0 = e.f1 = kNil;
@@ -203,14 +225,14 @@ struct Poly {
     segDist(d3{0, 0, 0}, vloc(a), vloc(b), m);
     e.in = m.in; e.d2 = m.d2; e.wx = m.w.x; e.wy = m.w.y; e.wz = m.w.z;
     e.stamp = stamp++;
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     return i;
   }
   MFCL_D int16_t addFace(int16_t e1, int16_t e2, int16_t e3) {
     if (e1 == kNil || e2 == kNil || e3 == kNil) return kNil;
     const int16_t i = allocSlot(F, nf);
     if (i == kNil) return kNil;
-    MFCL_LDS EFace& f = F[i];
+    auto& f = F[i];
     f.e0 = e1; f.e1 = e2; f.e2 = e3;
     f.vis = kFUnknown;
     const int16_t a = E[e1].v0, b = E[e1].v1;
@@ -219,23 +241,23 @@ struct Poly {
     triDist0(vloc(a), vloc(b), vloc(c), m);
     f.in = m.in; f.d2 = m.d2; f.wx = m.w.x; f.wy = m.w.y; f.wz = m.w.z;
     f.stamp = stamp++;
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     // the face stays in the polytope even when an edge already has two faces
     const int16_t es[3] = {e1, e2, e3};
     for (int k = 0; k < 3; ++k) {
-      MFCL_LDS EEdge& ek = E[es[k]];
+      auto& ek = E[es[k]];
       if (ek.f0 == kNil) {
         ek.f0 = i;
       } else {
         if (ek.f1 != kNil) return kNil;
         ek.f1 = i;
       }
-      __builtin_amdgcn_wave_barrier();
+      M::sync();
     }
     return i;
   }
   MFCL_D void faceVerts(int16_t fi, int16_t& a, int16_t& b, int16_t& c) const {
-    const MFCL_LDS EFace& f = F[fi];
+    const auto& f = F[fi];
     a = E[f.e0].v0;
     b = E[f.e0].v1;
     c = (E[f.e1].v0 != a && E[f.e1].v0 != b) ? E[f.e1].v0 : E[f.e1].v1;
@@ -262,14 +284,14 @@ struct Poly {
         if (better(d2, 1, st, bd, bk, bs)) { bd = d2; bk = 1; bs = st; bi = s; }
       }
     for (int s = l; s < nv; s += 64) {
-      const MFCL_LDS EEdge& e = E[s];
+      const auto& e = E[s];
       const int st = e.stamp;
       if (st < 0) continue;
       if (!excl_vertex && !e.in) continue;
       if (better(e.d2, 2, st, bd, bk, bs)) { bd = e.d2; bk = 2; bs = st; bi = s; }
     }
     for (int s = l; s < nf; s += 64) {
-      const MFCL_LDS EFace& f = F[s];
+      const auto& f = F[s];
       const int st = f.stamp;
       if (st < 0 || !f.in) continue;
       if (better(f.d2, 3, st, bd, bk, bs)) { bd = f.d2; bk = 3; bs = st; bi = s; }
@@ -331,12 +353,15 @@ struct Poly {
   }
   MFCL_D bool anyLane(bool b) const { return __ballot(b) != 0ull; }
 
-  // epa_polytope_expand.hpp:32-281. 0 ok, 1 failed, 2 malloc failed.
+  // epa_polytope_expand.hpp:32-281. 0 ok, 1 failed, 2 malloc failed,
+  // 3 (only when may_spill) this polytope's capacity might not hold the
+  // expansion -- nothing has been changed then, and the caller continues
+  // in a polytope of the full capacity.
   // The reference's depth-first flood fill marks the faces reachable from
   // `start` through faces that see nv; that set, and the border / internal
   // edge marks, do not depend on the visiting order, so it runs here as
   // rounds of a parallel breadth-first fill.
-  MFCL_D int expand(d3 nv_, d3 nd, int16_t start) {
+  MFCL_D int expand(d3 nv_, d3 nd, int nh0, int nh1, int16_t start, bool may_spill) {
     const int l = __lane_id();
     for (int s = l; s < nv; s += 64) {
       V[s].remove = 1;
@@ -344,19 +369,19 @@ struct Poly {
       E[s].vis = kUnknown;
     }
     for (int s = l; s < nf; s += 64) F[s].vis = kFUnknown;
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     F[start].vis = kVisible;
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     while (true) {
       bool changed = false;
       for (int s = l; s < nf; s += 64) {
-        const MFCL_LDS EFace& f = F[s];
+        const auto& f = F[s];
         if (f.stamp < 0 || f.vis != kFUnknown) continue;
         const int16_t es[3] = {f.e0, f.e1, f.e2};
         bool near_visible = false;
         for (int k = 0; k < 3; ++k) {
           if (es[k] == kNil) continue;
-          const MFCL_LDS EEdge& e = E[es[k]];
+          const auto& e = E[es[k]];
           const int16_t g = e.f0 == s ? e.f1 : e.f0;
           if (g != kNil && F[g].vis == kVisible) near_visible = true;
         }
@@ -365,49 +390,50 @@ struct Poly {
         F[s].vis = out ? kVisible : kHidden;
         changed = true;
       }
-      __builtin_amdgcn_wave_barrier();
+      M::sync();
       if (!anyLane(changed)) break;
     }
     // an open edge on a visible face is a malformed polytope
     bool bad = false;
     for (int s = l; s < nf; s += 64) {
-      const MFCL_LDS EFace& f = F[s];
+      const auto& f = F[s];
       if (f.stamp < 0 || f.vis != kVisible) continue;
       const int16_t es[3] = {f.e0, f.e1, f.e2};
       for (int k = 0; k < 3; ++k) {
         if (es[k] == kNil) { bad = true; continue; }
-        const MFCL_LDS EEdge& e = E[es[k]];
+        const auto& e = E[es[k]];
         const int16_t g = e.f0 == s ? e.f1 : e.f0;
         if (g == kNil) bad = true;
       }
     }
     if (anyLane(bad)) return 1;
+    if (may_spill && !fits()) return 3;
     // edge marks from the final face classes
     for (int s = l; s < nv; s += 64) {
-      MFCL_LDS EEdge& e = E[s];
+      auto& e = E[s];
       if (e.stamp < 0) continue;
       const bool v0 = e.f0 != kNil && F[e.f0].vis == kVisible;
       const bool v1 = e.f1 != kNil && F[e.f1].vis == kVisible;
       if (v0 && v1) e.vis = kInternal;
       else if (v0 || v1) e.vis = kBorder;
     }
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     for (int s = l; s < nv; s += 64) {
-      const MFCL_LDS EEdge& e = E[s];
+      const auto& e = E[s];
       if (e.stamp < 0 || e.vis == kInternal) continue;
       V[e.v0].remove = 0;
       V[e.v1].remove = 0;
     }
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     // drop the visible faces (detaching them from their border edges), the
     // internal edges and the vertices left without a non-internal edge
     for (int s = l; s < nf; s += 64) {
-      MFCL_LDS EFace& f = F[s];
+      auto& f = F[s];
       if (f.stamp < 0 || f.vis != kVisible) continue;
       const int16_t es[3] = {f.e0, f.e1, f.e2};
       for (int k = 0; k < 3; ++k) {
         if (es[k] == kNil) continue;
-        MFCL_LDS EEdge& ek = E[es[k]];
+        auto& ek = E[es[k]];
         if (ek.vis == kInternal) continue;  // removed below
         if (ek.f0 == s) ek.f0 = ek.f1;
         ek.f1 = kNil;
@@ -415,15 +441,15 @@ struct Poly {
       f.stamp = -1;
     }
     for (int s = l; s < nv; s += 64) {
-      MFCL_LDS EEdge& e = E[s];
+      auto& e = E[s];
       if (e.stamp >= 0 && e.vis == kInternal) e.stamp = -1;
     }
     for (int s = l; s < nv; s += 64) {
-      MFCL_LDS EVert& v = V[s];
+      auto& v = V[s];
       if (v.stamp >= 0 && v.remove) v.stamp = -1;
     }
-    __builtin_amdgcn_wave_barrier();
-    const int16_t nvi = addVertex(nv_, nd);
+    M::sync();
+    const int16_t nvi = addVertex(nv_, nd, nh0, nh1);
     if (nvi == kNil) return 2;
     return coverBorder(nvi);
   }
@@ -449,23 +475,23 @@ struct Poly {
       if (isb) BL[nb + __popcll(m & below)].e = static_cast<int16_t>(s);
       nb += __popcll(m);
     }
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     for (int j = l; j < nb; j += 64) {
       const int st = E[BL[j].e].stamp;
       int rank = 0;
       for (int k = 0; k < nb; ++k) rank += E[BL[k].e].stamp > st;
       BL[j].rank = static_cast<int16_t>(rank);
     }
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     bool bad = false;
     int new_edges = 0;
     for (int j = l; j < nb; j += 64) {
-      const MFCL_LDS EEdge& ej = E[BL[j].e];
+      const auto& ej = E[BL[j].e];
       const int r = BL[j].rank, a = ej.v0, b = ej.v1;
       int t0 = 0, t1 = 0;
       for (int k = 0; k < nb; ++k) {
         if (BL[k].rank >= r) continue;
-        const MFCL_LDS EEdge& ek = E[BL[k].e];
+        const auto& ek = E[BL[k].e];
         t0 += ek.v0 == a || ek.v1 == a;
         t1 += ek.v0 == b || ek.v1 == b;
       }
@@ -475,7 +501,7 @@ struct Poly {
       new_edges += (t0 == 0) + (t1 == 0);
     }
     if (anyLane(bad)) return 2;
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     for (int j = l; j < nb; j += 64) {
       const int r = BL[j].rank;
       int pre = 0;
@@ -510,18 +536,18 @@ struct Poly {
         cf += __popcll(m);
       }
     }
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     // new edges: the k-th created edge takes the k-th free edge slot
     for (int j = l; j < nb; j += 64) {
-      const MFCL_LDS BorderRec& bj = BL[j];
-      const MFCL_LDS EEdge& ej = E[bj.e];
+      const auto& bj = BL[j];
+      const auto& ej = E[bj.e];
       const int edges_before = bj.pre - bj.rank;
       int q = 0;
       for (int side = 0; side < 2; ++side) {
         if ((side ? bj.t1 : bj.t0) != 0) continue;
         const int16_t vq = side ? ej.v1 : ej.v0;
         const int16_t slot = BL[edges_before + q].fe;
-        MFCL_LDS EEdge& x = E[slot];
+        auto& x = E[slot];
         x.v0 = nvi;
         x.v1 = vq;
         x.f0 = x.f1 = kNil;
@@ -534,16 +560,16 @@ struct Poly {
         ++q;
       }
     }
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     // faces: the face of rank r takes the r-th free face slot
     for (int j = l; j < nb; j += 64) {
-      const MFCL_LDS BorderRec& bj = BL[j];
+      const auto& bj = BL[j];
       const int16_t ei = bj.e;
-      MFCL_LDS EEdge& ej = E[ei];
+      auto& ej = E[ei];
       const int16_t a = ej.v0, b = ej.v1;
       const int16_t ne0 = V[a].new_edge, ne1 = V[b].new_edge;
       const int16_t slot = BL[bj.rank].ff;
-      MFCL_LDS EFace& f = F[slot];
+      auto& f = F[slot];
       f.e0 = ei; f.e1 = ne0; f.e2 = ne1;
       f.vis = kFUnknown;
       MinD m;
@@ -551,25 +577,65 @@ struct Poly {
       f.in = m.in; f.d2 = m.d2; f.wx = m.w.x; f.wy = m.w.y; f.wz = m.w.z;
       f.stamp = stamp + bj.pre + (bj.t0 == 0) + (bj.t1 == 0);
     }
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     // links: the border edge keeps its hidden face as f0; each new edge gets
     // its lower-ranked face as f0 and the other as f1
     bool clash = false;
     for (int j = l; j < nb; j += 64) {
-      const MFCL_LDS BorderRec& bj = BL[j];
-      MFCL_LDS EEdge& ej = E[bj.e];
+      const auto& bj = BL[j];
+      auto& ej = E[bj.e];
       const int16_t slot = BL[bj.rank].ff;
       if (ej.f0 == kNil) ej.f0 = slot;
       else if (ej.f1 != kNil) clash = true;
       else ej.f1 = slot;
-      MFCL_LDS EEdge& x0 = E[V[ej.v0].new_edge];
-      MFCL_LDS EEdge& x1 = E[V[ej.v1].new_edge];
+      auto& x0 = E[V[ej.v0].new_edge];
+      auto& x1 = E[V[ej.v1].new_edge];
       if (bj.t0 == 0) x0.f0 = slot; else x0.f1 = slot;
       if (bj.t1 == 0) x1.f0 = slot; else x1.f1 = slot;
     }
     stamp += new_edges + nb;
-    __builtin_amdgcn_wave_barrier();
+    M::sync();
     return anyLane(clash) ? 2 : 0;
+  }
+
+  // Whether the expansion whose visible faces are marked certainly fits:
+  // after the visible faces, the internal edges and the vertices left
+  // without a non-internal edge are freed, one vertex, at most two new
+  // edges per border edge and one face per border edge must find slots.
+  // Marks only; nothing is changed.
+  MFCL_D bool fits() const {
+    const int l = __lane_id();
+    int free_v = 0, free_e = 0, free_f = 0, nb = 0;
+    for (int base = 0; base < nf; base += 64) {
+      const int s = base + l;
+      bool fr = false;
+      if (s < nf) fr = F[s].stamp < 0 || F[s].vis == kVisible;
+      free_f += __popcll(__ballot(fr));
+    }
+    for (int base = 0; base < nv; base += 64) {
+      const int s = base + l;
+      bool fe = false, bd = false, fv = false;
+      if (s < nv) {
+        const auto& e = E[s];
+        if (e.stamp < 0) {
+          fe = true;
+        } else {
+          const bool v0 = e.f0 != kNil && F[e.f0].vis == kVisible;
+          const bool v1 = e.f1 != kNil && F[e.f1].vis == kVisible;
+          fe = v0 && v1;
+          bd = v0 != v1;
+        }
+        // a vertex stays when some edge of the polytope that is not internal
+        // touches it; counted below as free only when it has no such edge
+        fv = V[s].stamp < 0;
+      }
+      free_e += __popcll(__ballot(fe));
+      nb += __popcll(__ballot(bd));
+      free_v += __popcll(__ballot(fv));
+    }
+    // the vertices removed by the expansion are not counted as free: the
+    // test only needs one vertex slot, so a free one must exist already
+    return free_v >= 1 && free_e >= 2 * nb && free_f >= nb;
   }
 
  private:
@@ -601,6 +667,9 @@ struct Poly {
   }
 };
 
+using Poly = PolyT<LdsMem>;
+using PolyG = PolyT<GlbMem>;
+
 // Result of the EPA stage in shape-0 frame.
 struct EpaOut {
   int status;
@@ -608,6 +677,7 @@ struct EpaOut {
   d3 p0, p1;
   int iters;
   int scanned_extra;  // hull vertices scanned by the out-of-line helpers
+  int spilled;        // 1: continued in the device-memory polytope
 };
 
 // The out-of-line helpers take the Minkowski difference by value (a
@@ -772,6 +842,7 @@ struct RawF {
   int kind;
   MinD m;
   MVD a, b, c;
+  int h[3][2];  // support vertices of a, b, c (-1: primitive or unknown)
 };
 
 MFCL_D void ldsSet(MFCL_LDS d3& o, d3 v) {
@@ -781,55 +852,68 @@ MFCL_D void ldsSet(MFCL_LDS d3& o, d3 v) {
 }
 MFCL_D d3 ldsGet(const MFCL_LDS d3& o) { return d3{o.x, o.y, o.z}; }
 
-MFCL_D void toRaw(const Poly& P, int kind, int16_t i, MFCL_LDS RawF& r) {
+template <class PT>
+MFCL_D void toRaw(const PT& P, int kind, int16_t i, MFCL_LDS RawF& r) {
   r.kind = kind;
+  auto rec = [&](int k, int16_t v) {
+    r.h[k][0] = P.V[v].h0;
+    r.h[k][1] = P.V[v].h1;
+  };
   if (kind == 1) {
     const d3 v = P.vloc(i);
     r.m.in = true; r.m.d2 = sqnorm(v); ldsSet(r.m.w, v);
     ldsSet(r.a.v, v); ldsSet(r.a.d, P.vdir(i));
+    rec(0, i);
   } else if (kind == 2) {
-    const MFCL_LDS EEdge& e = P.E[i];
+    const auto& e = P.E[i];
     r.m.in = e.in; r.m.d2 = e.d2; ldsSet(r.m.w, d3{e.wx, e.wy, e.wz});
     ldsSet(r.a.v, P.vloc(e.v0)); ldsSet(r.a.d, P.vdir(e.v0));
     ldsSet(r.b.v, P.vloc(e.v1)); ldsSet(r.b.d, P.vdir(e.v1));
+    rec(0, e.v0);
+    rec(1, e.v1);
   } else {
-    const MFCL_LDS EFace& f = P.F[i];
+    const auto& f = P.F[i];
     r.m.in = f.in; r.m.d2 = f.d2; ldsSet(r.m.w, d3{f.wx, f.wy, f.wz});
     int16_t ia, ib, ic;
     P.faceVerts(i, ia, ib, ic);
     ldsSet(r.a.v, P.vloc(ia)); ldsSet(r.a.d, P.vdir(ia));
     ldsSet(r.b.v, P.vloc(ib)); ldsSet(r.b.d, P.vdir(ib));
     ldsSet(r.c.v, P.vloc(ic)); ldsSet(r.c.d, P.vdir(ic));
+    rec(0, ia);
+    rec(1, ib);
+    rec(2, ic);
   }
 }
 
 // epa.hpp:11-113; returns 0 ok, 1 failed, 2 converge
-template <class MK>
-MFCL_D int nextDir(const Poly& P, MK& md, int kind, int16_t fi, double tol, d3& nd, d3& nv, int16_t& start) {
+// nh0 / nh1 receive the support vertices of the last support point.
+template <class PT, class MK>
+MFCL_D int nextDir(const PT& P, MK& md, int kind, int16_t fi, double tol, d3& nd, d3& nv, int16_t& start, int& nh0,
+                   int& nh1) {
   constexpr double kOuter = 1e-3 * 1e-3;
   auto face_cand = [&](int16_t f, bool try_w) -> int {
     d3 fn;
     if (!P.faceNormalOut(f, fn, nullptr)) return 1;
     const d3 on = P.faceAnchor(f);
-    const MFCL_LDS EFace& F = P.F[f];
+    const auto& F = P.F[f];
     if (try_w && F.d2 > kOuter) {
       nd = normalized(d3{F.wx, F.wy, F.wz});
-      nv = md.support(nd);
+      nv = md.supportFrom(nd, -1, -1, nh0, nh1);
       if (dot(fn, nv - on) >= 0.0) { start = f; return 0; }
     }
     nd = fn;
-    nv = md.support(nd);
+    nv = md.supportFrom(nd, -1, -1, nh0, nh1);
     if (dot(fn, nv - on) > tol) { start = f; return 0; }
     return 2;
   };
   if (kind == 3) return face_cand(fi, true);
   if (kind == 2) {
-    const MFCL_LDS EEdge& e = P.E[fi];
+    const auto& e = P.E[fi];
     const int16_t f0 = e.f0, f1 = e.f1;
     if (f0 == kNil || f1 == kNil) return 1;  // open edge: malformed polytope
     if (e.d2 > kOuter) {
       nd = normalized(d3{e.wx, e.wy, e.wz});
-      nv = md.support(nd);
+      nv = md.supportFrom(nd, -1, -1, nh0, nh1);
       if (P.outside(f0, nv)) { start = f0; return 0; }
       if (P.outside(f1, nv)) { start = f1; return 0; }
     }
@@ -855,12 +939,18 @@ MFCL_D bool epaTerminate(const MFCL_LDS RawF& f, d3 nv, double tol) {
 }
 
 // epa.hpp:341-498
+// support0 / support1 of a recorded direction come from the recorded
+// support vertices when there are (the same points the support searches
+// returned when the polytope vertex was created).
 template <class MK>
-__device__ __attribute__((noinline)) void assignPair(MK md, d3 cand_d, const MFCL_LDS RawF& f, EpaOut& o) {
+__device__ __attribute__((noinline)) void assignPair(MK md, d3 cand_d, int ch0, int ch1, const MFCL_LDS RawF& f,
+                                                     EpaOut& o) {
   ScanTally tally(md, o);
+  auto p0 = [&](int k, const MFCL_LDS MVD& m) { return md.point0At(f.h[k][0], ldsGet(m.d)); };
+  auto p1 = [&](int k, const MFCL_LDS MVD& m) { return md.point1At(f.h[k][1], ldsGet(m.d)); };
   if (f.kind == 1) {
-    o.p0 = md.support0(ldsGet(f.a.d));
-    o.p1 = md.support1(-ldsGet(f.a.d));
+    o.p0 = p0(0, f.a);
+    o.p1 = p1(0, f.a);
     o.depth = norm(ldsGet(f.a.v));
     return;
   }
@@ -873,15 +963,15 @@ __device__ __attribute__((noinline)) void assignPair(MK md, d3 cand_d, const MFC
     if (fabs(ab.y) > mx) { ax = 1; mx = fabs(ab.y); }
     if (fabs(ab.z) > mx) { ax = 2; mx = fabs(ab.z); }
     if (mx <= 1e-10) {
-      o.p0 = md.support0(ldsGet(f.a.d));
-      o.p1 = md.support1(-ldsGet(f.a.d));
+      o.p0 = p0(0, f.a);
+      o.p1 = p1(0, f.a);
       return;
     }
     const double s = (comp(p, ax) - comp(a, ax)) / (comp(b, ax) - comp(a, ax));
     if (!(s < 0 || s > 1.0)) {
       const double wa = 1.0 - s, wb = s;
-      o.p0 = wa * md.support0(ldsGet(f.a.d)) + wb * md.support0(ldsGet(f.b.d));
-      o.p1 = wa * md.support1(-ldsGet(f.a.d)) + wb * md.support1(-ldsGet(f.b.d));
+      o.p0 = wa * p0(0, f.a) + wb * p0(1, f.b);
+      o.p1 = wa * p1(0, f.a) + wb * p1(1, f.b);
       return;
     }
   } else if (f.kind == 3) {
@@ -894,84 +984,172 @@ __device__ __attribute__((noinline)) void assignPair(MK md, d3 cand_d, const MFC
     const double chk = norm(cross(dl0, a - p)) / area;
     const double w2 = 1.0 - w0 - w1;
     if (!(fabs(chk - w2) > 0.01)) {
-      o.p0 = w0 * md.support0(ldsGet(f.a.d)) + w1 * md.support0(ldsGet(f.b.d)) + w2 * md.support0(ldsGet(f.c.d));
-      o.p1 = w0 * md.support1(-ldsGet(f.a.d)) + w1 * md.support1(-ldsGet(f.b.d)) + w2 * md.support1(-ldsGet(f.c.d));
+      o.p0 = w0 * p0(0, f.a) + w1 * p0(1, f.b) + w2 * p0(2, f.c);
+      o.p1 = w0 * p1(0, f.a) + w1 * p1(1, f.b) + w2 * p1(2, f.c);
       return;
     }
   }
-  const d3 q0 = md.support0(cand_d);
-  const d3 q1 = md.support1(-cand_d);
+  const d3 q0 = md.point0At(ch0, cand_d);
+  const d3 q1 = md.point1At(ch1, cand_d);
   o.p0 = q0;
   o.p1 = q1;
   o.depth = norm(q0 - q1);
 }
 
-// epa.hpp:137-257
+// Copies an LDS polytope into a device-memory one of larger capacity: the
+// same slot numbers, the extra slots free.  Slot numbers never influence a
+// result, so continuing there equals a run at the larger capacity from the
+// start.
+template <class R>
+MFCL_D void copyRecord(R* dst, const MFCL_LDS R* src) {
+  static_assert(sizeof(R) % 4 == 0, "records copy as words");
+  const MFCL_LDS uint32_t* a = reinterpret_cast<const MFCL_LDS uint32_t*>(src);
+  uint32_t* b = reinterpret_cast<uint32_t*>(dst);
+#pragma unroll
+  for (int k = 0; k < static_cast<int>(sizeof(R) / 4); ++k) b[k] = a[k];
+}
+MFCL_D void spillPoly(const Poly& L, PolyG& G) {
+  const int l = __lane_id();
+  for (int s = l; s < G.nv; s += 64) {
+    if (s < L.nv) {
+      copyRecord(&G.V[s], &L.V[s]);
+      copyRecord(&G.E[s], &L.E[s]);
+    } else {
+      G.V[s].stamp = -1;
+      G.E[s].stamp = -1;
+    }
+  }
+  for (int s = l; s < G.nf; s += 64) {
+    if (s < L.nf) copyRecord(&G.F[s], &L.F[s]);
+    else G.F[s].stamp = -1;
+  }
+  G.stamp = L.stamp;
+  GlbMem::sync();
+}
+
+// Where a pair continues when its LDS polytope might not hold an
+// expansion: a device-memory polytope arena (null: nowhere) of capacity
+// `faces` faces.
+struct SpillTarget {
+  char* base;
+  int faces;
+};
+
 template <class MK>
-MFCL_D void epaEvaluate(Poly& P, MK& md, const SmpD& s, double tol, int max_iter, EpaOut& o,
-                        MFCL_LDS RawF& raw) {
-  o.iters = 0;
-  o.scanned_extra = 0;
-  o.depth = 0;
-  o.p0 = d3{0, 0, 0};
-  o.p1 = d3{0, 0, 0};
-  const int s2p = simplexToPolytope(P, md, s, tol, o);
-  if (s2p == 2) { o.status = kEpaFailed; return; }
-  if (s2p == 1) { o.depth = 0.0; o.status = kEpaTouching; return; }
-  int it = 0;
+__device__ __attribute__((noinline)) EpaOut epaContinue(Poly& P, MK md, double tol, int max_iter, EpaOut o,
+                                                        MFCL_LDS RawF& raw, int it, d3 nv, d3 nd, int nh0, int nh1,
+                                                        int16_t start, SpillTarget st);
+
+// The EPA iterations (epa.hpp:137-237) from iteration `it`; with `pending`
+// the iteration's expansion (nv, nd, start) is still to run.  A polytope in
+// LDS whose expansion may not fit (expand() == 3, only with a spill target)
+// continues in a device-memory polytope of full capacity (epaContinue, out
+// of line: it runs for few pairs and keeps its registers out of this loop).
+template <bool SPILL, class PT, class MK>
+MFCL_D void epaLoop(PT& P, MK& md, double tol, int max_iter, EpaOut& o, MFCL_LDS RawF& raw, int it, bool pending,
+                    d3 nv, d3 nd, int nh0, int nh1, int16_t start, SpillTarget st) {
   while (true) {
-    int kind;
-    int16_t fi;
-    P.minToOrigin(false, kind, fi);
-    if (kind == 0) { o.status = kEpaFailed; return; }
-    if (kind == 1) {
-      const double vd2 = sqnorm(P.vloc(fi));
-      constexpr double kRatio = 1.0 - 1e-3;
-      int k2;
-      int16_t i2;
-      P.minToOrigin(true, k2, i2);
-      if (k2 == 0) { o.status = kEpaFailed; return; }
-      if (k2 == 2) {
-        if (kRatio * P.E[i2].d2 < vd2) { kind = 2; fi = i2; }
-      } else if (k2 == 3) {
-        if (kRatio * P.F[i2].d2 < vd2) { kind = 3; fi = i2; }
+    if (!pending) {
+      int kind;
+      int16_t fi;
+      P.minToOrigin(false, kind, fi);
+      if (kind == 0) { o.status = kEpaFailed; return; }
+      if (kind == 1) {
+        const double vd2 = sqnorm(P.vloc(fi));
+        constexpr double kRatio = 1.0 - 1e-3;
+        int k2;
+        int16_t i2;
+        P.minToOrigin(true, k2, i2);
+        if (k2 == 0) { o.status = kEpaFailed; return; }
+        if (k2 == 2) {
+          if (kRatio * P.E[i2].d2 < vd2) { kind = 2; fi = i2; }
+        } else if (k2 == 3) {
+          if (kRatio * P.F[i2].d2 < vd2) { kind = 3; fi = i2; }
+        }
+      }
+      if (kind != 2 && kind != 3) { o.status = kEpaFailed; return; }
+      toRaw(P, kind, fi, raw);
+      nd = d3{0, 0, 0};
+      nv = d3{0, 0, 0};
+      start = kNil;
+      nh0 = nh1 = -1;
+      const int ds = nextDir(P, md, kind, fi, tol, nd, nv, start, nh0, nh1);
+      if (ds == 1) { o.status = kEpaFailed; return; }
+      if (ds == 2) {
+        assignPair(md, nd, nh0, nh1, raw, o);
+        o.iters = it;
+        o.status = kEpaOK;
+        return;
+      }
+      if (start == kNil) { o.status = kEpaFailed; return; }
+      if (epaTerminate(raw, nv, tol)) {
+        assignPair(md, nd, nh0, nh1, raw, o);
+        o.iters = it;
+        o.status = kEpaOK;
+        return;
       }
     }
-    if (kind != 2 && kind != 3) { o.status = kEpaFailed; return; }
-    toRaw(P, kind, fi, raw);
-    d3 nd = {0, 0, 0}, nv = {0, 0, 0};
-    int16_t start = kNil;
-    const int ds = nextDir(P, md, kind, fi, tol, nd, nv, start);
-    if (ds == 1) { o.status = kEpaFailed; return; }
-    if (ds == 2) {
-      assignPair(md, nd, raw, o);
-      o.iters = it;
-      o.status = kEpaOK;
-      return;
+    pending = false;
+    const int ex = P.expand(nv, nd, nh0, nh1, start, SPILL && st.base != nullptr);
+    if constexpr (SPILL && std::is_same<PT, Poly>::value) {
+      if (ex == 3) {
+        o = epaContinue(P, md, tol, max_iter, o, raw, it, nv, nd, nh0, nh1, start, st);
+        return;
+      }
     }
-    if (start == kNil) { o.status = kEpaFailed; return; }
-    if (epaTerminate(raw, nv, tol)) {
-      assignPair(md, nd, raw, o);
-      o.iters = it;
-      o.status = kEpaOK;
-      return;
-    }
-    const int ex = P.expand(nv, nd, start);
     if (ex == 1) { o.status = kEpaFailed; return; }
     if (ex == 2) {
-      assignPair(md, nd, raw, o);
+      assignPair(md, nd, nh0, nh1, raw, o);
       o.iters = it;
       o.status = kEpaMallocFailed;
       return;
     }
     ++it;
     if (it >= max_iter) {
-      assignPair(md, nd, raw, o);
+      assignPair(md, nd, nh0, nh1, raw, o);
       o.iters = it;
       o.status = kEpaIterLimit;
       return;
     }
   }
+}
+
+template <class MK>
+__device__ __attribute__((noinline)) EpaOut epaContinue(Poly& P, MK md, double tol, int max_iter, EpaOut o,
+                                                        MFCL_LDS RawF& raw, int it, d3 nv, d3 nd, int nh0, int nh1,
+                                                        int16_t start, SpillTarget st) {
+  const ArenaDims fd = arenaDims(st.faces);
+  PolyG G;
+  G.V = reinterpret_cast<EVert*>(st.base);
+  G.E = reinterpret_cast<EEdge*>(st.base + sizeof(EVert) * fd.nv);
+  G.F = reinterpret_cast<EFace*>(st.base + sizeof(EVert) * fd.nv + sizeof(EEdge) * fd.nv);
+  G.BL = reinterpret_cast<BorderRec*>(st.base + sizeof(EVert) * fd.nv + sizeof(EEdge) * fd.nv +
+                                      sizeof(EFace) * fd.nf);
+  G.nv = fd.nv;
+  G.nf = fd.nf;
+  spillPoly(P, G);
+  o.spilled = 1;
+  const int scanned0 = md.scanned;
+  epaLoop<false>(G, md, tol, max_iter, o, raw, it, true, nv, nd, nh0, nh1, start, SpillTarget{nullptr, 0});
+  o.scanned_extra += md.scanned - scanned0;
+  return o;
+}
+
+// epa.hpp:137-257.  st: where the pair continues if P (of smaller capacity
+// than the request's) might not hold an expansion.
+template <bool SPILL, class MK>
+MFCL_D void epaEvaluate(Poly& P, MK& md, const SmpD& s, double tol, int max_iter, EpaOut& o,
+                        MFCL_LDS RawF& raw, SpillTarget st) {
+  o.iters = 0;
+  o.scanned_extra = 0;
+  o.spilled = 0;
+  o.depth = 0;
+  o.p0 = d3{0, 0, 0};
+  o.p1 = d3{0, 0, 0};
+  const int s2p = simplexToPolytope(P, md, s, tol, o);
+  if (s2p == 2) { o.status = kEpaFailed; return; }
+  if (s2p == 1) { o.depth = 0.0; o.status = kEpaTouching; return; }
+  epaLoop<SPILL>(P, md, tol, max_iter, o, raw, 0, false, d3{0, 0, 0}, d3{0, 0, 0}, -1, -1, kNil, st);
 }
 
 }  // namespace mfcl

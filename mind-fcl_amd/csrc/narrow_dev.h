@@ -230,7 +230,33 @@ struct Walk {
   float3 w[kWalkChunk];
 };
 
-MFCL_D void walkInit(Walk& W, const DevShape& s, d3 d, bool active) {
+// Walk starts: the reference seeds every walk from the best of six axis-
+// direction caches (convex-inl.h:151-199); a start >= 0 replaces that seed
+// with a given vertex (the caller's earlier support point for a nearby
+// direction).  On a convex hull a walk ends where no neighbour improves the
+// support value, which is the unique maximiser whatever the start; only
+// exact ties between support values (a face orthogonal to the direction)
+// can make the end depend on the start.
+MFCL_D int walkSeed(const DevShape& s, d3 d) {
+  int init = -1;
+  double max_dot = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    // the seed directions of convex-inl.h:151-199: +-UnitX, +-UnitY, +-UnitZ
+    const double sg = (i & 1) ? -1.0 : 1.0;
+    const double ax = (i >> 1) == 0 ? sg : sg * 0.0;
+    const double ay = (i >> 1) == 1 ? sg : sg * 0.0;
+    const double az = (i >> 1) == 2 ? sg : sg * 0.0;
+    const double dv = ax * d.x + ay * d.y + az * d.z;
+    if (init < 0 || dv > max_dot) {
+      init = s.seed[i];
+      max_dot = dv;
+    }
+  }
+  return init;
+}
+
+MFCL_D void walkInit(Walk& W, const DevShape& s, d3 d, bool active, int start = -1) {
   W.d = d;
   W.keep = active;
   W.visited = 0;
@@ -239,8 +265,9 @@ MFCL_D void walkInit(Walk& W, const DevShape& s, d3 d, bool active) {
   W.start = 0;
   W.count = 0;
   if (!active) return;
-  int init = -1;
+  int init = start;
   double max_dot = 0;
+  if (init < 0)
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     // the seed directions of convex-inl.h:151-199: +-UnitX, +-UnitY, +-UnitZ
@@ -350,7 +377,7 @@ struct BlkState {
   bool keep;
 };
 
-MFCL_D void blkInit(BlkState& W, const DevShape& s, d3 d, bool active) {
+MFCL_D void blkInit(BlkState& W, const DevShape& s, d3 d, bool active, int start = -1) {
   W.d = d;
   W.keep = active;
   W.visited = 0;
@@ -358,8 +385,9 @@ MFCL_D void blkInit(BlkState& W, const DevShape& s, d3 d, bool active) {
   W.ext_v = 0;
   W.B.cnt = 0;
   if (!active) return;
-  int init = -1;
+  int init = start;
   double max_dot = 0;
+  if (init < 0)
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     // the seed directions of convex-inl.h:151-199: +-UnitX, +-UnitY, +-UnitZ
@@ -430,9 +458,9 @@ MFCL_D void blkStep(BlkState& W, const DevShape& s) {
   if (W.keep) blkLoad(s, W.ext, W.B);
 }
 
-MFCL_D int hullWalkBlk(const DevShape& s, d3 d, int* scanned) {
+MFCL_D int hullWalkBlk(const DevShape& s, d3 d, int* scanned, int start = -1) {
   BlkState W;
-  blkInit(W, s, d, true);
+  blkInit(W, s, d, true, start);
   while (W.keep) blkStep(W, s);
   *scanned += W.visited;
   return W.ext;
@@ -440,10 +468,10 @@ MFCL_D int hullWalkBlk(const DevShape& s, d3 d, int* scanned) {
 
 // The walk over blocks when the hull has them and is not staged in LDS,
 // else over the adjacency table.
-MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
-  if (s.wblk && !s.lv) return hullWalkBlk(s, d, scanned);
+MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned, int start = -1) {
+  if (s.wblk && !s.lv) return hullWalkBlk(s, d, scanned, start);
   Walk W;
-  walkInit(W, s, d, true);
+  walkInit(W, s, d, true, start);
   while (W.keep) {
     walkFetchRecord(W, s);
     walkFetchNeighbours(W, s);
@@ -456,11 +484,11 @@ MFCL_D int hullWalk(const DevShape& s, d3 d, int* scanned) {
 // Two walks (either may be inactive) advanced together; ia / ib receive
 // the extreme vertices of the active ones.
 MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevShape& sb, d3 db, bool act_b,
-                      int& ib, int* scanned) {
+                      int& ib, int* scanned, int start_a = -1, int start_b = -1) {
   if (sa.wblk && !sa.lv && sb.wblk && !sb.lv) {
     BlkState A, B;
-    blkInit(A, sa, da, act_a);
-    blkInit(B, sb, db, act_b);
+    blkInit(A, sa, da, act_a, start_a);
+    blkInit(B, sb, db, act_b, start_b);
     while (A.keep || B.keep) {
       blkStep(A, sa);
       blkStep(B, sb);
@@ -471,8 +499,8 @@ MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevS
     return;
   }
   Walk A, B;
-  walkInit(A, sa, da, act_a);
-  walkInit(B, sb, db, act_b);
+  walkInit(A, sa, da, act_a, start_a);
+  walkInit(B, sb, db, act_b, start_b);
   while (A.keep || B.keep) {
     walkFetchRecord(A, sa);
     walkFetchRecord(B, sb);
@@ -487,8 +515,8 @@ MFCL_D void hullWalk2(const DevShape& sa, d3 da, bool act_a, int& ia, const DevS
 }
 
 template <int G>
-MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned) {
-  if (s.nbr) return hullWalk(s, d, scanned);
+MFCL_D int hullArgmax(const DevShape& s, d3 d, int lg, int* scanned, int start = -1) {
+  if (s.nbr) return hullWalk(s, d, scanned, start);
   const int r = hullArgmaxOOL<G>(s.verts, s.lv, s.ls, s.nv, s.rbound, d.x, d.y, d.z, lg);
   *scanned += (r >> 30) ? 2 * s.nv : s.nv;
   return r & ((1 << 30) - 1);
@@ -532,9 +560,9 @@ MFCL_D d3 shapeSupport(const DevShape& s, d3 dir, int lg, int* scanned) {
 
 // shapeSupport plus the index of the hull vertex returned (-1 for primitives).
 template <int G>
-MFCL_D d3 shapeSupportI(const DevShape& s, d3 dir, int lg, int* scanned, int& idx) {
+MFCL_D d3 shapeSupportI(const DevShape& s, d3 dir, int lg, int* scanned, int& idx, int start = -1) {
   if (s.type == kConvex) {
-    idx = hullArgmax<G>(s, dir, lg, scanned);
+    idx = hullArgmax<G>(s, dir, lg, scanned, start);
     const float3 q = hullVert(s, idx);
     return {q.x, q.y, q.z};
   }
@@ -673,20 +701,44 @@ struct MinkL {
   }
   // support points of both shapes (vertex indices, -1 for a primitive);
   // two walking hulls walk together
-  MFCL_D void supportPair(d3 d, d3& a, d3& x, int& i0, int& i1) {
+  // st0 / st1: walk starts (-1: the reference's seed)
+  MFCL_D void supportPair(d3 d, d3& a, d3& x, int& i0, int& i1, int st0 = -1, int st1 = -1) {
     reloadFence();
     const DevShape s0 = sh0(), s1 = sh1();
     const d3 d1 = mul(rot1(), -d);
     if (s0.nbr && s1.nbr) {
-      hullWalk2(s0, d, true, i0, s1, d1, true, i1, &scanned);
+      hullWalk2(s0, d, true, i0, s1, d1, true, i1, &scanned, st0, st1);
       const float3 q0 = hullVert(s0, i0), q1 = hullVert(s1, i1);
       a = d3{q0.x, q0.y, q0.z};
       x = d3{q1.x, q1.y, q1.z};
       return;
     }
-    a = shapeSupportI<G>(s0, d, lg, &scanned, i0);
+    a = shapeSupportI<G>(s0, d, lg, &scanned, i0, st0);
     reloadFence();
-    x = shapeSupportI<G>(sh1(), d1, lg, &scanned, i1);
+    x = shapeSupportI<G>(sh1(), d1, lg, &scanned, i1, st1);
+  }
+  // support(d) with walk starts; i0 / i1 receive the support vertices
+  // (-1 for a primitive)
+  MFCL_D d3 supportFrom(d3 d, int st0, int st1, int& i0, int& i1) {
+    d3 a, x;
+    supportPair(d, a, x, i0, i1, st0, st1);
+    reloadFence();
+    return a - apply(tf0(), x);
+  }
+  // support0(d) / support1(-d) from a known support vertex when there is one
+  MFCL_D d3 point0At(int i0, d3 d) {
+    if (i0 >= 0) {
+      const float3 q = hullVert(sh0(), i0);
+      return d3{q.x, q.y, q.z};
+    }
+    return support0(d);
+  }
+  MFCL_D d3 point1At(int i1, d3 d) {
+    if (i1 >= 0) {
+      const float3 q = hullVert(sh1(), i1);
+      return apply(tf0(), d3{q.x, q.y, q.z});
+    }
+    return support1(-d);
   }
   MFCL_D d3 support(d3 d) {
     d3 a, x;

@@ -492,10 +492,21 @@ constexpr size_t kRawBytes = (sizeof(RawF) + 15) & ~static_cast<size_t>(15);
 constexpr size_t kPolyBytes = (sizeof(Poly) + 15) & ~static_cast<size_t>(15);
 inline size_t epaSliceBytes(int cap_faces) { return kCfgBytes + kRawBytes + kPolyBytes + arenaBytes(cap_faces); }
 
+// Wavefront w of the grid may continue a pair that outgrows its LDS
+// polytope in the device-memory polytope arena + w * arena_slice (full
+// capacity) when w < arena_slices; the others rerun such a pair.
+struct EpaArena {
+  char* base;
+  uint32_t slices;
+  size_t slice;
+  uint32_t* spilled;  // pairs continued in device memory
+};
+
+template <bool SPILL>
 __global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO io, Params P, const QEntry* queue,
                                                         const uint32_t* qcount, int cap_faces, size_t slice_bytes,
                                                         QEntry* overflow, uint32_t* ocount, uint32_t* next,
-                                                        unsigned long long* scanned_total) {
+                                                        unsigned long long* scanned_total, EpaArena ga) {
   extern __shared__ __align__(16) char lds_arena[];
   const int lane = threadIdx.x & 63;
   const ArenaDims dims = arenaDims(cap_faces);
@@ -514,6 +525,10 @@ __global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO
                                   sizeof(EFace) * dims.nf);
   poly.nv = dims.nv;
   poly.nf = dims.nf;
+  // this wavefront's device-memory polytope of the request's full capacity
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const bool can_spill = cap_faces < P.epa_max_faces && ga.base != nullptr && wave < ga.slices;
+  const SpillTarget st{can_spill ? ga.base + static_cast<size_t>(wave) * ga.slice : nullptr, P.epa_max_faces};
   const uint32_t count = *qcount;
   int scanned = 0;
   // pairs are taken one at a time from a shared counter: EPA iteration
@@ -536,8 +551,9 @@ __global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO
     writeCfg(cfg, A, ta, B, tb, d3{0, 0, 0}, d3{0, 0, 0});
     MinkL<64> md{cfg, lane, 0};
     EpaOut eo;
-    epaEvaluate(poly, md, e.s, P.epa_tol, P.epa_max_it, eo, *raw);
+    epaEvaluate<SPILL>(poly, md, e.s, P.epa_tol, P.epa_max_it, eo, *raw, st);
     scanned += md.scanned + eo.scanned_extra;
+    if (eo.spilled && lane == 0) atomicAdd(ga.spilled, 1u);
     if (eo.status == kEpaMallocFailed && cap_faces < P.epa_max_faces) {
       if (lane == 0) {
         const uint32_t o = atomicAdd(ocount, 1u);
@@ -698,23 +714,24 @@ static size_t laneDirBytes(uint64_t n) {
   const uint64_t waves = (n + kBlock - 1) / kBlock * (kBlock / 64);
   return static_cast<size_t>(waves) * kLaneDirDoubles * sizeof(double);
 }
-size_t convexWsBytes(uint64_t n) {
+size_t epaArenaBytes(int slices, int max_faces) { return static_cast<size_t>(slices) * arenaBytes(max_faces); }
+size_t convexWsBytes(uint64_t n, size_t arena_bytes) {
   if (n == 0) n = 1;
   return align256(2048) + align256(4 * n) + align256(n) + 2 * align256(n * sizeof(QEntry)) +
-         align256(laneDirBytes(n));
+         align256(laneDirBytes(n)) + align256(arena_bytes);
 }
-uint64_t convexWsMaxPairs(size_t bytes) {
+uint64_t convexWsMaxPairs(size_t bytes, size_t arena_bytes) {
   // convexWsBytes grows by at least sizeof(QEntry) per pair: bisect
   uint64_t lo = 0, hi = bytes / (2 * sizeof(QEntry)) + 1;
   while (lo < hi) {
     const uint64_t mid = (lo + hi + 1) / 2;
-    if (convexWsBytes(mid) <= bytes) lo = mid;
+    if (convexWsBytes(mid, arena_bytes) <= bytes) lo = mid;
     else hi = mid - 1;
   }
   return lo;
 }
-bool carveConvexWs(void* base, size_t bytes, uint64_t n, ConvexWs& w) {
-  if (!base || convexWsBytes(n) > bytes) return false;
+bool carveConvexWs(void* base, size_t bytes, uint64_t n, size_t arena_bytes, ConvexWs& w) {
+  if (!base || convexWsBytes(n, arena_bytes) > bytes) return false;
   char* p = static_cast<char*>(base);
   auto take = [&](size_t b) {
     char* r = p;
@@ -727,6 +744,8 @@ bool carveConvexWs(void* base, size_t bytes, uint64_t n, ConvexWs& w) {
   w.queue = take((n ? n : 1) * sizeof(QEntry));
   w.oflow = take((n ? n : 1) * sizeof(QEntry));
   w.lane_dirs = reinterpret_cast<double*>(take(laneDirBytes(n ? n : 1)));
+  w.arena = arena_bytes ? take(arena_bytes) : nullptr;
+  w.arena_bytes = arena_bytes;
   return true;
 }
 
@@ -757,7 +776,9 @@ int staticScratch(int device, void** out) {
   return MFCL_OK;
 }
 
-uint64_t staticScratchMaxPairs() { return convexWsMaxPairs(kStaticScratchBytes); }
+uint64_t staticScratchMaxPairs() {
+  return convexWsMaxPairs(kStaticScratchBytes, epaArenaBytes(kStaticArenaSlices, 256));
+}
 
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
@@ -855,13 +876,24 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
     while (cap_small > 4 && 4 * epaSliceBytes(cap_small) > static_cast<size_t>(ctx->max_lds_per_block)) cap_small /= 2;
     const size_t slice = epaSliceBytes(cap_small);
     const unsigned eb = static_cast<unsigned>(ctx->tune.epa_threads / kBlock);
-    epaWaveKernel<<<eb, kBlock, 4 * slice, s>>>(T, io, P, queue, qcount, cap_small, slice, oflow, ocount, ocount + 1,
-                                                sc + 1);
-    // overflow pairs: full capacity, one wavefront per block
+    EpaArena ga{nullptr, 0, arenaBytes(P.epa_max_faces), ocount + 3};
+    if (ws.arena && cap_small < P.epa_max_faces && ctx->tune.epa_spill) {
+      ga.base = static_cast<char*>(ws.arena);
+      ga.slices = static_cast<uint32_t>(std::min<size_t>(ws.arena_bytes / ga.slice, 4u * eb));
+    }
+    if (ga.base)
+      epaWaveKernel<true><<<eb, kBlock, 4 * slice, s>>>(T, io, P, queue, qcount, cap_small, slice, oflow, ocount,
+                                                        ocount + 1, sc + 1, ga);
+    else
+      epaWaveKernel<false><<<eb, kBlock, 4 * slice, s>>>(T, io, P, queue, qcount, cap_small, slice, oflow, ocount,
+                                                         ocount + 1, sc + 1, ga);
+    // pairs that outgrew the LDS polytope on a wavefront without a device-
+    // memory polytope: rerun at full capacity, one wavefront per block
     if (cap_small < P.epa_max_faces) {
       const unsigned ob = static_cast<unsigned>(ctx->tune.epa_overflow_waves);
-      epaWaveKernel<<<ob, 64, full_slice, s>>>(T, io, P, oflow, ocount, P.epa_max_faces, full_slice, oflow, ocount, ocount + 2,
-                                               sc + 1);
+      EpaArena none{nullptr, 0, 0, ocount + 3};
+      epaWaveKernel<false><<<ob, 64, full_slice, s>>>(T, io, P, oflow, ocount, P.epa_max_faces, full_slice, oflow,
+                                                      ocount, ocount + 2, sc + 1, none);
     }
   }
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));

@@ -264,9 +264,10 @@ class Context:
 
     def last_counters(self) -> np.ndarray:
         """[pairs, hull vertices read (GJK + EPA stages), EPA pairs, EPA
-        reruns at full capacity, hull vertices read by the EPA stage]"""
-        c = np.zeros(5, np.uint64)
-        self._check(self.lib.mfcl_last_counters_n(self.h, _ptr(c), 5))
+        reruns at full capacity, hull vertices read by the EPA stage, EPA
+        pairs continued in a device-memory polytope of full capacity]"""
+        c = np.zeros(6, np.uint64)
+        self._check(self.lib.mfcl_last_counters_n(self.h, _ptr(c), 6))
         return c
 
     def reserve(self, kind: int, count: int):

@@ -21,6 +21,7 @@ MESH_BLOCKS_PER_CU = 3
 MESH_STACK = 4
 EPA_LDS_FACES = 5
 GUESS_POLICY = 6
+EPA_SPILL = 7
 
 
 def set_knob(ctx, knob: int, value: int) -> None:

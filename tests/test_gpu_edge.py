@@ -88,6 +88,34 @@ def test_solver_limits(oracle, gpu_ctx, setup, gjk_it, epa_faces, epa_it, group,
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("lds_faces,epa_threads", [(16, 131072), (24, 524288), (48, 131072)])
+def test_epa_continuation_and_rerun(oracle, gpu_ctx, setup, lds_faces, epa_threads):
+    """Pairs that outgrow the LDS polytope continue in a device-memory
+    polytope of full capacity (the first 2048 wavefronts of the EPA grid) or
+    rerun at full capacity (the others): results equal the oracle's at the
+    request's capacity, field for field."""
+    h, S, _, faces = setup
+    gpu_ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"], faces=faces)
+    ta, tb = oracle.pose12(h["pose_a"]), oracle.pose12(h["pose_b"])
+    ref, _ = S.signed_distance(h["pair_a"], h["pair_b"], ta, tb, threads=8)
+    autotune.set_knob(gpu_ctx, autotune.EPA_LDS_FACES, lds_faces)
+    autotune.set_knob(gpu_ctx, autotune.EPA_THREADS, epa_threads)
+    try:
+        out = gpu_ctx.convex_query(h["pair_a"], h["pair_b"], h["pose_a"], h["pose_b"],
+                                   mfcl.request(mfcl.MODE_SIGNED_DISTANCE))
+        c = gpu_ctx.last_counters()
+    finally:
+        autotune.set_knob(gpu_ctx, autotune.EPA_LDS_FACES, 64)
+        autotune.set_knob(gpu_ctx, autotune.EPA_THREADS, 131072)
+    diff = _compare(out, ref)
+    print(f"\n[epa lds_faces={lds_faces} threads={epa_threads}] EPA pairs {int(c[2])} continued in device memory "
+          f"{int(c[5])} rerun {int(c[3])} differing {diff}")
+    assert all(v == 0 for v in diff.values())
+    if lds_faces < 48:
+        assert c[5] + c[3] > 0
+
+
+@pytest.mark.gpu
 def test_empty_batch(gpu_ctx, setup):
     h, _, _, faces = setup
     gpu_ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"], faces=faces)
