@@ -434,7 +434,8 @@ def bench_mesh(ctx, args, threads):
     alg = 2 * 64 * bv + 2 * 36 * lf + 64 * n
     kms = float(np.mean(km))
     out = {"value": n * args.steps / el, "unit": "pairs/s", "pairs_per_launch": n, "kernel_ms": kms,
-           "hit_fraction": float(res["hit"].mean()), "bv_tests_mean": bv / n, "leaf_tests_mean": lf / n,
+           "hit_fraction": float((res["hit"] == 1).mean()), "overflow_pairs": int((res["hit"] < 0).sum()),
+           "bv_tests_mean": bv / n, "bv_tests_max": int(res["bv_tests"].max()), "leaf_tests_mean": lf / n,
            "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": alg},
            "workload": f"{n} mesh pairs over a pool of {args.mesh_pool} noisy icospheres "

@@ -107,6 +107,17 @@ typedef struct mfcl_contact {
  * box_box-inl.h:823-841; every other pair type produces one). */
 #define MFCL_MAX_PAIR_CONTACTS 4
 
+/* A contact of a penetration-enabled mesh-mesh query
+ * (mfcl_mesh_collide_contacts): the triangle ids b1, b2 and the Contact
+ * fields pos, normal and penetration_depth as the reference computes them
+ * (shape_pair_intersect-inl.h:226-254, intersect-inl.h:796-842).  64 bytes. */
+typedef struct mfcl_mesh_contact {
+  int32_t b1, b2;
+  double pos[3];
+  double normal[3];
+  double depth;
+} mfcl_mesh_contact;
+
 /* Mesh query result header, one per pair; contacts go to a separate
  * (n_pairs x max_contacts) array of (b1, b2) triangle index pairs. */
 typedef struct mfcl_mesh_result {
@@ -266,6 +277,22 @@ int mfcl_mesh_collide_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t*
                           const float* pose_a7, const float* pose_b7, uint64_t n,
                           int32_t max_contacts, mfcl_mesh_result* out, int32_t* contacts,
                           void* stream);
+/* The same with penetration enabled (CollisionRequest(max_contacts,
+ * enable_contact = true)): each colliding triangle pair adds as many
+ * contacts as intersect_Triangle's deepest points, 0 to 2
+ * (intersect-inl.h:796-842), truncated to the free slots
+ * (shape_pair_intersect-inl.h:226-254); the reference maps their position
+ * and normal to the world with the second mesh's transform (SimplexIntersect
+ * passes tf2 for both, shape_pair_intersect-inl.h:269), and so does this
+ * call.  out[i].hit = 1 when pair i has a contact.  contacts: n *
+ * max_contacts records; which contacts of a larger set are kept depends on
+ * the traversal order.  Host pointers. */
+int mfcl_mesh_collide_contacts(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                               const float* pose_a7, const float* pose_b7, uint64_t n, int32_t max_contacts,
+                               mfcl_mesh_result* out, mfcl_mesh_contact* contacts);
+int mfcl_mesh_collide_contacts_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                                   const float* pose_a7, const float* pose_b7, uint64_t n, int32_t max_contacts,
+                                   mfcl_mesh_result* out, mfcl_mesh_contact* contacts, void* stream);
 
 /* ---- broadphase AABB tree ----------------------------------------------- */
 /* Builds a flattened breadth-first AABB tree over n boxes (replaces

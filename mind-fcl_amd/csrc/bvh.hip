@@ -278,6 +278,7 @@ struct MeshArgs {
   int max_contacts;
   mfcl_mesh_result* out;
   int32_t* contacts;
+  mfcl_mesh_contact* pen_contacts;  // penetration mode: max_contacts records per query
   uint64_t* stacks;
   uint32_t stack_cap;
   uint32_t* next_query;
@@ -406,20 +407,136 @@ __device__ __forceinline__ int prefixBelow(unsigned long long m) {
   return __popcll(m & ((1ull << laneId()) - 1ull));
 }
 
-__global__ void __launch_bounds__(256) meshKernel(MeshArgs A) {
-  const int lane = laneId();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  uint64_t* stk = A.stacks + static_cast<uint64_t>(wave) * A.stack_cap;
+// intersect-inl.h:1036-1051 with normalize() of geometry-inl.h:188-197.
+MFCL_D bool buildTrianglePlaneD(d3 v1, d3 v2, d3 v3, d3& n, double& t) {
+  d3 n_ = cross(v2 - v1, v3 - v1);
+  const double sq = sqnorm(n_);
+  if (!(sq > 0)) return false;
+  const double l = sqrt(sq);
+  n_ = d3{n_.x / l, n_.y / l, n_.z / l};
+  n = n_;
+  t = dot(n_, v1);
+  return true;
+}
+
+// intersect-inl.h:845-882 (getEpsilon() = 1e-5, distanceToPlane = n.v - t).
+MFCL_D void deepestPointsD(const d3* pts, d3 n, double t, double& depth, d3* deepest, int& nd) {
+  constexpr double kEps = 1e-5;
+  double max_depth = -1.7976931348623157e308;
+  int k = 0, num_neg = 0, num_pos = 0, num_zero = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double dist = -(dot(n, pts[i]) - t);
+    if (dist > kEps) num_pos++;
+    else if (dist < -kEps) num_neg++;
+    else num_zero++;
+    if (dist > max_depth) {
+      max_depth = dist;
+      k = 1;
+      deepest[0] = pts[i];
+    } else if (dist + 1e-6 >= max_depth) {
+      k++;
+      deepest[k - 1] = pts[i];
+    }
+  }
+  if (max_depth < -kEps) k = 0;
+  if (num_zero == 0 && (num_neg == 0 || num_pos == 0)) k = 0;
+  depth = max_depth;
+  nd = k;
+}
+
+// Contacts of an intersecting triangle pair (intersect-inl.h:796-842): up to
+// two points in the first mesh's frame, the depth and the normal.
+MFCL_D void triContactsD(const float4* P, const float4* Qm, const m33& R, d3 T, d3 pts[2], int& npts, double& depth,
+                         d3& normal) {
+  const d3 P1 = {P[0].x, P[0].y, P[0].z}, P2 = {P[1].x, P[1].y, P[1].z}, P3 = {P[2].x, P[2].y, P[2].z};
+  const d3 Q1 = mul(R, d3{Qm[0].x, Qm[0].y, Qm[0].z}) + T;
+  const d3 Q2 = mul(R, d3{Qm[1].x, Qm[1].y, Qm[1].z}) + T;
+  const d3 Q3 = mul(R, d3{Qm[2].x, Qm[2].y, Qm[2].z}) + T;
+  d3 n1 = {0, 0, 0}, n2 = {0, 0, 0};
+  double t1 = 0, t2 = 0;
+  buildTrianglePlaneD(P1, P2, P3, n1, t1);
+  buildTrianglePlaneD(Q1, Q2, Q3, n2, t2);
+  const d3 Pv[3] = {P1, P2, P3}, Qv[3] = {Q1, Q2, Q3};
+  d3 d1[3], d2[3];
+  int k1 = 0, k2 = 0;
+  double pd1 = 0, pd2 = 0;
+  deepestPointsD(Qv, n1, t1, pd2, d2, k2);
+  deepestPointsD(Pv, n2, t2, pd1, d1, k1);
+  if (pd1 > pd2) {
+    npts = k2 < 2 ? k2 : 2;
+    pts[0] = d2[0];
+    pts[1] = d2[1];
+    normal = n1;
+    depth = pd2;
+  } else {
+    npts = k1 < 2 ? k1 : 2;
+    pts[0] = d1[0];
+    pts[1] = d1[1];
+    normal = -n2;
+    depth = pd1;
+  }
+}
+
+// One 256-thread block per query (queries from an atomic counter, a
+// persistent grid): each step pops up to 256 node pairs of the bounding-
+// volume test tree (bvh_solver-inl.h:75-164) off a block stack -- the top
+// kMeshLds entries in LDS, the rest in a per-block device-memory region --
+// tests them (FP32 OBB SAT with a rounding margin, FP64 triangle test), and
+// pushes children and contacts with block-wide prefix sums.  A whole block
+// per query cuts the number of dependent steps of the costliest queries,
+// which set the launch's duration.  Descent rule as the reference (the
+// larger inner node first); the pop order differs, so with a contact cap the
+// kept contacts are a subset of the full set rather than the reference's
+// first ones.  PEN: contacts with position, normal and depth (each
+// colliding triangle pair adds 0-2 of them, shape_pair_intersect-inl.h:
+// 226-254), and a query collides when it has a contact, as a
+// CollisionResult with contacts does.
+constexpr int kMeshBlock = 256;
+constexpr int kMeshLds = 3072;  // LDS stack entries per block
+
+// Exclusive prefix of v over the block (4 wavefronts) and the total; 64-bit
+// so several 16-bit counters travel in one scan.
+MFCL_D uint64_t blockScan(uint64_t v, uint64_t& total, uint64_t* wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint64_t base = 0;
+  total = 0;
+#pragma unroll
+  for (int k = 0; k < kMeshBlock / 64; ++k) {
+    const uint64_t c = wsum[k];
+    if (k < w) base += c;
+    total += c;
+  }
+  __syncthreads();
+  return base + x - v;
+}
+
+template <bool PEN>
+__global__ void __launch_bounds__(kMeshBlock, 3) meshKernel(MeshArgs A) {
+  __shared__ uint64_t lstk[kMeshLds];
+  __shared__ uint64_t wsum[kMeshBlock / 64];
+  __shared__ uint32_t s_q;
+  const int t = threadIdx.x;
+  uint64_t* gstk = A.stacks + static_cast<uint64_t>(blockIdx.x) * A.stack_cap;
   const float kScaleErr = 16.0f * 5.9604645e-08f;
   unsigned long long tot_bv = 0, tot_leaf = 0;
   while (true) {
-    uint32_t q = 0;
-    if (lane == 0) q = atomicAdd(A.next_query, 1u);
-    q = __shfl(q, 0);
+    if (t == 0) s_q = atomicAdd(A.next_query, 1u);
+    __syncthreads();
+    const uint32_t q = s_q;
+    __syncthreads();
     if (q >= A.n) break;
     const uint32_t ma = A.pa[q], mb = A.pb[q];
     if (ma >= A.n_meshes || mb >= A.n_meshes) {
-      if (lane == 0) A.out[q] = mfcl_mesh_result{-1, -1, 0, 0};
+      if (t == 0) A.out[q] = mfcl_mesh_result{-1, -1, 0, 0};
       continue;
     }
     const uint32_t base1 = A.node_base[ma], base2 = A.node_base[mb];
@@ -432,74 +549,110 @@ __global__ void __launch_bounds__(256) meshKernel(MeshArgs A) {
     const float Rq[9] = {(float)R.m00, (float)R.m01, (float)R.m02, (float)R.m10, (float)R.m11,
                          (float)R.m12, (float)R.m20, (float)R.m21, (float)R.m22};
     const float Tq[3] = {(float)T.x, (float)T.y, (float)T.z};
-    int top = 1;
-    if (lane == 0) stk[0] = 0ull;
-    int ncont = 0;
-    int bv = 0, leaf = 0;
+    tf3 tf2;
+    if constexpr (PEN) tf2 = poseFromQuat(A.poseb + 7ull * q);
+    // block-uniform traversal state, kept identical in every thread
+    int top = 1, gtop = 0, ncont = 0, bv = 0, leaf = 0;
     bool overflow = false;
+    if (t == 0) lstk[0] = 0ull;
+    __syncthreads();
     int32_t* cout = A.contacts + 2ull * A.max_contacts * q;
-    while (top > 0 && ncont < A.max_contacts) {
-      // make this wave's stack stores visible to its next loads
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-      const int k = top < 64 ? top : 64;
-      const int b0 = top - k;
-      const bool act = lane < k;
-      uint64_t e = act ? stk[b0 + lane] : 0ull;
-      top = b0;
-      const uint32_t n1 = static_cast<uint32_t>(e >> 32), n2 = static_cast<uint32_t>(e & 0xFFFFFFFFu);
-      bool is_leafpair = false, push1 = false, push2 = false;
-      int32_t c1 = 0, c2 = 0;
-      const bool bad = act && (n1 >= cnt1 || n2 >= cnt2);
-      if (ballot64(bad)) {
-        overflow = true;
-        break;
+    mfcl_mesh_contact* pout = A.pen_contacts + static_cast<uint64_t>(A.max_contacts) * q;
+    while (top + gtop > 0 && ncont < A.max_contacts) {
+      if (top == 0) {
+        // refill the LDS stack from the device-memory part
+        const int k = gtop < kMeshBlock ? gtop : kMeshBlock;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        if (t < k) lstk[t] = gstk[gtop - k + t];
+        __syncthreads();
+        gtop -= k;
+        top = k;
       }
+      const int k = top < kMeshBlock ? top : kMeshBlock;
+      const bool act = t < k;
+      const uint64_t e = act ? lstk[top - k + t] : 0ull;
+      top -= k;
+      const uint32_t n1 = static_cast<uint32_t>(e >> 32), n2 = static_cast<uint32_t>(e & 0xFFFFFFFFu);
+      bool is_leafpair = false, push1 = false, push2 = false, bad = false;
+      int32_t c1 = 0, c2 = 0;
       if (act) {
-        const ObbNode a = A.nodes[base1 + n1];
-        const ObbNode b = A.nodes[base2 + n2];
-        if (obbOverlapF(Rq, Tq, a, b, kScaleErr)) {
-          const bool la = a.child < 0, lb = b.child < 0;
-          c1 = a.child;
-          c2 = b.child;
-          if (la && lb) {
-            is_leafpair = true;
-          } else {
-            const float sa = a.e[0] * a.e[0] + a.e[1] * a.e[1] + a.e[2] * a.e[2];
-            const float sb = b.e[0] * b.e[0] + b.e[1] * b.e[1] + b.e[2] * b.e[2];
-            if (lb || (!la && sa > sb)) push1 = true;
-            else push2 = true;
+        bad = n1 >= cnt1 || n2 >= cnt2;
+        if (!bad) {
+          const ObbNode a = A.nodes[base1 + n1];
+          const ObbNode b = A.nodes[base2 + n2];
+          if (obbOverlapF(Rq, Tq, a, b, kScaleErr)) {
+            const bool la = a.child < 0, lb = b.child < 0;
+            c1 = a.child;
+            c2 = b.child;
+            if (la && lb) {
+              is_leafpair = true;
+            } else {
+              const float sa = a.e[0] * a.e[0] + a.e[1] * a.e[1] + a.e[2] * a.e[2];
+              const float sb = b.e[0] * b.e[0] + b.e[1] * b.e[1] + b.e[2] * b.e[2];
+              if (lb || (!la && sa > sb)) push1 = true;
+              else push2 = true;
+            }
           }
         }
       }
       bv += k;
-      // leaf tests (FP64), then compact hits into the contact list
-      bool hit = false;
-      int t1 = 0, t2 = 0;
+      // leaf tests (FP64); the number of contacts each lane adds
+      int t1 = 0, t2 = 0, nadd = 0;
+      d3 pts[2], nrm = {0, 0, 0};
+      double depth = 0;
       if (is_leafpair) {
         t1 = -c1 - 1;
         t2 = -c2 - 1;
-        hit = triTriD(tri1 + 3 * t1, tri2 + 3 * t2, R, T);
-      }
-      leaf += __popcll(ballot64(is_leafpair));
-      const unsigned long long hm = ballot64(hit);
-      if (hm) {
-        const int slot = ncont + prefixBelow(hm);
-        if (hit && slot < A.max_contacts) {
-          cout[2 * slot] = t1;
-          cout[2 * slot + 1] = t2;
+        if (triTriD(tri1 + 3 * t1, tri2 + 3 * t2, R, T)) {
+          if constexpr (PEN) triContactsD(tri1 + 3 * t1, tri2 + 3 * t2, R, T, pts, nadd, depth, nrm);
+          else nadd = 1;
         }
-        ncont += __popcll(hm);
-        if (ncont > A.max_contacts) ncont = A.max_contacts;
       }
-      // push children (two per expanding lane)
-      const unsigned long long pm = ballot64(push1 || push2);
-      const int total = 2 * __popcll(pm);
-      if (top + total > static_cast<int>(A.stack_cap)) {
+      // one scan for four counters (16 bits each): contacts added, pushes,
+      // leaf tests, bad node ids
+      const bool pushes = push1 || push2;
+      const uint64_t mine = static_cast<uint64_t>(nadd) | (static_cast<uint64_t>(pushes ? 1 : 0) << 16) |
+                            (static_cast<uint64_t>(is_leafpair ? 1 : 0) << 32) |
+                            (static_cast<uint64_t>(bad ? 1 : 0) << 48);
+      uint64_t tot = 0;
+      const uint64_t pre = blockScan(mine, tot, wsum);
+      const int n_add = static_cast<int>(tot & 0xFFFF), n_push = static_cast<int>((tot >> 16) & 0xFFFF);
+      leaf += static_cast<int>((tot >> 32) & 0xFFFF);
+      if ((tot >> 48) & 0xFFFF) {
         overflow = true;
         break;
       }
-      if (push1 || push2) {
-        const int pos = top + 2 * prefixBelow(pm);
+      const int slot0 = ncont + static_cast<int>(pre & 0xFFFF);
+      for (int j = 0; j < nadd; ++j) {
+        const int slot = slot0 + j;
+        if (slot >= A.max_contacts) break;
+        if (A.contacts) {
+          cout[2 * slot] = t1;
+          cout[2 * slot + 1] = t2;
+        }
+        if constexpr (PEN) {
+          // the reference maps them with the second mesh's transform
+          // (shape_pair_intersect-inl.h:244-248 with tf1 = tf2 passed at :269)
+          const d3 pw = apply(tf2, pts[j]), nw = mul(tf2.R, nrm);
+          mfcl_mesh_contact c;
+          c.b1 = t1;
+          c.b2 = t2;
+          c.pos[0] = pw.x; c.pos[1] = pw.y; c.pos[2] = pw.z;
+          c.normal[0] = nw.x; c.normal[1] = nw.y; c.normal[2] = nw.z;
+          c.depth = depth;
+          pout[slot] = c;
+        }
+      }
+      ncont += n_add;
+      if (ncont > A.max_contacts) ncont = A.max_contacts;
+      // pushes: two per expanding thread, LDS first, the excess to device memory
+      const int ppos = 2 * static_cast<int>((pre >> 16) & 0xFFFF);
+      const int total = 2 * n_push;
+      if (gtop + (top + total > kMeshLds ? top + total - kMeshLds : 0) > static_cast<int>(A.stack_cap)) {
+        overflow = true;
+        break;
+      }
+      if (pushes) {
         uint64_t x, y;
         if (push1) {
           x = (static_cast<uint64_t>(c1) << 32) | n2;
@@ -508,12 +661,23 @@ __global__ void __launch_bounds__(256) meshKernel(MeshArgs A) {
           x = (static_cast<uint64_t>(n1) << 32) | static_cast<uint32_t>(c2);
           y = (static_cast<uint64_t>(n1) << 32) | static_cast<uint32_t>(c2 + 1);
         }
-        stk[pos] = x;
-        stk[pos + 1] = y;
+        const uint64_t v2[2] = {x, y};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int pos = top + ppos + h;
+          if (pos < kMeshLds) lstk[pos] = v2[h];
+          else gstk[gtop + pos - kMeshLds] = v2[h];
+        }
       }
-      top += total;
+      if (top + total > kMeshLds) {
+        gtop += top + total - kMeshLds;
+        top = kMeshLds;
+      } else {
+        top += total;
+      }
+      __syncthreads();
     }
-    if (lane == 0) {
+    if (t == 0) {
       mfcl_mesh_result r;
       r.hit = overflow ? -1 : (ncont > 0 ? 1 : 0);
       r.n_contacts = overflow ? -1 : ncont;
@@ -523,8 +687,9 @@ __global__ void __launch_bounds__(256) meshKernel(MeshArgs A) {
     }
     tot_bv += bv;
     tot_leaf += leaf;
+    __syncthreads();
   }
-  if (lane == 0) {
+  if (t == 0) {
     atomicAdd(&A.totals[0], tot_bv);
     atomicAdd(&A.totals[1], tot_leaf);
   }
@@ -535,15 +700,16 @@ __global__ void zeroU32(uint32_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-// The traversal's workspace: 2 KB of counters, then one stack of
-// mesh_stack entries per persistent wavefront (mesh_blocks_per_cu blocks
-// of 4 wavefronts per CU).
+// The traversal's workspace: 2 KB of counters, then one device-memory stack
+// of mesh_stack entries per persistent block (mesh_blocks_per_cu blocks of
+// 4 wavefronts per CU).
 int reserveMesh(mfcl_ctx* ctx) {
   const uint32_t waves = static_cast<uint32_t>(ctx->n_cu * ctx->tune.mesh_blocks_per_cu * 4);
   const uint32_t cap = static_cast<uint32_t>(ctx->tune.mesh_stack);
   if (ctx->ws_arena.p && ctx->mesh_ws_waves == waves && ctx->mesh_ws_cap == cap) return MFCL_OK;
   ctx->mesh_ws_waves = ctx->mesh_ws_cap = 0;
-  if (int rc = ensure(ctx, ctx->ws_arena, 2048 + static_cast<size_t>(waves) * cap * sizeof(uint64_t))) return rc;
+  // one device-memory stack per block (4 wavefronts)
+  if (int rc = ensure(ctx, ctx->ws_arena, 2048 + static_cast<size_t>(waves / 4) * cap * sizeof(uint64_t))) return rc;
   ctx->mesh_ws_waves = waves;
   ctx->mesh_ws_cap = cap;
   return MFCL_OK;
@@ -551,7 +717,7 @@ int reserveMesh(mfcl_ctx* ctx) {
 
 int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
-                      int32_t* contacts, hipStream_t s) {
+                      int32_t* contacts, mfcl_mesh_contact* pen_contacts, hipStream_t s) {
   if (n == 0) return MFCL_OK;
   if (ctx->n_meshes == 0) {
     ctx->err = "no meshes uploaded";
@@ -563,7 +729,7 @@ int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
     ctx->err = "mesh traversal stacks not sized for the current knobs";
     return MFCL_E_STATE;
   }
-  const int blocks = static_cast<int>(waves / 4);
+  const int blocks = static_cast<int>(waves / 4);  // one 256-thread block per query
   char* misc = static_cast<char*>(ctx->ws_arena.p);
   uint32_t* next = reinterpret_cast<uint32_t*>(misc + 1024);
   unsigned long long* totals = reinterpret_cast<unsigned long long*>(misc + 1536);
@@ -582,13 +748,15 @@ int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   A.max_contacts = max_contacts;
   A.out = out;
   A.contacts = contacts;
+  A.pen_contacts = pen_contacts;
   A.stacks = reinterpret_cast<uint64_t*>(misc + 2048);
   A.stack_cap = cap;
   A.next_query = next;
   A.totals = totals;
   zeroU32<<<1, 512, 0, s>>>(reinterpret_cast<uint32_t*>(misc), 512);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
-  meshKernel<<<blocks, 256, 0, s>>>(A);
+  if (pen_contacts) meshKernel<true><<<blocks, kMeshBlock, 0, s>>>(A);
+  else meshKernel<false><<<blocks, kMeshBlock, 0, s>>>(A);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
   MFCL_HIP_TRY(ctx, hipGetLastError());
   ctx->d_counters = reinterpret_cast<uint64_t*>(totals);

@@ -602,25 +602,45 @@ int mfcl_mesh_num_nodes(mfcl_ctx* ctx, uint32_t mesh, uint32_t* n_nodes) {
   return MFCL_OK;
 }
 
-int mfcl_mesh_collide_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+static int meshCollideDev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                           const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
-                          int32_t* contacts, void* stream) {
+                          int32_t* contacts, mfcl_mesh_contact* pen, void* stream) {
   if (!ctx) return MFCL_E_INVALID;
   if (max_contacts < 1 || max_contacts > 1024) {
     ctx->err = "max_contacts must be in [1, 1024]";
     return MFCL_E_INVALID;
   }
+  if (n && (!pa || !pb || !posea || !poseb || !out)) {
+    ctx->err = "null device pointer";
+    return MFCL_E_INVALID;
+  }
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  const int rc = meshCollideLaunch(ctx, pa, pb, posea, poseb, n, max_contacts, out, contacts, s);
+  const int rc = meshCollideLaunch(ctx, pa, pb, posea, poseb, n, max_contacts, out, contacts, pen, s);
   if (rc) return rc;
   if (ctx->timing && n) return finishTiming(ctx, 4, 2, 3);
   return MFCL_OK;
 }
 
-int mfcl_mesh_collide(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
-                      const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
-                      int32_t* contacts) {
+int mfcl_mesh_collide_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                          const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
+                          int32_t* contacts, void* stream) {
+  return meshCollideDev(ctx, pa, pb, posea, poseb, n, max_contacts, out, contacts, nullptr, stream);
+}
+
+int mfcl_mesh_collide_contacts_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                                   const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
+                                   mfcl_mesh_contact* contacts, void* stream) {
+  if (ctx && n && !contacts) {
+    ctx->err = "null contact array";
+    return MFCL_E_INVALID;
+  }
+  return meshCollideDev(ctx, pa, pb, posea, poseb, n, max_contacts, out, nullptr, contacts, stream);
+}
+
+static int meshCollideHost(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                           const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
+                           int32_t* contacts, mfcl_mesh_contact* pen) {
   if (!ctx) return MFCL_E_INVALID;
   if (n == 0) return MFCL_OK;
   if (max_contacts < 1 || max_contacts > 1024) {
@@ -634,8 +654,8 @@ int mfcl_mesh_collide(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
     }
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   const size_t b_ids = 4 * n, b_pose = 28 * n, b_out = sizeof(mfcl_mesh_result) * n,
-               b_c = sizeof(int32_t) * 2 * max_contacts * n;
-  if (int rc = ensure(ctx, ctx->ws_io, 2 * b_ids + 2 * b_pose + b_out + b_c + 6 * 256)) return rc;
+               b_c = sizeof(int32_t) * 2 * max_contacts * n, b_p = pen ? sizeof(mfcl_mesh_contact) * max_contacts * n : 0;
+  if (int rc = ensure(ctx, ctx->ws_io, 2 * b_ids + 2 * b_pose + b_out + b_c + b_p + 7 * 256)) return rc;
   char* base = static_cast<char*>(ctx->ws_io.p);
   auto carve = [&](size_t bytes) {
     char* p = base;
@@ -648,17 +668,35 @@ int mfcl_mesh_collide(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
   float* dqb = reinterpret_cast<float*>(carve(b_pose));
   mfcl_mesh_result* dout = reinterpret_cast<mfcl_mesh_result*>(carve(b_out));
   int32_t* dc = reinterpret_cast<int32_t*>(carve(b_c));
+  mfcl_mesh_contact* dp = pen ? reinterpret_cast<mfcl_mesh_contact*>(carve(b_p)) : nullptr;
   hipStream_t s = ctx->stream;
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpa, pa, b_ids, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpb, pb, b_ids, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqa, posea, b_pose, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqb, poseb, b_pose, hipMemcpyHostToDevice, s));
-  int rc = mfcl_mesh_collide_dev(ctx, dpa, dpb, dqa, dqb, n, max_contacts, dout, dc, s);
+  int rc = meshCollideDev(ctx, dpa, dpb, dqa, dqb, n, max_contacts, dout, pen ? nullptr : dc, dp, s);
   if (rc) return rc;
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(out, dout, b_out, hipMemcpyDeviceToHost, s));
-  if (contacts) MFCL_HIP_TRY(ctx, hipMemcpyAsync(contacts, dc, b_c, hipMemcpyDeviceToHost, s));
+  if (contacts && !pen) MFCL_HIP_TRY(ctx, hipMemcpyAsync(contacts, dc, b_c, hipMemcpyDeviceToHost, s));
+  if (pen) MFCL_HIP_TRY(ctx, hipMemcpyAsync(pen, dp, b_p, hipMemcpyDeviceToHost, s));
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
   return MFCL_OK;
+}
+
+int mfcl_mesh_collide(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                      const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
+                      int32_t* contacts) {
+  return meshCollideHost(ctx, pa, pb, posea, poseb, n, max_contacts, out, contacts, nullptr);
+}
+
+int mfcl_mesh_collide_contacts(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                               const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
+                               mfcl_mesh_contact* contacts) {
+  if (ctx && n && !contacts) {
+    ctx->err = "null contact array";
+    return MFCL_E_INVALID;
+  }
+  return meshCollideHost(ctx, pa, pb, posea, poseb, n, max_contacts, out, nullptr, contacts);
 }
 
 int mfcl_aabbtree_build(mfcl_ctx* ctx, const float* mn3, const float* mx3, const int64_t* ids, uint32_t n) {

@@ -84,7 +84,7 @@ struct mfcl_ctx {
     int epa_lds_faces{64};      // face capacity of the LDS polytope (overflow continues or reruns at full size)
     int epa_spill{1};           // overflow continues in a device-memory polytope (1) or reruns (0)
     int epa_overflow_waves{512};   // persistent wavefronts (one per block) for overflow pairs
-    int mesh_blocks_per_cu{4};  // persistent mesh-traversal blocks (4 waves) per CU
+    int mesh_blocks_per_cu{3};  // persistent mesh-traversal blocks (4 waves, one query each) per CU
     int mesh_stack{16384};      // node-pair stack entries per wavefront
   } tune;
   std::string err;
@@ -202,7 +202,7 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
                       mfcl_convex_result* out, mfcl_contact* contacts, const ConvexWs& ws, hipStream_t s);
 int meshCollideLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
-                      int32_t* contacts, hipStream_t s);
+                      int32_t* contacts, mfcl_mesh_contact* pen_contacts, hipStream_t s);
 int aabbQueryLaunch(mfcl_ctx* ctx, const float* qmn, const float* qmx, uint64_t nq, int32_t cap,
                     int32_t* counts, int64_t* ids, hipStream_t s);
 

@@ -108,6 +108,10 @@ assert CONVEX_RESULT_DTYPE.itemsize == 128
 CONTACT_DTYPE = np.dtype([("normal", np.float64, 3), ("pos", np.float64, 3), ("depth", np.float64)])
 assert CONTACT_DTYPE.itemsize == 56
 
+MESH_CONTACT_DTYPE = np.dtype([("b1", np.int32), ("b2", np.int32), ("pos", np.float64, 3), ("normal", np.float64, 3),
+                               ("depth", np.float64)])
+assert MESH_CONTACT_DTYPE.itemsize == 64
+
 MESH_RESULT_DTYPE = np.dtype(
     [("hit", np.int32), ("n_contacts", np.int32), ("bv_tests", np.int32), ("leaf_tests", np.int32)]
 )
@@ -159,6 +163,8 @@ def load_library() -> C.CDLL:
     lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
     lib.mfcl_mesh_collide.argtypes = [p, p, p, p, p, u64, i32, p, p]
     lib.mfcl_mesh_collide_dev.argtypes = [p, p, p, p, p, u64, i32, p, p, p]
+    lib.mfcl_mesh_collide_contacts.argtypes = [p, p, p, p, p, u64, i32, p, p]
+    lib.mfcl_mesh_collide_contacts_dev.argtypes = [p, p, p, p, p, u64, i32, p, p, p]
     lib.mfcl_aabbtree_build.argtypes = [p, p, p, p, u32]
     lib.mfcl_aabbtree_query.argtypes = [p, p, p, u64, i32, p, p]
     lib.mfcl_aabbtree_query_dev.argtypes = [p, p, p, u64, i32, p, p, p]
@@ -409,6 +415,24 @@ class Context:
             )
         )
         return out, contacts
+
+    def mesh_collide_contacts(self, pair_a, pair_b, pose_a, pose_b, max_contacts: int = 16):
+        """Penetration-enabled mesh collide: (results, contacts[n, max_contacts]
+        of MESH_CONTACT_DTYPE; records past results['n_contacts'] are zero)."""
+        pair_a = np.ascontiguousarray(pair_a, np.uint32)
+        pair_b = np.ascontiguousarray(pair_b, np.uint32)
+        pose_a = np.ascontiguousarray(pose_a, np.float32).reshape(-1, 7)
+        pose_b = np.ascontiguousarray(pose_b, np.float32).reshape(-1, 7)
+        n = len(pair_a)
+        out = np.zeros(n, MESH_RESULT_DTYPE)
+        cont = np.zeros((n, max_contacts), MESH_CONTACT_DTYPE)
+        self._check(
+            self.lib.mfcl_mesh_collide_contacts(
+                self.h, _ptr(pair_a), _ptr(pair_b), _ptr(pose_a), _ptr(pose_b), n, max_contacts, _ptr(out),
+                _ptr(cont)
+            )
+        )
+        return out, cont
 
     def mesh_collide_dev(self, pa, pb, qa, qb, n, max_contacts, out, contacts):
         self._check(

@@ -305,11 +305,81 @@ bool triTriIntersect(const V3& P1, const V3& P2, const V3& P3, const V3& Q1in, c
   return true;
 }
 
+namespace {
+// intersect-inl.h:1036-1051 with normalize() of geometry-inl.h:188-197:
+// unit normal and offset of the triangle's plane; false (and n, t left as
+// given) for a degenerate triangle.
+bool buildTrianglePlane(const V3& v1, const V3& v2, const V3& v3, V3* n, double* t) {
+  V3 n_ = cross(v2 - v1, v3 - v1);
+  const double sq = sqnorm(n_);
+  if (!(sq > 0)) return false;
+  const double l = std::sqrt(sq);
+  n_ = V3(n_.x / l, n_.y / l, n_.z / l);
+  *n = n_;
+  *t = dot(n_, v1);
+  return true;
+}
+
+// intersect-inl.h:845-882 (getEpsilon() = 1e-5, distanceToPlane = n.v - t)
+void computeDeepestPoints(const V3* pts, int np, const V3& n, double t, double* depth, V3* deepest, int* nd) {
+  constexpr double kEps = 1e-5;
+  double max_depth = -std::numeric_limits<double>::max();
+  int k = 0, num_neg = 0, num_pos = 0, num_zero = 0;
+  for (int i = 0; i < np; ++i) {
+    const double dist = -(dot(n, pts[i]) - t);
+    if (dist > kEps) num_pos++;
+    else if (dist < -kEps) num_neg++;
+    else num_zero++;
+    if (dist > max_depth) {
+      max_depth = dist;
+      k = 1;
+      deepest[0] = pts[i];
+    } else if (dist + 1e-6 >= max_depth) {
+      k++;
+      deepest[k - 1] = pts[i];
+    }
+  }
+  if (max_depth < -kEps) k = 0;
+  if (num_zero == 0 && (num_neg == 0 || num_pos == 0)) k = 0;
+  *depth = max_depth;
+  *nd = k;
+}
+}  // namespace
+
+bool triTriContacts(const V3& P1, const V3& P2, const V3& P3, const V3& Q1in, const V3& Q2in, const V3& Q3in,
+                    const M3& R, const V3& T, V3 pts[2], int* npts, double* depth, V3* normal) {
+  if (!triTriIntersect(P1, P2, P3, Q1in, Q2in, Q3in, R, T)) return false;
+  const V3 Q1 = R * Q1in + T, Q2 = R * Q2in + T, Q3 = R * Q3in + T;
+  // the reference leaves n / t unset for a degenerate triangle; zero here
+  V3 n1(0, 0, 0), n2(0, 0, 0);
+  double t1 = 0, t2 = 0;
+  buildTrianglePlane(P1, P2, P3, &n1, &t1);
+  buildTrianglePlane(Q1, Q2, Q3, &n2, &t2);
+  V3 d1[3], d2[3];
+  int k1 = 0, k2 = 0;
+  double pd1 = 0, pd2 = 0;
+  const V3 P[3] = {P1, P2, P3}, Q[3] = {Q1, Q2, Q3};
+  computeDeepestPoints(Q, 3, n1, t1, &pd2, d2, &k2);
+  computeDeepestPoints(P, 3, n2, t2, &pd1, d1, &k1);
+  if (pd1 > pd2) {
+    *npts = std::min(k2, 2);
+    for (int i = 0; i < *npts; ++i) pts[i] = d2[i];
+    *normal = n1;
+    *depth = pd2;
+  } else {
+    *npts = std::min(k1, 2);
+    for (int i = 0; i < *npts; ++i) pts[i] = d1[i];
+    *normal = -n2;
+    *depth = pd1;
+  }
+  return true;
+}
+
 // bvh_solver-inl.h:75-164: explicit stack seeded with (0, 0); descend into
 // node 1 when node 2 is a leaf or node 1 is the larger inner node; the right
 // child is pushed last and so visited first.
 void meshCollide(const Mesh& m1, const Tf& tf1, const Mesh& m2, const Tf& tf2, int max_contacts,
-                 MeshCollideOut& out) {
+                 MeshCollideOut& out, bool penetration) {
   out = MeshCollideOut();
   const M3 R = transpose(tf1.R) * tf2.R;
   const V3 T = tmul(tf1.R, tf2.t - tf1.t);
@@ -328,10 +398,32 @@ void meshCollide(const Mesh& m1, const Tf& tf1, const Mesh& m2, const Tf& tf2, i
       const int* ta = &m1.tris[3 * p1];
       const int* tb = &m2.tris[3 * p2];
       out.leaf_tests++;
-      if (triTriIntersect(m1.verts[ta[0]], m1.verts[ta[1]], m1.verts[ta[2]], m2.verts[tb[0]],
-                          m2.verts[tb[1]], m2.verts[tb[2]], R, T)) {
-        out.contacts.emplace_back(p1, p2);
-        if (max_contacts > 0 && static_cast<int>(out.contacts.size()) >= max_contacts) return;
+      if (!penetration) {
+        if (triTriIntersect(m1.verts[ta[0]], m1.verts[ta[1]], m1.verts[ta[2]], m2.verts[tb[0]],
+                            m2.verts[tb[1]], m2.verts[tb[2]], R, T)) {
+          out.contacts.emplace_back(p1, p2);
+          if (max_contacts > 0 && static_cast<int>(out.contacts.size()) >= max_contacts) return;
+        }
+        continue;
+      }
+      // shape_pair_intersect-inl.h:226-254: as many contacts as deepest
+      // points, truncated to the free slots; positions and normals mapped
+      // to the world by the transform the reference passes there, which is
+      // the second mesh's (SimplexIntersect calls trianglePairIntersect
+      // with tf2 in both transform slots, :269)
+      V3 pts[2];
+      int np = 0;
+      double depth = 0;
+      V3 nrm;
+      if (triTriContacts(m1.verts[ta[0]], m1.verts[ta[1]], m1.verts[ta[2]], m2.verts[tb[0]], m2.verts[tb[1]],
+                         m2.verts[tb[2]], R, T, pts, &np, &depth, &nrm)) {
+        const int have = static_cast<int>(out.pen.size());
+        if (max_contacts > 0 && max_contacts < have + np) np = max_contacts > have ? max_contacts - have : 0;
+        for (int i = 0; i < np; ++i) {
+          out.pen.push_back({p1, p2, tf2 * pts[i], tf2.R * nrm, depth});
+          out.contacts.emplace_back(p1, p2);
+        }
+        if (max_contacts > 0 && static_cast<int>(out.pen.size()) >= max_contacts) return;
       }
       continue;
     }

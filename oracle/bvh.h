@@ -43,14 +43,28 @@ bool obbOverlap(const M3& R0, const V3& T0, const OBB& b1, const OBB& b2);
 bool triTriIntersect(const V3& P1, const V3& P2, const V3& P3, const V3& Q1, const V3& Q2,
                      const V3& Q3, const M3& R, const V3& T);
 
+// A contact of a penetration-enabled mesh query (world frame as the
+// reference computes it, see triTriContacts).
+struct MeshContact {
+  int b1, b2;
+  V3 pos, normal;
+  double depth;
+};
 struct MeshCollideOut {
-  std::vector<std::pair<int, int>> contacts;  // (b1, b2) in discovery order
+  std::vector<std::pair<int, int>> contacts;  // (b1, b2) in discovery order (one per contact)
+  std::vector<MeshContact> pen;               // penetration mode: the contacts themselves
   int64_t bv_tests{0};
   int64_t leaf_tests{0};
 };
-// max_contacts <= 0: collect every colliding triangle pair.
+// max_contacts <= 0: collect every contact.  penetration: contacts carry
+// position, normal and depth (intersect-inl.h:796-842), and a colliding
+// triangle pair adds as many contacts as deepest points it yields (0-2).
 void meshCollide(const Mesh& m1, const Tf& tf1, const Mesh& m2, const Tf& tf2, int max_contacts,
-                 MeshCollideOut& out);
+                 MeshCollideOut& out, bool penetration = false);
+// intersect-inl.h:724-842: the 17-axis test and, when the triangles
+// intersect, up to 2 contact points (P's frame), depth and normal.
+bool triTriContacts(const V3& P1, const V3& P2, const V3& P3, const V3& Q1in, const V3& Q2in, const V3& Q3in,
+                    const M3& R, const V3& T, V3 pts[2], int* npts, double* depth, V3* normal);
 
 // Broadphase AABB tree.
 struct AABBd {

@@ -296,6 +296,71 @@ double orc_mesh_collide_batch(void* m, const int* pa, const int* pb, const doubl
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+struct OrcMeshContact {
+  int32_t b1, b2;
+  double pos[3], normal[3], depth;
+};
+
+static void putContact(OrcMeshContact& c, const MeshContact& m) {
+  c.b1 = m.b1;
+  c.b2 = m.b2;
+  c.pos[0] = m.pos.x; c.pos[1] = m.pos.y; c.pos[2] = m.pos.z;
+  c.normal[0] = m.normal.x; c.normal[1] = m.normal.y; c.normal[2] = m.normal.z;
+  c.depth = m.depth;
+}
+
+// Penetration-enabled mesh-mesh collide over a batch: per pair the number of
+// contacts and the first max_contacts of them (CollisionResult order).
+double orc_mesh_collide_pen_batch(void* m, const int* pa, const int* pb, const double* tfa12, const double* tfb12,
+                                  int64_t n, int max_contacts, int n_threads, int32_t* ncontacts,
+                                  OrcMeshContact* contacts) {
+  const auto* ms = static_cast<MeshSet*>(m);
+  const auto t0 = std::chrono::steady_clock::now();
+  parallelFor(n, n_threads, [&](int64_t i) {
+    MeshCollideOut o;
+    meshCollide(ms->meshes[pa[i]], tfFrom12(tfa12 + 12 * i), ms->meshes[pb[i]], tfFrom12(tfb12 + 12 * i),
+                max_contacts, o, true);
+    ncontacts[i] = static_cast<int32_t>(o.pen.size());
+    for (int k = 0; k < static_cast<int>(o.pen.size()) && k < max_contacts; ++k)
+      putContact(contacts[i * max_contacts + k], o.pen[k]);
+  });
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Every penetration contact of one mesh pair (full traversal), up to cap;
+// returns the total count.
+int64_t orc_mesh_collide_all_pen(void* m, int ia, int ib, const double* tfa12, const double* tfb12,
+                                 OrcMeshContact* out, int64_t cap) {
+  const auto* ms = static_cast<MeshSet*>(m);
+  MeshCollideOut o;
+  meshCollide(ms->meshes[ia], tfFrom12(tfa12), ms->meshes[ib], tfFrom12(tfb12), 0, o, true);
+  for (int64_t k = 0; k < static_cast<int64_t>(o.pen.size()) && k < cap; ++k) putContact(out[k], o.pen[k]);
+  return static_cast<int64_t>(o.pen.size());
+}
+
+// One triangle pair (Q given in its own frame, mapped by R, T):
+// intersect-inl.h:594-613 with contacts.  Returns 1 when they intersect.
+int orc_tri_tri_contacts(const double* P9, const double* Q9, const double* R9, const double* T3, double* pts6,
+                         int* npts, double* depth, double* normal3) {
+  M3 R;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R.m[i][j] = R9[3 * i + j];
+  const V3 T(T3[0], T3[1], T3[2]);
+  V3 pts[2], nrm;
+  int np = 0;
+  double d = 0;
+  const bool hit = triTriContacts(V3(P9[0], P9[1], P9[2]), V3(P9[3], P9[4], P9[5]), V3(P9[6], P9[7], P9[8]),
+                                  V3(Q9[0], Q9[1], Q9[2]), V3(Q9[3], Q9[4], Q9[5]), V3(Q9[6], Q9[7], Q9[8]), R, T,
+                                  pts, &np, &d, &nrm);
+  *npts = hit ? np : 0;
+  *depth = d;
+  normal3[0] = nrm.x; normal3[1] = nrm.y; normal3[2] = nrm.z;
+  for (int k = 0; k < (hit ? np : 0); ++k) {
+    pts6[3 * k] = pts[k].x; pts6[3 * k + 1] = pts[k].y; pts6[3 * k + 2] = pts[k].z;
+  }
+  return hit ? 1 : 0;
+}
+
 // All colliding triangle pairs of one mesh pair (full traversal), written as
 // (b1, b2) up to cap entries; returns the total count.
 int64_t orc_mesh_collide_all(void* m, int ia, int ib, const double* tfa12, const double* tfb12,

@@ -16,6 +16,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
 
+MESH_CONTACT_REC = np.dtype([("b1", np.int32), ("b2", np.int32), ("pos", np.float64, 3), ("normal", np.float64, 3),
+                             ("depth", np.float64)])
 COLLIDE_REC = np.dtype(
     [("hit", np.int32), ("n", np.int32), ("method", np.int32), ("gjk_status", np.int32), ("epa_status", np.int32),
      ("gjk_iters", np.int32), ("epa_iters", np.int32), ("pad", np.int32), ("depth", np.float64),
@@ -69,6 +71,12 @@ def lib() -> C.CDLL:
         L.orc_mesh_collide_all.restype = i64
         L.orc_mesh_collide_brute.argtypes = [p, i32, i32, p, p, p, i64]
         L.orc_mesh_collide_brute.restype = i64
+        L.orc_mesh_collide_pen_batch.argtypes = [p, p, p, p, p, i64, i32, i32, p, p]
+        L.orc_mesh_collide_pen_batch.restype = d
+        L.orc_mesh_collide_all_pen.argtypes = [p, i32, i32, p, p, p, i64]
+        L.orc_mesh_collide_all_pen.restype = i64
+        L.orc_tri_tri_contacts.argtypes = [p, p, p, p, p, p, p, p]
+        L.orc_tri_tri_contacts.restype = i32
         L.orc_tri_tri.argtypes = [p, p, p, p]
         L.orc_obb_disjoint.argtypes = [p, p, p, p]
         L.orc_aabbtree_create.argtypes = [p, p, p, i32]
@@ -242,6 +250,26 @@ class MeshSet:
                                             _p(hit), _p(nc), _p(cont), _p(bv), _p(lf))
         return dict(hit=hit, n=nc, contacts=cont, bv_tests=bv, leaf_tests=lf), secs
 
+    def collide_pen(self, pa, pb, tfa12, tfb12, max_contacts=16, threads=1):
+        """Penetration-enabled collide: (n_contacts[n], contacts[n, max_contacts]
+        of MESH_CONTACT_REC in CollisionResult order), seconds."""
+        pa = np.ascontiguousarray(pa, np.int32)
+        pb = np.ascontiguousarray(pb, np.int32)
+        tfa12 = np.ascontiguousarray(tfa12, np.float64).reshape(-1, 12)
+        tfb12 = np.ascontiguousarray(tfb12, np.float64).reshape(-1, 12)
+        n = len(pa)
+        nc = np.zeros(n, np.int32)
+        cont = np.zeros((n, max(1, max_contacts)), MESH_CONTACT_REC)
+        secs = lib().orc_mesh_collide_pen_batch(self.h, _p(pa), _p(pb), _p(tfa12), _p(tfb12), n, max_contacts,
+                                                threads, _p(nc), _p(cont))
+        return nc, cont, secs
+
+    def collide_all_pen(self, ia, ib, tfa12, tfb12, cap=1 << 16):
+        out = np.zeros(cap, MESH_CONTACT_REC)
+        n = lib().orc_mesh_collide_all_pen(self.h, ia, ib, _p(np.ascontiguousarray(tfa12, np.float64)),
+                                           _p(np.ascontiguousarray(tfb12, np.float64)), _p(out), cap)
+        return out[:min(n, cap)], int(n)
+
     def collide_all(self, ia, ib, tfa12, tfb12, cap=1 << 20):
         out = np.zeros((cap, 2), np.int32)
         n = lib().orc_mesh_collide_all(self.h, ia, ib, _p(np.ascontiguousarray(tfa12, np.float64)),
@@ -254,6 +282,18 @@ class MeshSet:
         n = lib().orc_mesh_collide_brute(self.h, ia, ib, _p(np.ascontiguousarray(tfa12, np.float64)),
                                          _p(np.ascontiguousarray(tfb12, np.float64)), _p(out), cap)
         return out[:min(n, cap)], int(n)
+
+
+def tri_tri_contacts(P9, Q9, R9, T3):
+    """(hit, points[npts, 3], depth, normal) of intersect_Triangle with
+    contacts (Q in its own frame, mapped into P's by R, T)."""
+    a = [np.ascontiguousarray(x, np.float64) for x in (P9, Q9, R9, T3)]
+    pts = np.zeros(6)
+    npts = C.c_int(0)
+    depth = C.c_double(0)
+    nrm = np.zeros(3)
+    hit = lib().orc_tri_tri_contacts(*[_p(x) for x in a], _p(pts), C.byref(npts), C.byref(depth), _p(nrm))
+    return bool(hit), pts[:3 * npts.value].reshape(-1, 3), depth.value, nrm
 
 
 def tri_tri(P9, Q9, R9, T3) -> bool:

@@ -539,3 +539,63 @@ def test_epa_depth_vs_support_function_minimum(oracle, k1, k2):
         assert got >= ref - 2e-6, (got, ref)
         checked += 1
     assert checked >= 20, checked
+
+
+# --------------------------------------------------- triangle-pair contacts
+
+def test_triangle_contacts_properties(oracle):
+    """intersect_Triangle with contacts (intersect-inl.h:796-842) on random
+    intersecting triangle pairs: the normal is the unit normal of one
+    triangle's plane (+ the first's, or - the second's); each contact point
+    is a vertex of the other triangle; the depth is that vertex's depth
+    below the plane (t - n.v) and no vertex of the other triangle lies
+    deeper; at most two points; the pair order reverses the normal."""
+    rng = np.random.default_rng(27)
+    I3 = np.eye(3).reshape(9)
+    n_hit = 0
+    for _ in range(3000):
+        P = rng.uniform(-1, 1, (3, 3))
+        Q = rng.uniform(-1, 1, (3, 3))
+        hit, pts, depth, nrm = oracle.tri_tri_contacts(P.reshape(9), Q.reshape(9), I3, np.zeros(3))
+        if not hit:
+            continue
+        n_hit += 1
+        nP = np.cross(P[1] - P[0], P[2] - P[0])
+        nP /= np.linalg.norm(nP)
+        nQ = np.cross(Q[1] - Q[0], Q[2] - Q[0])
+        nQ /= np.linalg.norm(nQ)
+        assert abs(np.linalg.norm(nrm) - 1) < 1e-12
+        use_p = np.allclose(nrm, nP, atol=1e-12)
+        assert use_p or np.allclose(nrm, -nQ, atol=1e-12)
+        other = Q if use_p else P
+        plane_n, plane_pt = (nP, P[0]) if use_p else (nQ, Q[0])
+        d_other = (plane_pt @ plane_n) - other @ plane_n  # depth below the plane
+        assert len(pts) <= 2
+        assert abs(depth - d_other.max()) < 1e-12
+        for p in pts:
+            j = int(np.argmin(np.linalg.norm(other - p, axis=1)))
+            assert np.linalg.norm(other[j] - p) < 1e-15
+            assert abs(d_other[j] - depth) <= 1e-6 + 1e-12
+    assert n_hit > 300
+
+
+def test_triangle_contacts_known_answer(oracle):
+    """A small triangle piercing a large one in the z = 0 plane with one
+    vertex 0.1 below it: the contact is that vertex, depth 0.1, normal +z
+    (the large triangle's side: its own vertices lie far on both sides of
+    the small triangle's plane, so that depth is the larger)."""
+    P = np.array([[-2.0, -2.0, 0.0], [3.0, -1.0, 0.0], [-1.0, 3.0, 0.0]])
+    Q = np.array([[0.1, 0.2, -0.1], [0.4, 0.1, 0.6], [0.0, 0.5, 0.5]])
+    hit, pts, depth, nrm = oracle.tri_tri_contacts(P.reshape(9), Q.reshape(9), np.eye(3).reshape(9), np.zeros(3))
+    assert hit
+    assert np.allclose(nrm, [0, 0, 1], atol=1e-15)
+    assert abs(depth - 0.1) < 1e-15
+    assert len(pts) == 1 and np.allclose(pts[0], Q[0], atol=1e-15)
+    # the same pair with Q given in a rotated frame (R, T) gives the same answer
+    rng = np.random.default_rng(28)
+    R = _rot(rng)
+    T = rng.uniform(-1, 1, 3)
+    Ql = (Q - T) @ R  # R^T (q - T)
+    hit2, pts2, depth2, nrm2 = oracle.tri_tri_contacts(P.reshape(9), Ql.reshape(9), R.reshape(9), T)
+    assert hit2 and abs(depth2 - 0.1) < 1e-12 and np.allclose(nrm2, [0, 0, 1], atol=1e-12)
+    assert np.allclose(pts2[0], Q[0], atol=1e-12)
