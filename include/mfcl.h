@@ -178,6 +178,28 @@ int mfcl_reserve(mfcl_ctx* ctx, int32_t kind, uint64_t count);
 /* Pairs the convex workspace holds now (0 before any reservation). */
 int mfcl_reserved_pairs(mfcl_ctx* ctx, uint64_t* pairs);
 
+/* ---- captured query sequences ------------------------------------------- */
+/* No counterpart in the reference.  Between mfcl_graph_begin and
+ * mfcl_graph_end, the "_dev" query calls made on this context with a NULL
+ * stream (mfcl_convex_query_dev, mfcl_convex_collide_contacts_dev,
+ * mfcl_mesh_collide_dev, mfcl_mesh_collide_contacts_dev,
+ * mfcl_aabbtree_query_dev, mfcl_mlp_guess_dev) are recorded into a HIP
+ * graph instead of running; mfcl_graph_launch replays the whole sequence
+ * with one launch.  The recorded calls keep their arguments (device
+ * pointers, sizes, requests, knobs): replays read the buffers' current
+ * contents.  While recording, per-kernel timing is off and calls that
+ * synchronise, allocate or copy host memory (uploads, mfcl_reserve*, the
+ * host-pointer queries, mfcl_arm_query_dev, training) fail with
+ * MFCL_E_STATE.  Sizes stay within the workspace reserved before
+ * mfcl_graph_begin. */
+typedef struct mfcl_graph mfcl_graph;
+int mfcl_graph_begin(mfcl_ctx* ctx);
+int mfcl_graph_end(mfcl_ctx* ctx, mfcl_graph** out);
+/* Enqueues the recorded sequence on stream (hipStream_t, NULL: the
+ * context's stream). */
+int mfcl_graph_launch(mfcl_graph* g, void* stream);
+int mfcl_graph_destroy(mfcl_graph* g);
+
 /* ---- device memory helpers (so callers need no HIP runtime) ------------- */
 int mfcl_dev_alloc(mfcl_ctx* ctx, uint64_t bytes, void** dptr);
 int mfcl_dev_free(mfcl_ctx* ctx, void* dptr);

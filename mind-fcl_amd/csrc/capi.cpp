@@ -122,6 +122,21 @@ int mfcl::uploadVec(mfcl_ctx* ctx, void** dptr, const void* src, size_t bytes) {
   return MFCL_OK;
 }
 
+// Calls that synchronise, allocate or copy host memory cannot be recorded.
+#define MFCL_NOT_WHILE_CAPTURING(ctx)                                              \
+  do {                                                                             \
+    if ((ctx) && (ctx)->capturing) {                                               \
+      (ctx)->err = "not allowed while a query sequence is being recorded";         \
+      return MFCL_E_STATE;                                                         \
+    }                                                                              \
+  } while (0)
+
+struct mfcl_graph {
+  mfcl_ctx* ctx;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
 extern "C" {
 
 void mfcl_request_defaults(mfcl_request* req, int32_t mode) {
@@ -171,6 +186,12 @@ int mfcl_create(mfcl_ctx** out, int device) {
 int mfcl_destroy(mfcl_ctx* ctx) {
   if (!ctx) return MFCL_E_INVALID;
   (void)hipSetDevice(ctx->device);
+  if (ctx->capturing) {  // abandon an unfinished recording
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(ctx->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    ctx->capturing = false;
+  }
   (void)hipStreamSynchronize(ctx->stream);
   freeAll(ctx);
   for (auto& e : ctx->ev) (void)hipEventDestroy(e);
@@ -183,6 +204,7 @@ const char* mfcl_last_error(const mfcl_ctx* ctx) { return ctx ? ctx->err.c_str()
 
 int mfcl_sync(mfcl_ctx* ctx) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   MFCL_HIP_TRY(ctx, hipDeviceSynchronize());
   return MFCL_OK;
@@ -218,6 +240,7 @@ static int refreshCounters(mfcl_ctx* ctx) {
 
 int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4) {
   if (!ctx || !c4) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   if (int rc = refreshCounters(ctx)) return rc;
   std::memcpy(c4, ctx->counters, 4 * sizeof(uint64_t));
   return MFCL_OK;
@@ -225,6 +248,7 @@ int mfcl_last_counters(mfcl_ctx* ctx, uint64_t* c4) {
 
 int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n) {
   if (!ctx || !c || n < 0) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   if (int rc = refreshCounters(ctx)) return rc;
   for (int32_t k = 0; k < n; ++k) c[k] = k < 6 ? ctx->counters[k] : 0;
   return MFCL_OK;
@@ -232,6 +256,7 @@ int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n) {
 
 int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   switch (knob) {
     case MFCL_TUNE_GJK_GROUP:
       if (value != 1 && value != 16 && value != 32 && value != 64) break;
@@ -268,8 +293,66 @@ int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
   return MFCL_E_INVALID;
 }
 
+int mfcl_graph_begin(mfcl_ctx* ctx) {
+  if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  ctx->timing_before_capture = ctx->timing;
+  ctx->timing = false;
+  return MFCL_OK;
+}
+
+int mfcl_graph_end(mfcl_ctx* ctx, mfcl_graph** out) {
+  if (!ctx || !out) return MFCL_E_INVALID;
+  *out = nullptr;
+  if (!ctx->capturing) {
+    ctx->err = "no query sequence is being recorded";
+    return MFCL_E_STATE;
+  }
+  ctx->capturing = false;
+  ctx->timing = ctx->timing_before_capture;
+  hipGraph_t g = nullptr;
+  MFCL_HIP_TRY(ctx, hipStreamEndCapture(ctx->stream, &g));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+    return MFCL_E_HIP;
+  }
+  auto* r = new (std::nothrow) mfcl_graph{ctx, g, ex};
+  if (!r) {
+    (void)hipGraphExecDestroy(ex);
+    (void)hipGraphDestroy(g);
+    return MFCL_E_NOMEM;
+  }
+  *out = r;
+  return MFCL_OK;
+}
+
+int mfcl_graph_launch(mfcl_graph* g, void* stream) {
+  if (!g) return MFCL_E_INVALID;
+  mfcl_ctx* ctx = g->ctx;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  MFCL_HIP_TRY(ctx, hipGraphLaunch(g->exec, s));
+  return MFCL_OK;
+}
+
+int mfcl_graph_destroy(mfcl_graph* g) {
+  if (!g) return MFCL_E_INVALID;
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  delete g;
+  return MFCL_OK;
+}
+
 int mfcl_reserve(mfcl_ctx* ctx, int32_t kind, uint64_t count) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   switch (kind) {
     case MFCL_WS_CONVEX:
@@ -294,22 +377,26 @@ int mfcl_reserved_pairs(mfcl_ctx* ctx, uint64_t* pairs) {
 
 int mfcl_dev_alloc(mfcl_ctx* ctx, uint64_t bytes, void** dptr) {
   if (!ctx || !dptr) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   MFCL_HIP_TRY(ctx, hipMalloc(dptr, bytes ? bytes : 1));
   return MFCL_OK;
 }
 int mfcl_dev_free(mfcl_ctx* ctx, void* dptr) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipFree(dptr));
   return MFCL_OK;
 }
 int mfcl_h2d(mfcl_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return MFCL_OK;
 }
 int mfcl_d2h(mfcl_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   MFCL_HIP_TRY(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return MFCL_OK;
@@ -360,6 +447,7 @@ int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
                        const uint32_t* vert_cnt, uint32_t n_shapes, const float* verts_xyz,
                        uint64_t n_verts) {
   if (!ctx || !type || !params4 || n_shapes == 0) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   freeShapes(ctx);
@@ -423,6 +511,7 @@ int mfcl_upload_shapes(mfcl_ctx* ctx, const int32_t* type, const float* params4,
 int mfcl_upload_convex_faces(mfcl_ctx* ctx, const int32_t* faces, uint64_t faces_len, const uint64_t* face_off,
                              const uint32_t* num_faces) {
   if (!ctx || !face_off || !num_faces || (faces_len && !faces)) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   if (ctx->n_shapes == 0) {
     ctx->err = "no shapes uploaded";
     return MFCL_E_STATE;
@@ -526,6 +615,7 @@ int mfcl_convex_query_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb,
 int mfcl_convex_query(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                       const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
                       mfcl_convex_result* out) {
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   return mfcl_convex_collide_contacts(ctx, pa, pb, posea, poseb, guess, n, req, out, nullptr);
 }
 
@@ -533,6 +623,7 @@ int mfcl_convex_collide_contacts(mfcl_ctx* ctx, const uint32_t* pa, const uint32
                                  const float* poseb, const float* guess, uint64_t n, const mfcl_request* req,
                                  mfcl_convex_result* out, mfcl_contact* contacts) {
   if (!ctx || !req) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   if (n == 0) return MFCL_OK;
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   if (n > 0xFFFFFFFFull) {
@@ -589,6 +680,7 @@ int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* ve
                        uint32_t n_meshes) {
   if (!ctx || !verts_xyz || !vert_off || !vert_cnt || !tris || !tri_off || !tri_cnt || n_meshes == 0)
     return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   freeMeshes(ctx);
@@ -642,6 +734,7 @@ static int meshCollideHost(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb
                            const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
                            int32_t* contacts, mfcl_mesh_contact* pen) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   if (n == 0) return MFCL_OK;
   if (max_contacts < 1 || max_contacts > 1024) {
     ctx->err = "max_contacts must be in [1, 1024]";
@@ -701,6 +794,7 @@ int mfcl_mesh_collide_contacts(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t
 
 int mfcl_aabbtree_build(mfcl_ctx* ctx, const float* mn3, const float* mx3, const int64_t* ids, uint32_t n) {
   if (!ctx || !mn3 || !mx3 || !ids || n == 0) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return buildAndUploadAabbTree(ctx, mn3, mx3, ids, n);
@@ -724,6 +818,7 @@ int mfcl_aabbtree_query_dev(mfcl_ctx* ctx, const float* qmn3, const float* qmx3,
 int mfcl_aabbtree_query(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uint64_t nq, int32_t cap,
                         int32_t* counts, int64_t* ids) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   if (nq == 0) return MFCL_OK;
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   const size_t b_box = 12 * nq, b_cnt = 4 * nq, b_ids = sizeof(int64_t) * (cap > 0 ? cap : 1) * nq;
@@ -752,6 +847,7 @@ int mfcl_aabbtree_query(mfcl_ctx* ctx, const float* qmn3, const float* qmx3, uin
 // ---- warm-start MLP ---------------------------------------------------------
 int mfcl_mlp_init(mfcl_ctx* ctx, uint64_t seed) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   return mlpInit(ctx, seed);
 }
 
@@ -759,11 +855,13 @@ int mfcl_mlp_num_params(void) { return kMlpParams; }
 
 int mfcl_mlp_set_params(mfcl_ctx* ctx, const float* params, int32_t n) {
   if (!ctx || !params) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   return mlpSetParams(ctx, params, n, true);
 }
 
 int mfcl_mlp_get_params(mfcl_ctx* ctx, float* params, float* grads, int32_t n) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   return mlpGetParams(ctx, params, n, grads);
 }
 
@@ -797,6 +895,7 @@ int mfcl_mlp_forward_dev(mfcl_ctx* ctx, const float* feat64, uint64_t n, float* 
 int mfcl_mlp_train_step_dev(mfcl_ctx* ctx, const float* feat64, const float* target4, uint64_t n,
                             const float* hyper6, int32_t apply_update, float* loss_host, void* stream) {
   if (!ctx || !feat64 || !target4) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   const int rc = mlpTrainStep(ctx, feat64, target4, n, hyper6, apply_update, loss_host, s);
   if (rc) return rc;
@@ -812,6 +911,7 @@ int mfcl_mlp_train_step_dev(mfcl_ctx* ctx, const float* feat64, const float* tar
 int mfcl_upload_heightmap(mfcl_ctx* ctx, const uint16_t* heights_mm, uint16_t half_x, uint16_t half_y, float res_x,
                           float res_y) {
   if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   auto pow2 = [](uint32_t v) { return v > 0 && (v & (v - 1)) == 0; };
   if (!heights_mm || !pow2(half_x) || !pow2(half_y) || half_x > 16384 || half_y > 16384 || !(res_x > 0) ||
       !(res_y > 0) || !std::isfinite(res_x) || !std::isfinite(res_y)) {
@@ -869,6 +969,7 @@ int mfcl_arm_last_heightmap_pairs(mfcl_ctx* ctx, uint64_t* n_pairs) {
 int mfcl_arm_query_dev(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_configs, float margin,
                        const float* box_pose7, int32_t* any_hit, float* min_dist, uint64_t* n_pairs, void* stream) {
   if (!ctx || !arm) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   if (n_configs && (!q || !box_pose7 || !any_hit || !min_dist)) {
     ctx->err = "null device pointer";
     return MFCL_E_INVALID;
@@ -881,6 +982,7 @@ int mfcl_arm_query_dev(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint6
 
 int mfcl_reserve_arm(mfcl_ctx* ctx, const mfcl_arm* arm, uint64_t n_configs) {
   if (!ctx || !arm) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
   if (arm->n_links < 1 || arm->n_links > MFCL_ARM_MAX_LINKS || arm->broadphase_cap < 1 ||
       arm->broadphase_cap > 512) {
     ctx->err = "n_links or broadphase_cap out of range";

@@ -141,6 +141,10 @@ struct mfcl_ctx {
 
   mfcl::MlpState mlp;
 
+  // graph capture (mfcl_graph_begin .. mfcl_graph_end)
+  bool capturing{false};
+  bool timing_before_capture{false};
+
   // timing
   bool timing{false};
   hipEvent_t ev[16]{};

@@ -156,6 +156,10 @@ def load_library() -> C.CDLL:
     lib.mfcl_convex_collide_contacts_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p, p]
     lib.mfcl_convex_query_static_dev.argtypes = [p, p, p, p, p, p, u64, C.POINTER(Request), p, p]
     lib.mfcl_convex_query_static_dev_max_pairs.argtypes = [C.POINTER(u64)]
+    lib.mfcl_graph_begin.argtypes = [p]
+    lib.mfcl_graph_end.argtypes = [p, C.POINTER(p)]
+    lib.mfcl_graph_launch.argtypes = [p, p]
+    lib.mfcl_graph_destroy.argtypes = [p]
     lib.mfcl_reserve.argtypes = [p, i32, u64]
     lib.mfcl_reserved_pairs.argtypes = [p, C.POINTER(u64)]
     lib.mfcl_reserve_arm.argtypes = [p, C.POINTER(Arm), u64]
@@ -233,6 +237,22 @@ class DeviceArray:
             self.ptr = C.c_void_p()
 
 
+class Graph:
+    """A recorded query sequence (mfcl_graph_*)."""
+
+    def __init__(self, ctx: "Context", handle: C.c_void_p):
+        self.ctx = ctx
+        self.h = handle
+
+    def launch(self):
+        self.ctx._check(self.ctx.lib.mfcl_graph_launch(self.h, None))
+
+    def destroy(self):
+        if self.h:
+            self.ctx.lib.mfcl_graph_destroy(self.h)
+            self.h = C.c_void_p()
+
+
 class Context:
     def __init__(self, device: int = 0):
         self.lib = load_library()
@@ -275,6 +295,15 @@ class Context:
         c = np.zeros(6, np.uint64)
         self._check(self.lib.mfcl_last_counters_n(self.h, _ptr(c), 6))
         return c
+
+    def graph_begin(self):
+        """Starts recording the following "_dev" queries (mfcl_graph_begin)."""
+        self._check(self.lib.mfcl_graph_begin(self.h))
+
+    def graph_end(self) -> "Graph":
+        g = C.c_void_p()
+        self._check(self.lib.mfcl_graph_end(self.h, C.byref(g)))
+        return Graph(self, g)
 
     def reserve(self, kind: int, count: int):
         """mfcl_reserve: size the workspace of the "_dev" queries of that kind

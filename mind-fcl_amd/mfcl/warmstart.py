@@ -196,7 +196,9 @@ def compare(plain: np.ndarray, warm: np.ndarray, tight: np.ndarray, scale: np.nd
                 warm_vs_plain_max_abs_diff=float(np.abs(dp[ok] - dw[ok]).max()) if ok.any() else 0.0)
 
 
-def evaluate(ctx: Context, n: int, seed: int, steps: int = 5, log=print):
+def evaluate(ctx: Context, n: int, seed: int, steps: int = 5, log=print, keep=False):
+    """Plain vs warm-started queries on n held-out pairs.  keep: also return
+    the result records, the guesses and the workload."""
     ps = PairSet(ctx, n, seed)
     # the tighter solve that accuracy is judged against
     ctx.convex_query_dev(ps.pa, ps.pb, ps.qa, ps.qb, n,
@@ -228,12 +230,19 @@ def evaluate(ctx: Context, n: int, seed: int, steps: int = 5, log=print):
     r = W.bounding_radius(ps.h["types"], ps.h["params"], ps.h["vert_cnt"], ps.h["verts"], ps.h["vert_off"])
     scale = r[ps.h["pair_a"]] + r[ps.h["pair_b"]]
     agree = compare(res["plain"]["out"], res["warm"]["out"], tight, scale)
+    kept = None
+    if keep:
+        g = np.zeros((n, 3), np.float32)
+        ps.guess.download(g)
+        kept = dict(plain=res["plain"]["out"], warm=res["warm"]["out"], tight=tight, guess=g, h=ps.h)
     ps.free()
     for k in ("plain", "warm"):
         res[k].pop("out")
     res["agreement"] = agree
     res["speedup"] = res["warm"]["pairs_per_s"] / res["plain"]["pairs_per_s"]
     log(f"[eval] {json.dumps(res)}")
+    if keep:
+        res["kept"] = kept
     return res
 
 
