@@ -278,6 +278,26 @@ int mfcl_convex_collide_contacts_dev(mfcl_ctx* ctx, const uint32_t* pair_a, cons
                                      uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
                                      mfcl_contact* contacts, void* stream);
 
+/* ---- convex hulls on the device ----------------------------------------- */
+/* The convex hull of each point set (what a caller of fcl::Convex computes
+ * before constructing it from the hull's vertices and faces,
+ * include/fcl/geometry/shape/convex-inl.h:50-73).  Set i is
+ * points[pt_off[i] .. pt_off[i] + pt_cnt[i]) (xyz triples, 4 to
+ * max_points <= 768 points).  Per set: status[i] (0 ok, 1 degenerate: fewer
+ * than 4 points or all (nearly) coplanar, 2 more than max_points points,
+ * 3 inconsistent orientation tests), hull_cnt[i] hull vertices whose indices
+ * within the set are hull_idx[pt_off[i] + k] in ascending order, and
+ * tri_cnt[i] triangles (outward orientation) over those k-numbered
+ * vertices at tris[6 pt_off[i] + 3 t + j].  Points within a relative 1e-12
+ * of a face's plane are left out of the hull.  Device pointers. */
+int mfcl_convex_hulls_dev(mfcl_ctx* ctx, const float* points_xyz, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                          uint32_t n_sets, int32_t max_points, uint32_t* hull_cnt, uint32_t* hull_idx,
+                          uint32_t* tri_cnt, int32_t* tris, int32_t* status, void* stream);
+/* The same with host pointers (copies in, runs, copies out, synchronises). */
+int mfcl_convex_hulls(mfcl_ctx* ctx, const float* points_xyz, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                      uint32_t n_sets, uint32_t* hull_cnt, uint32_t* hull_idx, uint32_t* tri_cnt, int32_t* tris,
+                      int32_t* status);
+
 /* ---- OBBRSS mesh-mesh --------------------------------------------------- */
 /* Builds one OBB tree per mesh on the host and uploads node and triangle
  * arrays (replaces BVHModel<OBBRSS>::beginModel/addSubModel/endModel,

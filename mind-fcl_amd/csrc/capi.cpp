@@ -675,6 +675,65 @@ int mfcl_convex_collide_contacts(mfcl_ctx* ctx, const uint32_t* pa, const uint32
   return MFCL_OK;
 }
 
+int mfcl_convex_hulls_dev(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                          uint32_t n_sets, int32_t max_points, uint32_t* hull_cnt, uint32_t* hull_idx,
+                          uint32_t* tri_cnt, int32_t* tris, int32_t* status, void* stream) {
+  if (!ctx) return MFCL_E_INVALID;
+  if (n_sets && (!points || !pt_off || !pt_cnt || !hull_cnt || !hull_idx || !tri_cnt || !tris || !status)) {
+    ctx->err = "null device pointer";
+    return MFCL_E_INVALID;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return hullsLaunch(ctx, points, pt_off, pt_cnt, n_sets, max_points, hull_cnt, hull_idx, tri_cnt, tris, status, s);
+}
+
+int mfcl_convex_hulls(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                      uint32_t n_sets, uint32_t* hull_cnt, uint32_t* hull_idx, uint32_t* tri_cnt, int32_t* tris,
+                      int32_t* status) {
+  if (!ctx) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  if (n_sets == 0) return MFCL_OK;
+  if (!points || !pt_off || !pt_cnt || !hull_cnt || !hull_idx || !tri_cnt || !tris || !status) return MFCL_E_INVALID;
+  uint64_t total = 0;
+  uint32_t maxp = 4;
+  for (uint32_t i = 0; i < n_sets; ++i) {
+    total = std::max<uint64_t>(total, pt_off[i] + pt_cnt[i]);
+    maxp = std::max(maxp, pt_cnt[i]);
+  }
+  maxp = std::min<uint32_t>(maxp, 768);
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t b_pts = 12 * total, b_off = 8 * static_cast<size_t>(n_sets), b_cnt = 4 * static_cast<size_t>(n_sets),
+               b_idx = 4 * total, b_tri = 24 * total;
+  if (int rc = ensure(ctx, ctx->ws_io, b_pts + b_off + 4 * b_cnt + b_idx + b_tri + 8 * 256)) return rc;
+  char* base = static_cast<char*>(ctx->ws_io.p);
+  auto carve = [&](size_t bytes) {
+    char* p = base;
+    base += (bytes + 255) & ~static_cast<size_t>(255);
+    return p;
+  };
+  float* dp = reinterpret_cast<float*>(carve(b_pts));
+  uint64_t* doff = reinterpret_cast<uint64_t*>(carve(b_off));
+  uint32_t* dcnt = reinterpret_cast<uint32_t*>(carve(b_cnt));
+  uint32_t* dhc = reinterpret_cast<uint32_t*>(carve(b_cnt));
+  uint32_t* dtc = reinterpret_cast<uint32_t*>(carve(b_cnt));
+  int32_t* dst = reinterpret_cast<int32_t*>(carve(b_cnt));
+  uint32_t* dhi = reinterpret_cast<uint32_t*>(carve(b_idx));
+  int32_t* dtr = reinterpret_cast<int32_t*>(carve(b_tri));
+  hipStream_t s = ctx->stream;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dp, points, b_pts, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(doff, pt_off, b_off, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dcnt, pt_cnt, b_cnt, hipMemcpyHostToDevice, s));
+  if (int rc = hullsLaunch(ctx, dp, doff, dcnt, n_sets, static_cast<int>(maxp), dhc, dhi, dtc, dtr, dst, s)) return rc;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(hull_cnt, dhc, b_cnt, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(tri_cnt, dtc, b_cnt, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(status, dst, b_cnt, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(hull_idx, dhi, b_idx, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(tris, dtr, b_tri, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return MFCL_OK;
+}
+
 int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* vert_off, const uint32_t* vert_cnt,
                        const uint32_t* tris, const uint64_t* tri_off, const uint32_t* tri_cnt,
                        uint32_t n_meshes) {

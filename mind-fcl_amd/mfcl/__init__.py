@@ -163,6 +163,8 @@ def load_library() -> C.CDLL:
     lib.mfcl_reserve.argtypes = [p, i32, u64]
     lib.mfcl_reserved_pairs.argtypes = [p, C.POINTER(u64)]
     lib.mfcl_reserve_arm.argtypes = [p, C.POINTER(Arm), u64]
+    lib.mfcl_convex_hulls.argtypes = [p, p, p, p, u32, p, p, p, p, p]
+    lib.mfcl_convex_hulls_dev.argtypes = [p, p, p, p, u32, i32, p, p, p, p, p, p]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
     lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
     lib.mfcl_mesh_collide.argtypes = [p, p, p, p, p, u64, i32, p, p]
@@ -413,6 +415,27 @@ class Context:
                 out.ptr, None
             )
         )
+
+    # ---- convex hulls
+    def convex_hulls(self, points, pt_off, pt_cnt):
+        """Device convex hulls of point sets: list of (status, hull vertex
+        indices (ascending, within the set), triangles over the hull's
+        vertex numbering)."""
+        points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        pt_off = np.ascontiguousarray(pt_off, np.uint64)
+        pt_cnt = np.ascontiguousarray(pt_cnt, np.uint32)
+        n = len(pt_cnt)
+        total = int(max((pt_off + pt_cnt).max(), 1)) if n else 1
+        hc, tc, st = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros(n, np.int32)
+        hi = np.zeros(total, np.uint32)
+        tr = np.zeros(6 * total, np.int32)
+        self._check(self.lib.mfcl_convex_hulls(self.h, _ptr(points), _ptr(pt_off), _ptr(pt_cnt), n, _ptr(hc),
+                                               _ptr(hi), _ptr(tc), _ptr(tr), _ptr(st)))
+        out = []
+        for i in range(n):
+            o = int(pt_off[i])
+            out.append((int(st[i]), hi[o:o + hc[i]].copy(), tr[6 * o:6 * o + 3 * tc[i]].reshape(-1, 3).copy()))
+        return out
 
     # ---- meshes
     def upload_meshes(self, verts, vert_off, vert_cnt, tris, tri_off, tri_cnt):
