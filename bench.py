@@ -1,9 +1,10 @@
 """Benchmark: batched convex-hull GJK/EPA pair queries per second on one MI355X.
 
 Workload (default): 1,048,576 pairs of convex hulls of N points on random
-ellipsoids (axes U[0.05, 0.4] m, N log-uniform in [8, 512]) with their hull
-faces, so hulls above 32 vertices find support points by the vertex walk as
-fcl::Convex does; centres U[0, 2]^3 m, uniform rotations;
+ellipsoids (axes U[0.05, 0.4] m, N log-uniform in [8, 512]), every hull
+built on the device from its own points (mfcl_upload_convex_points) with
+its faces, so hulls above 32 vertices find support points by the vertex
+walk as fcl::Convex does; centres U[0, 2]^3 m, uniform rotations;
 GJKSolver::shapeSignedDistance semantics (GJK distance, EPA depth when
 intersecting).  Inputs are resident in HBM before the timed region; one
 step = one batched query over all pairs.
@@ -65,18 +66,18 @@ def timed_median(fn, n_units, target_s=10.0, runs=5):
     return float(np.median(rates)), k, total
 
 
-def cpu_baseline(h, n_sample, threads, target_cpu_s=10.0):
-    """Oracle FP64 shapeSignedDistance over the first n_sample pairs, with
-    the same hull faces as the device (vertex-adjacency walk above 32
-    vertices, as fcl::Convex): one warm-up pass, then the median of 5 timed
-    runs totalling about target_cpu_s of wall time."""
+def cpu_baseline(h, geom, n_sample, threads, target_cpu_s=10.0):
+    """Oracle FP64 shapeSignedDistance over the first n_sample pairs, on
+    qhull's hulls of their point sets (geom: vertices and triangles, the
+    vertex-adjacency walk above 32 vertices, as fcl::Convex): one warm-up
+    pass, then the median of 5 timed runs totalling about target_cpu_s of
+    wall time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     m = 2 * n_sample
-    nverts = int(h["vert_off"][m - 1] + h["vert_cnt"][m - 1])
-    S = pyoracle.ShapeSet(h["types"][:m], h["params"][:m], h["verts"][:nverts], h["vert_off"][:m],
-                          h["vert_cnt"][:m], h["tris"], h["tri_off"][:m], h["tri_cnt"][:m], threads=threads)
+    v, vo, vc, t, to, tc = geom
+    S = pyoracle.ShapeSet(h["types"][:m], h["params"][:m], v, vo, vc, t, to, tc, threads=threads)
     ta = pyoracle.pose12(h["pose_a"][:n_sample])
     tb = pyoracle.pose12(h["pose_b"][:n_sample])
     pa, pb = h["pair_a"][:n_sample], h["pair_b"][:n_sample]
@@ -143,13 +144,26 @@ def main():
     from mfcl import workloads as W
 
     t0 = time.time()
-    h = W.hull_pairs(args.pairs, args.seed, faces=True)
+    h = W.hull_point_pairs(args.pairs, args.seed)
     n = args.pairs
-    log(f"[gen] {n} pairs, {len(h['verts'])} hull vertices in {time.time() - t0:.1f}s")
+    log(f"[gen] {n} pairs, {len(h['verts'])} points in {time.time() - t0:.1f}s")
+    cpu = host_cpu_info()
+    if args.cpu_threads:
+        cpu["threads"] = args.cpu_threads
+    geom = None
+    if not args.no_cpu:
+        # qhull's hulls of the CPU sample's point sets, in forked workers
+        # before anything touches the GPU
+        t0 = time.time()
+        m = min(args.cpu_pairs, n)
+        geom = W.hulls_of_sets(h["verts"], h["vert_off"], h["vert_cnt"], 2 * m, workers=min(16, cpu["threads"]))
+        log(f"[cpu] qhull hulls of the {2 * m} sample shapes in {time.time() - t0:.1f}s")
     ctx = mfcl.Context(0)
     t0 = time.time()
-    ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"],
-                      faces=mfcl.faces_from_triangles(h["tris"], h["tri_off"], h["tri_cnt"]))
+    # every shape is the hull of its own point set, built on the device
+    hull_cnt = ctx.upload_convex_points(h["verts"], h["vert_off"], h["vert_cnt"])
+    log(f"[hulls] {n * 2} hulls, {int(hull_cnt.sum())} vertices ({int((hull_cnt < h['vert_cnt']).sum())} sets "
+        f"with points inside their hull) built on the device in {time.time() - t0:.2f}s")
     dpa, dpb = ctx.to_device(h["pair_a"]), ctx.to_device(h["pair_b"])
     dqa, dqb = ctx.to_device(h["pose_a"]), ctx.to_device(h["pose_b"])
     dout = ctx.alloc(n * mfcl.CONVEX_RESULT_DTYPE.itemsize)
@@ -183,7 +197,7 @@ def main():
     dout.download(out)
     value = n * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
-    nv = h["vert_cnt"].astype(np.int64)
+    nv = hull_cnt.astype(np.int64)
     # GJK stage, per launch: every FP32 hull vertex of both shapes read once
     # (12 B each) and per pair 56 B of poses, 16 B of shape offsets and
     # counts and a 48 B result
@@ -221,8 +235,9 @@ def main():
         "dtype": "f64 (f32 support scans)",
         "data": "synthetic",
         "config": {
-            "workload": f"{n} convex-hull pairs, N log-uniform [8,512] points on ellipsoids (axes U[0.05,0.4] m) "
-                        "with hull faces (support by the vertex walk above 32 vertices, as fcl::Convex), "
+            "workload": f"{n} convex-hull pairs, every hull built on the device from its own N points "
+                        "(N log-uniform [8,512], on ellipsoids with axes U[0.05,0.4] m), with hull faces "
+                        "(support by the vertex walk above 32 vertices, as fcl::Convex), "
                         "centres U[0,2]^3, GJKSolver::shapeSignedDistance (GJK distance + EPA)",
             "pairs": n,
             "mean_vertices_per_hull": float(nv.mean()),
@@ -246,12 +261,9 @@ def main():
     params_path = os.path.join(ROOT, "mind-fcl_amd", "mfcl", "data", "warmstart_mlp.npz")
     if not args.no_warm and os.path.exists(params_path):
         result["warm_start"] = bench_warm(ctx, args, h, dpa, dpb, dqa, dqb, dout, n, out, value, params_path)
-    cpu = host_cpu_info()
-    if args.cpu_threads:
-        cpu["threads"] = args.cpu_threads
     if not args.no_cpu:
         threads = cpu["threads"]
-        rate, ref, info = cpu_baseline(h, min(args.cpu_pairs, n), threads)
+        rate, ref, info = cpu_baseline(h, geom, min(args.cpu_pairs, n), threads)
         m = min(args.cpu_pairs, n)
         sign_mism = int(np.sum((ref["distance"] < 0) != (out["distance"][:m] < 0)))
         not_exact = int(np.sum((ref["distance"] != out["distance"][:m]) | (ref["ok"] != out["ok"][:m])))

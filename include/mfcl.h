@@ -297,6 +297,19 @@ int mfcl_convex_hulls_dev(mfcl_ctx* ctx, const float* points_xyz, const uint64_t
 int mfcl_convex_hulls(mfcl_ctx* ctx, const float* points_xyz, const uint64_t* pt_off, const uint32_t* pt_cnt,
                       uint32_t n_sets, uint32_t* hull_cnt, uint32_t* hull_idx, uint32_t* tri_cnt, int32_t* tris,
                       int32_t* status);
+/* Builds the shape table from point sets without a host pass over the
+ * hulls: shape i is the convex hull of set i (as mfcl_convex_hulls, 4..768
+ * points), with the hull's vertices in ascending point order and its
+ * triangles as faces.  The result equals mfcl_upload_shapes (every shape
+ * MFCL_SHAPE_CONVEX, zero params) with those vertices followed by
+ * mfcl_upload_convex_faces with those triangles -- what the reference's
+ * callers do when they construct fcl::Convex from a hull
+ * (convex-inl.h:50-73, adjacency convex-inl.h:293-409) -- and replaces any
+ * previous shape table.  hull_cnt: optional host output, hull vertices per
+ * shape.  A set without a hull fails the call (MFCL_E_INVALID; the message
+ * names the set).  Host pointers. */
+int mfcl_upload_convex_points(mfcl_ctx* ctx, const float* points_xyz, const uint64_t* pt_off,
+                              const uint32_t* pt_cnt, uint32_t n_sets, uint32_t* hull_cnt);
 
 /* ---- OBBRSS mesh-mesh --------------------------------------------------- */
 /* Builds one OBB tree per mesh on the host and uploads node and triangle

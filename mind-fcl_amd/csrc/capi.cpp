@@ -734,6 +734,15 @@ int mfcl_convex_hulls(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off
   return MFCL_OK;
 }
 
+int mfcl_upload_convex_points(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                              uint32_t n_sets, uint32_t* hull_cnt) {
+  if (!ctx || !points || !pt_off || !pt_cnt || n_sets == 0) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return uploadConvexPoints(ctx, points, pt_off, pt_cnt, n_sets, hull_cnt);
+}
+
 int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* vert_off, const uint32_t* vert_cnt,
                        const uint32_t* tris, const uint64_t* tri_off, const uint32_t* tri_cnt,
                        uint32_t n_meshes) {

@@ -166,7 +166,11 @@ int uploadVec(mfcl_ctx* ctx, void** dptr, const void* src, size_t bytes);
 int uploadConvexFaces(mfcl_ctx* ctx, const int32_t* faces, const uint64_t* face_off, const uint32_t* num_faces,
                       uint64_t faces_len);
 int computeWalkSeeds(mfcl_ctx* ctx);
+// capi.cpp: frees the shape table (convex tables included)
+void freeShapes(mfcl_ctx* ctx);
 // hull.hip
+int uploadConvexPoints(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                       uint32_t n_sets, uint32_t* hull_cnt_out);
 int hullsLaunch(mfcl_ctx* ctx, const float* pts, const uint64_t* pt_off, const uint32_t* pt_cnt, uint32_t n_sets,
                 int max_points, uint32_t* hull_cnt, uint32_t* hull_idx, uint32_t* tri_cnt, int32_t* tris,
                 int32_t* status, hipStream_t s);

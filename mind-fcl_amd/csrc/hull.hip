@@ -451,4 +451,310 @@ int hullsLaunch(mfcl_ctx* ctx, const float* pts, const uint64_t* pt_off, const u
   return MFCL_OK;
 }
 
+// ---- shape table of the hulls (mfcl_upload_convex_points) ----------------
+//
+// One wavefront per hull, after hullKernel: gathers the hull vertices in
+// ascending point order into the vertex table, and computes what
+// mfcl_upload_shapes computes on the host for a convex shape (interior point
+// = vertex mean, convex-inl.h:64-70; the largest |coordinate| for the FP32
+// support-scan bound; the 16-float descriptor of the warm-start network) and
+// what mfcl_upload_convex_faces builds from the faces for hulls of more than
+// 32 vertices (the sorted neighbour list of every vertex in the reference's
+// encoding, Convex::FindVertexNeighbors, convex-inl.h:377-409).  Sums run
+// sequentially in vertex order, one lane per sum, so they round as the
+// host's loops do.
+__global__ void __launch_bounds__(64) convexTableKernel(const float* pts, const uint64_t* pt_off,
+                                                        const uint32_t* hull_cnt, const uint32_t* hull_idx,
+                                                        const uint32_t* tri_cnt, const int32_t* tris,
+                                                        const uint32_t* voff, const int64_t* nbr_off, int cap,
+                                                        float4* verts, float4* param, double4* interior, float* desc,
+                                                        int32_t* nbr) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) char tab_lds[];
+  const uint32_t set = blockIdx.x;
+  const int l = laneId64();
+  const int nv = static_cast<int>(hull_cnt[set]);
+  const int nt = static_cast<int>(tri_cnt[set]);
+  const uint64_t o = pt_off[set];
+  double* sums = reinterpret_cast<double*>(tab_lds);  // 16 doubles
+  float* x = reinterpret_cast<float*>(sums + 16);
+  float* y = x + cap;
+  float* z = y + cap;
+  int32_t* rec = reinterpret_cast<int32_t*>(z + cap);
+  int32_t* cur = rec + cap;
+  float4* vo = verts + voff[set];
+  float rb = 0.f;
+  for (int k = l; k < nv; k += 64) {
+    const uint64_t q = o + hull_idx[o + k];
+    const float px = pts[3 * q], py = pts[3 * q + 1], pz = pts[3 * q + 2];
+    x[k] = px;
+    y[k] = py;
+    z[k] = pz;
+    vo[k] = make_float4(px, py, pz, 0.f);
+    rb = fmaxf(rb, fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz))));
+  }
+  float mx[3] = {0.f, 0.f, 0.f};
+  for (int k = l; k < nv; k += 64) {
+    mx[0] = fmaxf(mx[0], fabsf(x[k]));
+    mx[1] = fmaxf(mx[1], fabsf(y[k]));
+    mx[2] = fmaxf(mx[2], fabsf(z[k]));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    rb = fmaxf(rb, __shfl_xor(rb, off));
+    mx[0] = fmaxf(mx[0], __shfl_xor(mx[0], off));
+    mx[1] = fmaxf(mx[1], __shfl_xor(mx[1], off));
+    mx[2] = fmaxf(mx[2], __shfl_xor(mx[2], off));
+  }
+  __syncthreads();
+  // coordinate sums (lanes 0-2), radius sum and maximum (lane 3)
+  if (l < 3) {
+    const float* c = l == 0 ? x : (l == 1 ? y : z);
+    double sacc = 0;
+    for (int k = 0; k < nv; ++k) sacc += c[k];
+    sums[l] = sacc;
+  } else if (l == 3) {
+    double rmax = 0, rsum = 0;
+    for (int k = 0; k < nv; ++k) {
+      const double px = x[k], py = y[k], pz = z[k];
+      const double r = sqrt(px * px + py * py + pz * pz);
+      rmax = fmax(rmax, r);
+      rsum += r;
+    }
+    sums[3] = rmax;
+    sums[4] = rsum;
+  }
+  __syncthreads();
+  const double mean[3] = {sums[0] / nv, sums[1] / nv, sums[2] / nv};
+  // second moments about the mean (lanes 0-5)
+  if (l < 6) {
+    const int i0 = l < 3 ? l : (l == 5 ? 1 : 0);
+    const int i1 = l < 3 ? l : (l == 3 ? 1 : 2);
+    const float* c0 = i0 == 0 ? x : (i0 == 1 ? y : z);
+    const float* c1 = i1 == 0 ? x : (i1 == 1 ? y : z);
+    double m = 0;
+    for (int k = 0; k < nv; ++k) {
+      const double u = c0[k] - mean[i0], w = c1[k] - mean[i1];
+      m += u * w;
+    }
+    sums[8 + l] = m;
+  }
+  __syncthreads();
+  if (l == 0) {
+    const double inv = 1.0 / nv;
+    interior[set] = double4{sums[0] * inv, sums[1] * inv, sums[2] * inv, 0};
+    param[set] = make_float4(0.f, 0.f, 0.f, rb);
+  }
+  if (l < 16) {
+    float d;
+    if (l < 3) d = l == 0 ? mx[0] : (l == 1 ? mx[1] : mx[2]);
+    else if (l < 6) d = static_cast<float>(mean[l - 3]);
+    else if (l < 12) d = static_cast<float>(sums[8 + l - 6] / nv);
+    else if (l == 12) d = static_cast<float>(log2(static_cast<double>(nv)) / 9.0);
+    else if (l == 13) d = static_cast<float>(sums[3]);
+    else if (l == 14) d = static_cast<float>(sums[4] / nv);
+    else d = 1.f;
+    desc[16 * static_cast<uint64_t>(set) + l] = d;
+  }
+  const int64_t no = nbr_off[set];
+  if (no < 0) return;
+  // neighbour records: on a closed, consistently oriented triangulation the
+  // neighbours of a vertex are the successors of it in its triangles, each
+  // once, so its degree is its triangle count
+  int32_t* tab = nbr + no;
+  const int32_t* tr = tris + 6 * o;
+  for (int v = l; v < nv; v += 64) cur[v] = 0;
+  __syncthreads();
+  for (int t = l; t < nt; t += 64) {
+    atomicAdd(&cur[tr[3 * t]], 1);
+    atomicAdd(&cur[tr[3 * t + 1]], 1);
+    atomicAdd(&cur[tr[3 * t + 2]], 1);
+  }
+  __syncthreads();
+  int run = nv;
+  for (int base = 0; base < nv; base += 64) {
+    const int v = base + l;
+    const int deg = v < nv ? cur[v] : 0;
+    const int len = v < nv ? 1 + deg : 0;
+    int incl = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off);
+      if (l >= off) incl += t;
+    }
+    if (v < nv) {
+      const int r = run + incl - len;
+      rec[v] = r;
+      tab[v] = r;
+      tab[r] = deg;
+    }
+    run += __shfl(incl, 63);
+  }
+  __syncthreads();
+  for (int v = l; v < nv; v += 64) cur[v] = 0;
+  __syncthreads();
+  for (int t = l; t < nt; t += 64) {
+    const int a = tr[3 * t], b = tr[3 * t + 1], c = tr[3 * t + 2];
+    tab[rec[a] + 1 + atomicAdd(&cur[a], 1)] = b;
+    tab[rec[b] + 1 + atomicAdd(&cur[b], 1)] = c;
+    tab[rec[c] + 1 + atomicAdd(&cur[c], 1)] = a;
+  }
+  __syncthreads();
+  for (int v = l; v < nv; v += 64) {
+    int32_t* r = tab + rec[v] + 1;
+    const int deg = cur[v];
+    for (int i = 1; i < deg; ++i) {
+      const int32_t key = r[i];
+      int j = i - 1;
+      while (j >= 0 && r[j] > key) {
+        r[j + 1] = r[j];
+        --j;
+      }
+      r[j + 1] = key;
+    }
+  }
+}
+
+namespace {
+// Device allocations of one call, released on every return.
+struct TempDev {
+  std::vector<void*> p;
+  ~TempDev() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <class T>
+  T* get(mfcl_ctx* ctx, size_t count, int& rc) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      ctx->err = "out of device memory building the convex shape table";
+      rc = MFCL_E_NOMEM;
+      return nullptr;
+    }
+    p.push_back(q);
+    return static_cast<T*>(q);
+  }
+};
+
+template <class T>
+int allocTable(mfcl_ctx* ctx, T** dst, size_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->err = "out of device memory building the convex shape table";
+    return MFCL_E_NOMEM;
+  }
+  *dst = static_cast<T*>(q);
+  return MFCL_OK;
+}
+}  // namespace
+
+int uploadConvexPoints(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                       uint32_t n_sets, uint32_t* hull_cnt_out) {
+  uint64_t total = 0;
+  uint32_t maxp = 4;
+  for (uint32_t i = 0; i < n_sets; ++i) {
+    if (pt_cnt[i] > static_cast<uint32_t>(kHullMaxPoints)) {
+      ctx->err = "point set " + std::to_string(i) + " has more than " + std::to_string(kHullMaxPoints) + " points";
+      return MFCL_E_INVALID;
+    }
+    total = std::max<uint64_t>(total, pt_off[i] + pt_cnt[i]);
+    maxp = std::max(maxp, pt_cnt[i]);
+  }
+  const hipStream_t s = ctx->stream;
+  int rc = MFCL_OK;
+  TempDev tmp;
+  float* dp = tmp.get<float>(ctx, 3 * total, rc);
+  uint64_t* doff = rc ? nullptr : tmp.get<uint64_t>(ctx, n_sets, rc);
+  uint32_t* dcnt = rc ? nullptr : tmp.get<uint32_t>(ctx, n_sets, rc);
+  uint32_t* dhc = rc ? nullptr : tmp.get<uint32_t>(ctx, n_sets, rc);
+  uint32_t* dtc = rc ? nullptr : tmp.get<uint32_t>(ctx, n_sets, rc);
+  int32_t* dst = rc ? nullptr : tmp.get<int32_t>(ctx, n_sets, rc);
+  uint32_t* dhi = rc ? nullptr : tmp.get<uint32_t>(ctx, total, rc);
+  int32_t* dtr = rc ? nullptr : tmp.get<int32_t>(ctx, 6 * total, rc);
+  if (rc) return rc;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dp, points, 12 * total, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(doff, pt_off, 8 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dcnt, pt_cnt, 4 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice, s));
+  if ((rc = hullsLaunch(ctx, dp, doff, dcnt, n_sets, static_cast<int>(maxp), dhc, dhi, dtc, dtr, dst, s))) return rc;
+  std::vector<uint32_t> hc(n_sets), tc(n_sets);
+  std::vector<int32_t> st(n_sets);
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(hc.data(), dhc, 4 * static_cast<size_t>(n_sets), hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(tc.data(), dtc, 4 * static_cast<size_t>(n_sets), hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(st.data(), dst, 4 * static_cast<size_t>(n_sets), hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < n_sets; ++i) {
+    if (st[i] == kHullOk) continue;
+    static const char* why[] = {"", "is degenerate (fewer than 4 points, or all on one plane)",
+                                "has too many points", "has no hull within the tolerance (near-degenerate input)"};
+    ctx->err = "point set " + std::to_string(i) + " " + why[st[i] >= 1 && st[i] <= 3 ? st[i] : 3];
+    return MFCL_E_INVALID;
+  }
+  // vertex and adjacency offsets; a hull's record area holds nv offsets and
+  // nv records of 1 + degree entries, and the degrees sum to 3 nt
+  std::vector<uint32_t> voff(n_sets);
+  std::vector<int64_t> noff(n_sets, -1);
+  uint64_t nverts = 0, nnbr = 0;
+  for (uint32_t i = 0; i < n_sets; ++i) {
+    voff[i] = static_cast<uint32_t>(nverts);
+    nverts += hc[i];
+    if (hc[i] > 32) {
+      noff[i] = static_cast<int64_t>(nnbr);
+      nnbr += 2 * static_cast<uint64_t>(hc[i]) + 3 * static_cast<uint64_t>(tc[i]);
+    }
+  }
+  if (nverts > 0xFFFFFFFFull || nnbr > 0x7FFFFFFFull) {
+    ctx->err = "convex tables exceed 2^32 vertices or 2^31 adjacency entries";
+    return MFCL_E_INVALID;
+  }
+  uint32_t* dvoff = tmp.get<uint32_t>(ctx, n_sets, rc);
+  if (rc) return rc;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dvoff, voff.data(), 4 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice, s));
+  freeShapes(ctx);
+  const std::vector<int32_t> type(n_sets, MFCL_SHAPE_CONVEX);
+  auto fail = [&](int r) {
+    freeShapes(ctx);
+    return r;
+  };
+  if ((rc = allocTable(ctx, &ctx->d_type, n_sets)) || (rc = allocTable(ctx, &ctx->d_param, n_sets)) ||
+      (rc = allocTable(ctx, &ctx->d_voff, n_sets)) || (rc = allocTable(ctx, &ctx->d_vcnt, n_sets)) ||
+      (rc = allocTable(ctx, &ctx->d_verts, nverts)) || (rc = allocTable(ctx, &ctx->d_interior, n_sets)) ||
+      (rc = allocTable(ctx, &ctx->d_desc, 16 * static_cast<size_t>(n_sets))) ||
+      (rc = allocTable(ctx, &ctx->d_nbr, nnbr + 32)) || (rc = allocTable(ctx, &ctx->d_nbr_off, n_sets)) ||
+      (rc = allocTable(ctx, &ctx->d_seed, 8 * static_cast<size_t>(n_sets))))
+    return fail(rc);
+  // walk blocks: 128 B per vertex; when the device cannot hold them the
+  // walks read the adjacency table instead (same results, slower)
+  if (hipMalloc(reinterpret_cast<void**>(&ctx->d_wblk), 8 * sizeof(float4) * std::max<uint64_t>(nverts, 1)) !=
+      hipSuccess) {
+    ctx->d_wblk = nullptr;
+    (void)hipGetLastError();
+  }
+  auto chk = [&](hipError_t e) {
+    if (e == hipSuccess) return MFCL_OK;
+    ctx->err = std::string("HIP error: ") + hipGetErrorString(e);
+    return MFCL_E_HIP;
+  };
+  if ((rc = chk(hipMemcpyAsync(ctx->d_type, type.data(), 4 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice, s))) ||
+      (rc = chk(hipMemcpyAsync(ctx->d_voff, voff.data(), 4 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice, s))) ||
+      (rc = chk(hipMemcpyAsync(ctx->d_vcnt, dhc, 4 * static_cast<size_t>(n_sets), hipMemcpyDeviceToDevice, s))) ||
+      (rc = chk(hipMemcpyAsync(ctx->d_nbr_off, noff.data(), 8 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice,
+                               s))) ||
+      (rc = chk(hipMemsetAsync(ctx->d_nbr + nnbr, 0, 32 * sizeof(int32_t), s))))
+    return fail(rc);
+  const size_t lds = 20 * static_cast<size_t>(maxp) + 16 * sizeof(double) + 64;
+  convexTableKernel<<<n_sets, 64, lds, s>>>(dp, doff, dhc, dhi, dtc, dtr, dvoff, ctx->d_nbr_off,
+                                             static_cast<int>(maxp), ctx->d_verts, ctx->d_param, ctx->d_interior,
+                                             ctx->d_desc, ctx->d_nbr);
+  if ((rc = chk(hipGetLastError()))) return fail(rc);
+  ctx->n_shapes = n_sets;
+  ctx->n_verts_alloc = std::max<uint64_t>(nverts, 1);
+  ctx->h_vcnt = hc;
+  ctx->h_voff = voff;
+  ctx->h_type = type;
+  if ((rc = computeWalkSeeds(ctx))) return fail(rc);
+  if (hull_cnt_out) std::memcpy(hull_cnt_out, hc.data(), 4 * static_cast<size_t>(n_sets));
+  return MFCL_OK;
+}
+
 }  // namespace mfcl

@@ -165,6 +165,7 @@ def load_library() -> C.CDLL:
     lib.mfcl_reserve_arm.argtypes = [p, C.POINTER(Arm), u64]
     lib.mfcl_convex_hulls.argtypes = [p, p, p, p, u32, p, p, p, p, p]
     lib.mfcl_convex_hulls_dev.argtypes = [p, p, p, p, u32, i32, p, p, p, p, p, p]
+    lib.mfcl_upload_convex_points.argtypes = [p, p, p, p, u32, p]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
     lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
     lib.mfcl_mesh_collide.argtypes = [p, p, p, p, p, u64, i32, p, p]
@@ -436,6 +437,18 @@ class Context:
             o = int(pt_off[i])
             out.append((int(st[i]), hi[o:o + hc[i]].copy(), tr[6 * o:6 * o + 3 * tc[i]].reshape(-1, 3).copy()))
         return out
+
+    def upload_convex_points(self, points, pt_off, pt_cnt) -> np.ndarray:
+        """Shape table of the convex hulls of point sets, built on the device
+        (vertices in ascending point order, hull triangles as faces).
+        Returns the hull vertex count of every shape."""
+        points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        pt_off = np.ascontiguousarray(pt_off, np.uint64)
+        pt_cnt = np.ascontiguousarray(pt_cnt, np.uint32)
+        hc = np.zeros(len(pt_cnt), np.uint32)
+        self._check(self.lib.mfcl_upload_convex_points(self.h, _ptr(points), _ptr(pt_off), _ptr(pt_cnt),
+                                                       len(pt_cnt), _ptr(hc)))
+        return hc
 
     # ---- meshes
     def upload_meshes(self, verts, vert_off, vert_cnt, tris, tri_off, tri_cnt):

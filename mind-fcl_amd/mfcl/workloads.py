@@ -197,6 +197,69 @@ def hull_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int =
                 pair_a=pa, pair_b=pb, pose_a=posea, pose_b=poseb, **extra)
 
 
+def hull_point_pairs(n_pairs: int = 1 << 20, seed: int = 2, nmin: int = 8, nmax: int = 512):
+    """The batched convex-hull workload with an independent point set per
+    hull (shape 2i, 2i+1 of pair i): N log-uniform in [nmin, nmax] points on
+    an ellipsoid with axes U[0.05, 0.4] m, centres U[0, 2]^3 m, uniform
+    rotations.  The hulls (vertices and faces) are built from the points by
+    the library (mfcl_upload_convex_points); verts/vert_off/vert_cnt hold
+    the points."""
+    rng = np.random.default_rng(seed)
+    n = 2 * n_pairs
+    verts, off, cnt, axes = ellipsoid_hulls(rng, n, nmin, nmax)
+    types = np.full(n, CONVEX, np.int32)
+    params = np.zeros((n, 4), np.float32)
+    pa = np.arange(0, n, 2, dtype=np.uint32)
+    pb = np.arange(1, n, 2, dtype=np.uint32)
+    posea = poses(rng, n_pairs, 0.0, 2.0)
+    poseb = poses(rng, n_pairs, 0.0, 2.0)
+    return dict(types=types, params=params, verts=verts, vert_off=off, vert_cnt=cnt, axes=axes, pair_a=pa,
+                pair_b=pb, pose_a=posea, pose_b=poseb)
+
+
+def _hull_faces_range(args):
+    verts, off, cnt, lo, hi = args
+    return [hull_faces(verts[int(off[i]):int(off[i]) + int(cnt[i])]) for i in range(lo, hi)]
+
+
+def hulls_of_sets(verts, off, cnt, n_sets: int, workers: int = 1):
+    """qhull hulls of the first n_sets point sets, as shape arrays:
+    (vertices, vert_off, vert_cnt, triangles, tri_off, tri_cnt), vertices in
+    ascending point order per set.  workers > 1 runs qhull in that many
+    forked processes (call before anything has touched the GPU)."""
+    jobs = []
+    step = max(1, (n_sets + 4 * max(workers, 1) - 1) // (4 * max(workers, 1)))
+    for lo in range(0, n_sets, step):
+        hi = min(n_sets, lo + step)
+        o0 = int(off[lo])
+        o1 = int(off[hi - 1]) + int(cnt[hi - 1])
+        jobs.append((verts[o0:o1], np.asarray(off[lo:hi], np.int64) - o0, cnt[lo:hi], 0, hi - lo))
+    if workers > 1:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(workers) as pool:
+            parts = pool.map(_hull_faces_range, jobs)
+    else:
+        parts = [_hull_faces_range(j) for j in jobs]
+    hv, hvc, ht, htc = [], [], [], []
+    k = 0
+    for part in parts:
+        for keep, tris in part:
+            o = int(off[k])
+            hv.append(verts[o + keep])
+            hvc.append(len(keep))
+            ht.append(tris)
+            htc.append(len(tris))
+            k += 1
+    hvc = np.array(hvc, np.uint32)
+    htc = np.array(htc, np.int64)
+    hvo = np.zeros(n_sets, np.uint64)
+    hvo[1:] = np.cumsum(hvc[:-1], dtype=np.uint64)
+    hto = np.zeros(n_sets, np.int64)
+    hto[1:] = np.cumsum(htc[:-1])
+    return np.concatenate(hv), hvo, hvc, np.concatenate(ht), hto, htc
+
+
 def hull_faces(v: np.ndarray):
     """Convex hull of a point set: (indices of hull vertices, triangles over
     those indices renumbered 0..k-1).  Uses qhull through scipy."""

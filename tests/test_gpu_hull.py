@@ -84,3 +84,84 @@ def test_device_hulls_degenerate(gpu_ctx):
     off[1:] = np.cumsum(cnt[:-1])
     res = gpu_ctx.convex_hulls(np.concatenate(sets), off, cnt)
     assert [r[0] for r in res] == [1, 1, 1, 1, 2]
+
+
+def _result_bytes(a):
+    return np.ascontiguousarray(a).view(np.uint8)
+
+
+@pytest.mark.gpu
+def test_convex_points_table_equals_host_path(gpu_ctx):
+    """mfcl_upload_convex_points (hull, vertex table, descriptors and
+    adjacency built on the device) against the host path of the reference's
+    callers: qhull's hull, mfcl_upload_shapes with its vertices and
+    mfcl_upload_convex_faces with its triangles.  Every query result must be
+    byte-identical (the walk tables, seeds, interior points and bounds are
+    then the same), the shape descriptors equal to float rounding."""
+    import mfcl
+
+    rng = np.random.default_rng(52)
+    v, off, cnt, _ = W.ellipsoid_hulls(rng, 1024, 8, 512)
+    sets = [v[off[i]:off[i] + cnt[i]] for i in range(len(cnt))]
+    for n in (12, 40, 300, 768):  # cube points: interior points are not hull vertices
+        sets.append(rng.uniform(-0.3, 0.3, (n, 3)).astype(np.float32))
+    cnt = np.array([len(s) for s in sets], np.uint32)
+    off = np.zeros(len(sets), np.uint64)
+    off[1:] = np.cumsum(cnt[:-1])
+    pts = np.concatenate(sets)
+    hc = gpu_ctx.upload_convex_points(pts, off, cnt)
+    # host path
+    hv, hvo, hvc, ht, hto, htc = [], [], [], [], [], []
+    vo = to = 0
+    for P in sets:
+        keep, tris = W.hull_faces(P)
+        hv.append(P[keep]); hvo.append(vo); hvc.append(len(keep)); vo += len(keep)
+        ht.append(tris); hto.append(to); htc.append(len(tris)); to += len(tris)
+    assert np.array_equal(hc, np.array(hvc, np.uint32))
+    n = len(sets)
+    host = mfcl.Context(0)
+    try:
+        host.upload_shapes(np.full(n, W.CONVEX, np.int32), np.zeros((n, 4), np.float32), np.array(hvo, np.uint64),
+                           np.array(hvc, np.uint32), np.concatenate(hv),
+                           faces=mfcl.faces_from_triangles(np.concatenate(ht), np.array(hto), np.array(htc)))
+        m = 16384
+        pa = rng.integers(0, n, m).astype(np.uint32)
+        pb = rng.integers(0, n, m).astype(np.uint32)
+        qa, qb = W.poses(rng, m, 0.0, 0.6), W.poses(rng, m, 0.0, 0.6)
+        n_hit = 0
+        for mode, pen in ((mfcl.MODE_SIGNED_DISTANCE, 0), (mfcl.MODE_COLLIDE, 1)):
+            req = mfcl.request(mode, penetration=pen)
+            a = gpu_ctx.convex_query(pa, pb, qa, qb, req)
+            b = host.convex_query(pa, pb, qa, qb, req)
+            assert np.array_equal(_result_bytes(a), _result_bytes(b)), mode
+            if mode == mfcl.MODE_COLLIDE:
+                n_hit = int(a["hit"].sum())
+        feats = []
+        for c in (gpu_ctx, host):
+            d = [c.to_device(x) for x in (pa, pb, qa, qb)]
+            f = c.alloc(m * 64 * 4)
+            c.mlp_features_dev(*d, m, f)
+            feats.append(f.download(np.zeros((m, 64), np.float32)))
+            for x in d + [f]:
+                x.free()
+        ferr = np.abs(feats[0] - feats[1]).max()
+        assert ferr <= 1e-6, ferr
+        print(f"\n[device shape table] {n} hulls ({int(hc.sum())} vertices, {int((hc > 32).sum())} walking), "
+              f"{m} pairs x 2 modes byte-identical to the host path, {n_hit} colliding; descriptor max diff {ferr:.1e}")
+    finally:
+        host.close()
+
+
+@pytest.mark.gpu
+def test_convex_points_rejects_bad_sets(gpu_ctx):
+    rng = np.random.default_rng(53)
+    good = rng.uniform(-1, 1, (20, 3)).astype(np.float32)
+    flat = good.copy()
+    flat[:, 2] = 0.5
+    for bad, msg in ((flat, "point set 1 is degenerate"), (rng.uniform(-1, 1, (800, 3)).astype(np.float32),
+                                                           "more than 768")):
+        sets = [good, bad]
+        cnt = np.array([len(s) for s in sets], np.uint32)
+        off = np.array([0, len(good)], np.uint64)
+        with pytest.raises(RuntimeError, match=msg):
+            gpu_ctx.upload_convex_points(np.concatenate(sets), off, cnt)
