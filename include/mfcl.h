@@ -321,6 +321,49 @@ int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* ve
                        const uint32_t* tri_cnt, uint32_t n_meshes);
 /* Number of BVH nodes of mesh i (host query). */
 int mfcl_mesh_num_nodes(mfcl_ctx* ctx, uint32_t mesh, uint32_t* n_nodes);
+/* Copies mesh i's tree to host memory: mfcl_mesh_num_nodes() nodes of 64
+ * bytes, breadth-first (float axes[9], column-major; float centre[3]; float
+ * half extents[3]; int32 child: >= 0 first child, its sibling next; < 0 leaf
+ * of triangle -(child + 1)). */
+int mfcl_mesh_get_nodes(mfcl_ctx* ctx, uint32_t mesh, void* nodes);
+/* Deforming meshes: BVHModel::beginReplaceModel / replaceSubModel /
+ * endReplaceModel(refit) (include/fcl/geometry/bvh/BVH_model-inl.h:
+ * 306-375) with every vertex of mesh i replaced (vert_cnt[i] xyz triples,
+ * the triangles kept).  refit != 0: the tree is refitted on the device
+ * (refitTree, BVH_model-inl.h:572-640) -- every node keeps its axes and
+ * triangles and gets the centre and extents that contain them; refit == 0:
+ * the tree is rebuilt (on the host, as mfcl_upload_meshes builds it).  Host
+ * pointer; synchronises. */
+int mfcl_mesh_replace_vertices(mfcl_ctx* ctx, uint32_t mesh, const float* verts_xyz, int32_t refit);
+/* The refit with a device pointer, enqueued on stream (hipStream_t or
+ * NULL): a vertex copy and three kernels, no allocation, capturable. */
+int mfcl_mesh_replace_vertices_dev(mfcl_ctx* ctx, uint32_t mesh, const float* verts_xyz, void* stream);
+/* Mesh files.  mfcl_load_obj reads a Wavefront OBJ file as the reference's
+ * tests do (test/test_fcl_utility.h:250-316, loadOBJFile): "v" lines give
+ * vertices, "f" lines 1-based vertex ids (each token's integer prefix); a
+ * face of k ids gives k - 2 triangles, the fan (0, t+1, t+2) once a "vn" or
+ * "vt" line was read and otherwise k - 2 copies of (0, 1, 2), as there.
+ * Call it with verts_xyz = tris = NULL to get *n_verts / *n_tris, then with
+ * buffers of those sizes.  No context needed. */
+int mfcl_load_obj(const char* path, float* verts_xyz, uint64_t* n_verts, uint32_t* tris, uint64_t* n_tris);
+/* No counterpart in the reference (which builds every tree from vertex and
+ * triangle data): the uploaded mesh set -- current vertices, triangles and
+ * built trees -- written to a flat binary file, and read back into a
+ * context without a tree build.  mfcl_load_meshes replaces the mesh set and
+ * rejects files that are truncated, corrupted (checksum) or malformed
+ * (MFCL_E_INVALID). */
+int mfcl_save_meshes(mfcl_ctx* ctx, const char* path);
+int mfcl_load_meshes(mfcl_ctx* ctx, const char* path);
+/* The same file built on the host alone (trees built as mfcl_upload_meshes
+ * builds them), e.g. offline, and its header read back after a full check
+ * (counts; MFCL_E_INVALID for a damaged file).  No context or GPU needed;
+ * mfcl_host_last_error() describes the last failure of these and of
+ * mfcl_load_obj on the calling thread. */
+int mfcl_build_mesh_file(const char* path, const float* verts_xyz, const uint64_t* vert_off,
+                         const uint32_t* vert_cnt, const uint32_t* tris, const uint64_t* tri_off,
+                         const uint32_t* tri_cnt, uint32_t n_meshes);
+int mfcl_mesh_file_info(const char* path, uint32_t* n_meshes, uint64_t* n_tris, uint64_t* n_nodes);
+const char* mfcl_host_last_error(void);
 /* Batched fcl::collide(BVHModel<OBBRSS>, BVHModel<OBBRSS>) with
  * CollisionRequest(max_contacts), penetration disabled
  * (narrowphase/detail/traversal/collision/bvh_solver-inl.h:75-164).

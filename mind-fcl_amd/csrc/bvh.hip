@@ -82,7 +82,8 @@ struct Builder {
   const float* v;   // mesh vertices xyz
   const uint32_t* t;  // mesh triangles (local ids)
   std::vector<HV> cen;  // centroids
-  std::vector<uint32_t> prim;
+  std::vector<uint32_t> prim;   // triangle ids, leaf order once built
+  std::vector<uint32_t> range;  // per node: first slot of prim it covers, count
 
   HV vert(uint32_t i) const { return {v[3 * i], v[3 * i + 1], v[3 * i + 2]}; }
 
@@ -171,8 +172,11 @@ struct Builder {
     fifo.reserve(2 * n_tris);
     nodes.emplace_back();
     fifo.push_back({0, 0, n_tris});
+    range.assign(2 * (2 * static_cast<size_t>(n_tris) - 1), 0);
     for (size_t h = 0; h < fifo.size(); ++h) {
       const R r = fifo[h];
+      range[2 * r.node] = r.first;
+      range[2 * r.node + 1] = r.count;
       HV axes[3];
       ObbNode nd = fit(prim.data() + r.first, static_cast<int>(r.count), axes);
       if (r.count == 1) {
@@ -197,68 +201,313 @@ struct Builder {
 
 }  // namespace
 
-int buildAndUploadMeshes(mfcl_ctx* ctx, const float* verts, const uint64_t* voff, const uint32_t* vcnt,
-                         const uint32_t* tris, const uint64_t* toff, const uint32_t* tcnt, uint32_t n) {
-  std::vector<std::vector<ObbNode>> per(n);
+int buildMeshSet(std::string& err, const float* verts, const uint64_t* voff, const uint32_t* vcnt,
+                 const uint32_t* tris, const uint64_t* toff, const uint32_t* tcnt, uint32_t n, MeshSetHost& m) {
   for (uint32_t i = 0; i < n; ++i) {
     if (tcnt[i] == 0 || vcnt[i] == 0) {
-      ctx->err = "mesh " + std::to_string(i) + " is empty";
+      err = "mesh " + std::to_string(i) + " is empty";
       return MFCL_E_INVALID;
     }
     for (uint64_t k = 0; k < 3ull * tcnt[i]; ++k)
       if (tris[3 * toff[i] + k] >= vcnt[i]) {
-        ctx->err = "mesh " + std::to_string(i) + " has a triangle vertex id out of range";
+        err = "mesh " + std::to_string(i) + " has a triangle vertex id out of range";
         return MFCL_E_INVALID;
       }
   }
+  std::vector<std::vector<ObbNode>> per(n);
+  std::vector<std::vector<uint32_t>> rng(n), prm(n);
   const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
   {
     std::vector<std::thread> th;
     for (unsigned w = 0; w < nth; ++w)
       th.emplace_back([&, w]() {
         for (uint32_t i = w; i < n; i += nth) {
-          Builder b{verts + 3 * voff[i], tris + 3 * toff[i], {}, {}};
+          Builder b{verts + 3 * voff[i], tris + 3 * toff[i], {}, {}, {}};
           b.build(tcnt[i], per[i]);
+          rng[i] = std::move(b.range);
+          prm[i] = std::move(b.prim);
         }
       });
     for (auto& x : th) x.join();
   }
-  std::vector<uint32_t> node_base(n), tri_base(n), node_count(n);
-  uint64_t total_nodes = 0, total_tris = 0;
+  m = MeshSetHost();
+  m.vert_base.resize(n);
+  m.vert_count.assign(vcnt, vcnt + n);
+  m.tri_base.resize(n);
+  m.tri_count.assign(tcnt, tcnt + n);
+  m.node_base.resize(n);
+  m.node_count.resize(n);
+  uint64_t nv = 0, nt = 0, nn = 0;
   for (uint32_t i = 0; i < n; ++i) {
-    node_base[i] = static_cast<uint32_t>(total_nodes);
-    node_count[i] = static_cast<uint32_t>(per[i].size());
-    tri_base[i] = static_cast<uint32_t>(total_tris);
-    total_nodes += per[i].size();
-    total_tris += tcnt[i];
+    m.vert_base[i] = nv;
+    m.tri_base[i] = static_cast<uint32_t>(nt);
+    m.node_base[i] = static_cast<uint32_t>(nn);
+    m.node_count[i] = static_cast<uint32_t>(per[i].size());
+    nv += vcnt[i];
+    nt += tcnt[i];
+    nn += per[i].size();
   }
-  if (total_nodes > 0x7FFFFFFFull || total_tris > 0x7FFFFFFFull) {
-    ctx->err = "mesh set too large (more than 2^31 nodes or triangles)";
+  if (nn > 0x7FFFFFFFull || nt > 0x7FFFFFFFull) {
+    err = "mesh set too large (more than 2^31 nodes or triangles)";
     return MFCL_E_INVALID;
   }
-  std::vector<ObbNode> nodes(total_nodes);
-  std::vector<float4> tri4(3 * total_tris);
+  m.verts.resize(3 * nv);
+  m.tris.resize(3 * nt);
+  m.nodes.resize(nn * sizeof(ObbNode));
+  m.node_range.resize(2 * nn);
+  m.prim.resize(nt);
   for (uint32_t i = 0; i < n; ++i) {
-    std::memcpy(nodes.data() + node_base[i], per[i].data(), per[i].size() * sizeof(ObbNode));
-    for (uint32_t k = 0; k < tcnt[i]; ++k)
-      for (int j = 0; j < 3; ++j) {
-        const float* p = verts + 3 * (voff[i] + tris[3 * (toff[i] + k) + j]);
-        tri4[3 * (tri_base[i] + k) + j] = float4{p[0], p[1], p[2], 0.f};
-      }
+    std::memcpy(m.verts.data() + 3 * m.vert_base[i], verts + 3 * voff[i], 12 * static_cast<size_t>(vcnt[i]));
+    std::memcpy(m.tris.data() + 3 * static_cast<size_t>(m.tri_base[i]), tris + 3 * toff[i],
+                12 * static_cast<size_t>(tcnt[i]));
+    std::memcpy(m.nodes.data() + sizeof(ObbNode) * m.node_base[i], per[i].data(), per[i].size() * sizeof(ObbNode));
+    std::memcpy(m.node_range.data() + 2 * static_cast<size_t>(m.node_base[i]), rng[i].data(), 8 * per[i].size());
+    std::memcpy(m.prim.data() + m.tri_base[i], prm[i].data(), 4 * static_cast<size_t>(tcnt[i]));
   }
-  MFCL_HIP_TRY(ctx, hipMalloc(&ctx->d_nodes, nodes.size() * sizeof(ObbNode)));
-  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_nodes, nodes.data(), nodes.size() * sizeof(ObbNode), hipMemcpyHostToDevice));
-  MFCL_HIP_TRY(ctx, hipMalloc(&ctx->d_tris, tri4.size() * sizeof(float4)));
-  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_tris, tri4.data(), tri4.size() * sizeof(float4), hipMemcpyHostToDevice));
-  MFCL_HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_mesh_node_base), n * sizeof(uint32_t)));
-  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_mesh_node_base, node_base.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-  MFCL_HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_mesh_node_count), n * sizeof(uint32_t)));
-  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_mesh_node_count, node_count.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-  MFCL_HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->d_mesh_tri_base), n * sizeof(uint32_t)));
-  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_mesh_tri_base, tri_base.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-  ctx->mesh_host.node_base = node_base;
-  ctx->mesh_host.node_count = node_count;
+  return MFCL_OK;
+}
+
+// Nodes over more triangles than this are refitted by a whole wavefront.
+constexpr uint32_t kRefitLaneMax = 32;
+
+int uploadMeshSet(mfcl_ctx* ctx, const MeshSetHost& m) {
+  const uint32_t n = static_cast<uint32_t>(m.node_base.size());
+  const size_t nn = m.nodes.size() / sizeof(ObbNode), nt = m.prim.size();
+  MeshHost h;
+  h.node_base = m.node_base;
+  h.node_count = m.node_count;
+  h.tri_base = m.tri_base;
+  h.tri_count = m.tri_count;
+  h.vert_base = m.vert_base;
+  h.vert_count = m.vert_count;
+  h.large_base.resize(n);
+  h.large_count.resize(n);
+  std::vector<uint32_t> large;
+  for (uint32_t i = 0; i < n; ++i) {
+    h.large_base[i] = static_cast<uint32_t>(large.size());
+    for (uint32_t k = 0; k < m.node_count[i]; ++k)
+      if (m.node_range[2 * (static_cast<size_t>(m.node_base[i]) + k) + 1] > kRefitLaneMax)
+        large.push_back(m.node_base[i] + k);
+    h.large_count[i] = static_cast<uint32_t>(large.size()) - h.large_base[i];
+  }
+  // triangles with their vertices gathered, as the traversal reads them
+  std::vector<float4> tri4(3 * nt);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t k = 0; k < m.tri_count[i]; ++k)
+      for (int j = 0; j < 3; ++j) {
+        const size_t t = static_cast<size_t>(m.tri_base[i]) + k;
+        const float* p = m.verts.data() + 3 * (m.vert_base[i] + m.tris[3 * t + j]);
+        tri4[3 * t + j] = float4{p[0], p[1], p[2], 0.f};
+      }
+  auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+    MFCL_HIP_TRY(ctx, hipMalloc(dst, std::max<size_t>(bytes, 4)));
+    if (bytes) MFCL_HIP_TRY(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return MFCL_OK;
+  };
+  int rc;
+  if ((rc = up(&ctx->d_nodes, m.nodes.data(), m.nodes.size())) ||
+      (rc = up(&ctx->d_tris, tri4.data(), tri4.size() * sizeof(float4))) ||
+      (rc = up(reinterpret_cast<void**>(&ctx->d_mesh_node_base), m.node_base.data(), 4 * static_cast<size_t>(n))) ||
+      (rc = up(reinterpret_cast<void**>(&ctx->d_mesh_node_count), m.node_count.data(), 4 * static_cast<size_t>(n))) ||
+      (rc = up(reinterpret_cast<void**>(&ctx->d_mesh_tri_base), m.tri_base.data(), 4 * static_cast<size_t>(n))) ||
+      (rc = up(reinterpret_cast<void**>(&ctx->d_mesh_verts), m.verts.data(), 4 * m.verts.size())) ||
+      (rc = up(reinterpret_cast<void**>(&ctx->d_mesh_tri_idx), m.tris.data(), 4 * m.tris.size())) ||
+      (rc = up(reinterpret_cast<void**>(&ctx->d_node_range), m.node_range.data(), 4 * m.node_range.size())) ||
+      (rc = up(reinterpret_cast<void**>(&ctx->d_mesh_prim), m.prim.data(), 4 * m.prim.size())) ||
+      (rc = up(reinterpret_cast<void**>(&ctx->d_refit_large), large.data(), 4 * large.size())))
+    return rc;
+  (void)nn;
+  ctx->mesh_host = std::move(h);
   ctx->n_meshes = n;
+  return MFCL_OK;
+}
+
+int buildAndUploadMeshes(mfcl_ctx* ctx, const float* verts, const uint64_t* voff, const uint32_t* vcnt,
+                         const uint32_t* tris, const uint64_t* toff, const uint32_t* tcnt, uint32_t n) {
+  MeshSetHost m;
+  if (int rc = buildMeshSet(ctx->err, verts, voff, vcnt, tris, toff, tcnt, n, m)) return rc;
+  return uploadMeshSet(ctx, m);
+}
+
+int downloadMeshSet(mfcl_ctx* ctx, MeshSetHost& m) {
+  const MeshHost& h = ctx->mesh_host;
+  const uint32_t n = ctx->n_meshes;
+  m = MeshSetHost();
+  m.vert_base = h.vert_base;
+  m.vert_count = h.vert_count;
+  m.tri_base = h.tri_base;
+  m.tri_count = h.tri_count;
+  m.node_base = h.node_base;
+  m.node_count = h.node_count;
+  const uint64_t nv = n ? h.vert_base[n - 1] + h.vert_count[n - 1] : 0;
+  const uint64_t nt = n ? static_cast<uint64_t>(h.tri_base[n - 1]) + h.tri_count[n - 1] : 0;
+  const uint64_t nn = n ? static_cast<uint64_t>(h.node_base[n - 1]) + h.node_count[n - 1] : 0;
+  m.verts.resize(3 * nv);
+  m.tris.resize(3 * nt);
+  m.nodes.resize(nn * sizeof(ObbNode));
+  m.node_range.resize(2 * nn);
+  m.prim.resize(nt);
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  MFCL_HIP_TRY(ctx, hipMemcpy(m.verts.data(), ctx->d_mesh_verts, 4 * m.verts.size(), hipMemcpyDeviceToHost));
+  MFCL_HIP_TRY(ctx, hipMemcpy(m.tris.data(), ctx->d_mesh_tri_idx, 4 * m.tris.size(), hipMemcpyDeviceToHost));
+  MFCL_HIP_TRY(ctx, hipMemcpy(m.nodes.data(), ctx->d_nodes, m.nodes.size(), hipMemcpyDeviceToHost));
+  MFCL_HIP_TRY(ctx, hipMemcpy(m.node_range.data(), ctx->d_node_range, 4 * m.node_range.size(), hipMemcpyDeviceToHost));
+  MFCL_HIP_TRY(ctx, hipMemcpy(m.prim.data(), ctx->d_mesh_prim, 4 * m.prim.size(), hipMemcpyDeviceToHost));
+  return MFCL_OK;
+}
+
+// ------------------------------------------------------------ refit
+// BVHModel::beginReplaceModel / replaceSubModel / endReplaceModel(refit)
+// (BVH_model-inl.h:306-375, refit 572-640) keep the tree and recompute
+// every node's volume for the new vertices.  Here every node keeps its axes
+// (FP32, as stored) and the triangles it covers, and gets the centre and
+// half extents of their vertices' projections on those axes, padded by the
+// same rule as the build, so it contains its triangles again; the
+// traversal's pair sets stay exact because the leaf tests decide them.
+
+// Centre and half extents from the projection ranges on the node's axes.
+MFCL_D void refitWrite(ObbNode& nd, const double lo[3], const double hi[3]) {
+  double cx = 0, cy = 0, cz = 0, emax = 0;
+  for (int a = 0; a < 3; ++a) {
+    const double mid = 0.5 * (lo[a] + hi[a]);
+    cx += static_cast<double>(nd.ax[3 * a]) * mid;
+    cy += static_cast<double>(nd.ax[3 * a + 1]) * mid;
+    cz += static_cast<double>(nd.ax[3 * a + 2]) * mid;
+    emax = fmax(emax, 0.5 * (hi[a] - lo[a]));
+  }
+  const double pad = (fabs(cx) + fabs(cy) + fabs(cz) + 3 * emax) * 1.5e-5 + 1e-12;
+  for (int a = 0; a < 3; ++a) nd.e[a] = static_cast<float>(0.5 * (hi[a] - lo[a]) + pad);
+  nd.c[0] = static_cast<float>(cx);
+  nd.c[1] = static_cast<float>(cy);
+  nd.c[2] = static_cast<float>(cz);
+}
+
+MFCL_D void refitAccum(const ObbNode& nd, const float4* tri, double lo[3], double hi[3]) {
+  for (int j = 0; j < 3; ++j) {
+    const float4 p = tri[j];
+    for (int a = 0; a < 3; ++a) {
+      const double pr = static_cast<double>(nd.ax[3 * a]) * p.x + static_cast<double>(nd.ax[3 * a + 1]) * p.y +
+                        static_cast<double>(nd.ax[3 * a + 2]) * p.z;
+      lo[a] = fmin(lo[a], pr);
+      hi[a] = fmax(hi[a], pr);
+    }
+  }
+}
+
+// New vertices into the vertex copy and the gathered triangles.
+__global__ void meshGatherKernel(const float* v, const uint32_t* idx, uint32_t n_tris, float4* tri4) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tris) return;
+  for (int j = 0; j < 3; ++j) {
+    const uint32_t k = idx[3 * t + j];
+    tri4[3 * static_cast<uint64_t>(t) + j] = make_float4(v[3 * k], v[3 * k + 1], v[3 * k + 2], 0.f);
+  }
+}
+
+// One lane per node over at most kRefitLaneMax triangles.
+__global__ void refitLaneKernel(ObbNode* nodes, const uint32_t* range, const uint32_t* prim, const float4* tri4,
+                                uint32_t node0, uint32_t n_nodes) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_nodes) return;
+  const uint32_t g = node0 + k;
+  const uint32_t first = range[2 * g], count = range[2 * g + 1];
+  if (count > kRefitLaneMax) return;
+  ObbNode nd = nodes[g];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = 0; i < count; ++i) refitAccum(nd, tri4 + 3 * static_cast<uint64_t>(prim[first + i]), lo, hi);
+  refitWrite(nd, lo, hi);
+  for (int a = 0; a < 3; ++a) {
+    nodes[g].c[a] = nd.c[a];
+    nodes[g].e[a] = nd.e[a];
+  }
+}
+
+// One wavefront per listed node.
+__global__ void __launch_bounds__(64) refitWaveKernel(ObbNode* nodes, const uint32_t* range, const uint32_t* prim,
+                                                      const float4* tri4, const uint32_t* list) {
+  const uint32_t g = list[blockIdx.x];
+  const uint32_t first = range[2 * g], count = range[2 * g + 1];
+  const int l = threadIdx.x;
+  ObbNode nd = nodes[g];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = l; i < count; i += 64) refitAccum(nd, tri4 + 3 * static_cast<uint64_t>(prim[first + i]), lo, hi);
+  for (int off = 32; off > 0; off >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fmin(lo[a], __shfl_xor(lo[a], off));
+      hi[a] = fmax(hi[a], __shfl_xor(hi[a], off));
+    }
+  if (l != 0) return;
+  refitWrite(nd, lo, hi);
+  for (int a = 0; a < 3; ++a) {
+    nodes[g].c[a] = nd.c[a];
+    nodes[g].e[a] = nd.e[a];
+  }
+}
+
+static int meshGather(mfcl_ctx* ctx, uint32_t mesh, const float* verts, bool device_ptr, hipStream_t s) {
+  const MeshHost& h = ctx->mesh_host;
+  float* vdst = ctx->d_mesh_verts + 3 * h.vert_base[mesh];
+  const size_t vbytes = 12 * static_cast<size_t>(h.vert_count[mesh]);
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(vdst, verts, vbytes, device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                   s));
+  const uint32_t tb = h.tri_base[mesh], nt = h.tri_count[mesh];
+  float4* tri4 = static_cast<float4*>(ctx->d_tris) + 3 * static_cast<uint64_t>(tb);
+  meshGatherKernel<<<(nt + 255) / 256, 256, 0, s>>>(vdst, ctx->d_mesh_tri_idx + 3 * static_cast<uint64_t>(tb), nt,
+                                                    tri4);
+  MFCL_HIP_TRY(ctx, hipGetLastError());
+  return MFCL_OK;
+}
+
+int meshReplaceVertices(mfcl_ctx* ctx, uint32_t mesh, const float* verts, bool device_ptr, hipStream_t s) {
+  if (int rc = meshGather(ctx, mesh, verts, device_ptr, s)) return rc;
+  const MeshHost& h = ctx->mesh_host;
+  const uint32_t tb = h.tri_base[mesh];
+  float4* tri4 = static_cast<float4*>(ctx->d_tris) + 3 * static_cast<uint64_t>(tb);
+  ObbNode* nodes = static_cast<ObbNode*>(ctx->d_nodes);
+  const uint32_t nb = h.node_base[mesh], nn = h.node_count[mesh];
+  refitLaneKernel<<<(nn + 255) / 256, 256, 0, s>>>(nodes, ctx->d_node_range, ctx->d_mesh_prim + tb, tri4, nb, nn);
+  if (h.large_count[mesh])
+    refitWaveKernel<<<h.large_count[mesh], 64, 0, s>>>(nodes, ctx->d_node_range, ctx->d_mesh_prim + tb, tri4,
+                                                       ctx->d_refit_large + h.large_base[mesh]);
+  MFCL_HIP_TRY(ctx, hipGetLastError());
+  return MFCL_OK;
+}
+
+// endReplaceModel(refit = false): the tree is rebuilt on the host from the
+// new vertices; it has the same node count, so it replaces the old one in
+// place.
+int meshRebuild(mfcl_ctx* ctx, uint32_t mesh, const float* verts_host) {
+  const MeshHost& h = ctx->mesh_host;
+  const uint32_t nt = h.tri_count[mesh], tb = h.tri_base[mesh], nb = h.node_base[mesh];
+  std::vector<uint32_t> idx(3 * static_cast<size_t>(nt));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  MFCL_HIP_TRY(ctx, hipMemcpy(idx.data(), ctx->d_mesh_tri_idx + 3 * static_cast<uint64_t>(tb), 12 * static_cast<size_t>(nt),
+                              hipMemcpyDeviceToHost));
+  Builder b{verts_host, idx.data(), {}, {}, {}};
+  std::vector<ObbNode> nodes;
+  b.build(nt, nodes);
+  if (nodes.size() != h.node_count[mesh]) {
+    ctx->err = "rebuilt tree has a different node count";
+    return MFCL_E_STATE;
+  }
+  MFCL_HIP_TRY(ctx, hipMemcpy(static_cast<ObbNode*>(ctx->d_nodes) + nb, nodes.data(), nodes.size() * sizeof(ObbNode),
+                              hipMemcpyHostToDevice));
+  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_node_range + 2 * static_cast<uint64_t>(nb), b.range.data(), 4 * b.range.size(),
+                              hipMemcpyHostToDevice));
+  MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_mesh_prim + tb, b.prim.data(), 4 * static_cast<size_t>(nt), hipMemcpyHostToDevice));
+  // the wavefront-refit list of this mesh may change with the partition
+  std::vector<uint32_t> large;
+  for (uint32_t k = 0; k < nodes.size(); ++k)
+    if (b.range[2 * k + 1] > kRefitLaneMax) large.push_back(nb + k);
+  if (large.size() != h.large_count[mesh]) {
+    ctx->err = "rebuilt tree has a different refit layout";
+    return MFCL_E_STATE;
+  }
+  if (!large.empty())
+    MFCL_HIP_TRY(ctx, hipMemcpy(ctx->d_refit_large + h.large_base[mesh], large.data(), 4 * large.size(),
+                                hipMemcpyHostToDevice));
+  if (int rc = meshGather(ctx, mesh, verts_host, false, ctx->stream)) return rc;
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return MFCL_OK;
 }
 

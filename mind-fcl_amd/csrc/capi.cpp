@@ -60,6 +60,11 @@ void freeMeshes(mfcl_ctx* c) {
   freePtr(reinterpret_cast<void*&>(c->d_mesh_node_base));
   freePtr(reinterpret_cast<void*&>(c->d_mesh_node_count));
   freePtr(reinterpret_cast<void*&>(c->d_mesh_tri_base));
+  freePtr(reinterpret_cast<void*&>(c->d_mesh_verts));
+  freePtr(reinterpret_cast<void*&>(c->d_mesh_tri_idx));
+  freePtr(reinterpret_cast<void*&>(c->d_node_range));
+  freePtr(reinterpret_cast<void*&>(c->d_mesh_prim));
+  freePtr(reinterpret_cast<void*&>(c->d_refit_large));
   c->n_meshes = 0;
   c->mesh_host = MeshHost();
 }
@@ -759,6 +764,102 @@ int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* ve
 int mfcl_mesh_num_nodes(mfcl_ctx* ctx, uint32_t mesh, uint32_t* n_nodes) {
   if (!ctx || !n_nodes || mesh >= ctx->n_meshes) return MFCL_E_INVALID;
   *n_nodes = ctx->mesh_host.node_count[mesh];
+  return MFCL_OK;
+}
+
+int mfcl_mesh_get_nodes(mfcl_ctx* ctx, uint32_t mesh, void* nodes) {
+  if (!ctx || !nodes || mesh >= ctx->n_meshes) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  MFCL_HIP_TRY(ctx, hipMemcpy(nodes, static_cast<char*>(ctx->d_nodes) + 64ull * ctx->mesh_host.node_base[mesh],
+                              64ull * ctx->mesh_host.node_count[mesh], hipMemcpyDeviceToHost));
+  return MFCL_OK;
+}
+
+int mfcl_mesh_replace_vertices(mfcl_ctx* ctx, uint32_t mesh, const float* verts_xyz, int32_t refit) {
+  if (!ctx || !verts_xyz || mesh >= ctx->n_meshes) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (!refit) return meshRebuild(ctx, mesh, verts_xyz);
+  if (int rc = meshReplaceVertices(ctx, mesh, verts_xyz, false, ctx->stream)) return rc;
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MFCL_OK;
+}
+
+int mfcl_mesh_replace_vertices_dev(mfcl_ctx* ctx, uint32_t mesh, const float* verts_xyz, void* stream) {
+  if (!ctx || !verts_xyz || mesh >= ctx->n_meshes) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return meshReplaceVertices(ctx, mesh, verts_xyz, true, stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+int mfcl_save_meshes(mfcl_ctx* ctx, const char* path) {
+  if (!ctx || !path) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  if (ctx->n_meshes == 0) {
+    ctx->err = "no meshes uploaded";
+    return MFCL_E_STATE;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MeshSetHost m;
+  if (int rc = downloadMeshSet(ctx, m)) return rc;
+  return writeMeshFile(m, path, ctx->err);
+}
+
+int mfcl_load_meshes(mfcl_ctx* ctx, const char* path) {
+  if (!ctx || !path) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MeshSetHost m;
+  if (int rc = readMeshFile(path, m, ctx->err)) return rc;
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  freeMeshes(ctx);
+  if (int rc = uploadMeshSet(ctx, m)) {
+    freeMeshes(ctx);
+    return rc;
+  }
+  return reserveMesh(ctx);
+}
+
+// Host-only entry points report errors through their return code alone
+// (there is no context to hold a message); the last message is kept here.
+static thread_local std::string g_host_err;
+const char* mfcl_host_last_error(void) { return g_host_err.c_str(); }
+
+int mfcl_build_mesh_file(const char* path, const float* verts_xyz, const uint64_t* vert_off, const uint32_t* vert_cnt,
+                         const uint32_t* tris, const uint64_t* tri_off, const uint32_t* tri_cnt, uint32_t n_meshes) {
+  if (!path || !verts_xyz || !vert_off || !vert_cnt || !tris || !tri_off || !tri_cnt || n_meshes == 0)
+    return MFCL_E_INVALID;
+  MeshSetHost m;
+  if (int rc = buildMeshSet(g_host_err, verts_xyz, vert_off, vert_cnt, tris, tri_off, tri_cnt, n_meshes, m))
+    return rc;
+  return writeMeshFile(m, path, g_host_err);
+}
+
+int mfcl_mesh_file_info(const char* path, uint32_t* n_meshes, uint64_t* n_tris, uint64_t* n_nodes) {
+  if (!path || !n_meshes || !n_tris || !n_nodes) return MFCL_E_INVALID;
+  MeshSetHost m;
+  if (int rc = readMeshFile(path, m, g_host_err)) return rc;
+  *n_meshes = static_cast<uint32_t>(m.node_base.size());
+  *n_tris = m.prim.size();
+  *n_nodes = m.nodes.size() / 64;
+  return MFCL_OK;
+}
+
+int mfcl_load_obj(const char* path, float* verts_xyz, uint64_t* n_verts, uint32_t* tris, uint64_t* n_tris) {
+  if (!path || !n_verts || !n_tris) return MFCL_E_INVALID;
+  std::vector<float> v;
+  std::vector<uint32_t> t;
+  if (int rc = loadObj(path, v, t, g_host_err)) return rc;
+  const uint64_t nv = v.size() / 3, nt = t.size() / 3;
+  if (verts_xyz || tris) {
+    if (!verts_xyz || !tris || *n_verts < nv || *n_tris < nt) return MFCL_E_INVALID;
+    std::memcpy(verts_xyz, v.data(), 4 * v.size());
+    std::memcpy(tris, t.data(), 4 * t.size());
+  }
+  *n_verts = nv;
+  *n_tris = nt;
   return MFCL_OK;
 }
 

@@ -129,6 +129,29 @@ int uploadConvexFaces(mfcl_ctx* ctx, const int32_t* faces, const uint64_t* face_
     }
     off[i] = it->second;
   }
+  // the walk data (record offsets, walk blocks) is stored per vertex, so
+  // walking shapes whose vertex ranges overlap must be the same hull with
+  // the same adjacency
+  {
+    std::vector<std::tuple<uint32_t, uint32_t, int64_t, uint32_t>> r;
+    for (uint32_t i = 0; i < n; ++i)
+      if (off[i] >= 0) r.emplace_back(ctx->h_voff[i], ctx->h_vcnt[i], off[i], i);
+    std::sort(r.begin(), r.end());
+    uint64_t end = 0;
+    for (size_t k = 0; k < r.size(); ++k) {
+      const auto [vo, vc, to, id] = r[k];
+      if (k > 0) {
+        const auto [po, pc, pt, pid] = r[k - 1];
+        const bool same = vo == po && vc == pc && to == pt;
+        if (!same && vo < end) {
+          ctx->err = "convex shapes " + std::to_string(pid) + " and " + std::to_string(id) +
+                     " share vertices but not their faces (the support walk needs one adjacency per vertex)";
+          return MFCL_E_INVALID;
+        }
+      }
+      end = std::max<uint64_t>(end, static_cast<uint64_t>(vo) + vc);
+    }
+  }
   if (all.size() > 0x7FFFFFFFull) {
     ctx->err = "adjacency tables exceed 2^31 entries";
     return MFCL_E_INVALID;

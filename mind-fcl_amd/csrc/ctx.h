@@ -55,6 +55,29 @@ struct MlpState {
 struct MeshHost {
   std::vector<uint32_t> node_base;  // first node of each mesh
   std::vector<uint32_t> node_count;
+  std::vector<uint32_t> tri_base;   // first triangle (and leaf-order slot) of each mesh
+  std::vector<uint32_t> tri_count;
+  std::vector<uint64_t> vert_base;  // first vertex of each mesh in the vertex copy
+  std::vector<uint32_t> vert_count;
+  std::vector<uint32_t> large_base;  // refit: each mesh's nodes over many triangles,
+  std::vector<uint32_t> large_count; // a slice of the device list d_refit_large
+};
+
+// A mesh set as stored in host memory (built trees, or read from a file):
+// vertices (xyz per mesh, concatenated), triangles (local vertex ids),
+// 64-byte OBB nodes breadth-first, per node the range of leaf-order
+// triangle slots it covers (first relative to its mesh, count), and per
+// mesh the leaf-order triangle ids.
+struct MeshSetHost {
+  std::vector<float> verts;
+  std::vector<uint64_t> vert_base;
+  std::vector<uint32_t> vert_count;
+  std::vector<uint32_t> tris;
+  std::vector<uint32_t> tri_base, tri_count;
+  std::vector<char> nodes;            // 64 B per node
+  std::vector<uint32_t> node_base, node_count;
+  std::vector<uint32_t> node_range;   // 2 per node: first, count
+  std::vector<uint32_t> prim;         // per triangle slot
 };
 
 // Layers of a heightmap, top (0) to bottom (n_layers - 1), as
@@ -115,6 +138,11 @@ struct mfcl_ctx {
   uint32_t* d_mesh_node_base{nullptr};
   uint32_t* d_mesh_node_count{nullptr};
   uint32_t* d_mesh_tri_base{nullptr};
+  float* d_mesh_verts{nullptr};         // current vertices (xyz), for refits and saves
+  uint32_t* d_mesh_tri_idx{nullptr};    // triangles: 3 local vertex ids
+  uint32_t* d_node_range{nullptr};      // per node: first leaf-order slot (mesh-relative), count
+  uint32_t* d_mesh_prim{nullptr};       // per leaf-order slot: mesh-local triangle id
+  uint32_t* d_refit_large{nullptr};     // nodes refitted by a whole wavefront
   mfcl::MeshHost mesh_host;
 
   // broadphase tree
@@ -207,6 +235,17 @@ uint64_t staticScratchMaxPairs();
 int reserveConvex(mfcl_ctx* ctx, uint64_t n);
 // Mesh traversal stacks for the current tune knobs (bvh.hip; host call).
 int reserveMesh(mfcl_ctx* ctx);
+// bvh.hip: OBB trees of a mesh set on the host; upload of a built set
+int buildMeshSet(std::string& err, const float* verts, const uint64_t* voff, const uint32_t* vcnt,
+                 const uint32_t* tris, const uint64_t* toff, const uint32_t* tcnt, uint32_t n, MeshSetHost& out);
+int uploadMeshSet(mfcl_ctx* ctx, const MeshSetHost& m);
+int downloadMeshSet(mfcl_ctx* ctx, MeshSetHost& m);
+int meshReplaceVertices(mfcl_ctx* ctx, uint32_t mesh, const float* verts, bool device_ptr, hipStream_t s);
+int meshRebuild(mfcl_ctx* ctx, uint32_t mesh, const float* verts_host);
+// meshio.cpp
+int loadObj(const char* path, std::vector<float>& v, std::vector<uint32_t>& t, std::string& err);
+int writeMeshFile(const MeshSetHost& m, const char* path, std::string& err);
+int readMeshFile(const char* path, MeshSetHost& m, std::string& err);
 
 // Launchers (narrowphase.hip, bvh.hip, broadphase.hip).
 int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,

@@ -168,6 +168,16 @@ def load_library() -> C.CDLL:
     lib.mfcl_upload_convex_points.argtypes = [p, p, p, p, u32, p]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
     lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
+    lib.mfcl_mesh_get_nodes.argtypes = [p, u32, p]
+    lib.mfcl_mesh_replace_vertices.argtypes = [p, u32, p, i32]
+    lib.mfcl_mesh_replace_vertices_dev.argtypes = [p, u32, p, p]
+    lib.mfcl_save_meshes.argtypes = [p, C.c_char_p]
+    lib.mfcl_load_meshes.argtypes = [p, C.c_char_p]
+    lib.mfcl_load_obj.argtypes = [C.c_char_p, p, C.POINTER(u64), p, C.POINTER(u64)]
+    lib.mfcl_build_mesh_file.argtypes = [C.c_char_p, p, p, p, p, p, p, u32]
+    lib.mfcl_mesh_file_info.argtypes = [C.c_char_p, C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)]
+    lib.mfcl_host_last_error.argtypes = []
+    lib.mfcl_host_last_error.restype = C.c_char_p
     lib.mfcl_mesh_collide.argtypes = [p, p, p, p, p, u64, i32, p, p]
     lib.mfcl_mesh_collide_dev.argtypes = [p, p, p, p, p, u64, i32, p, p, p]
     lib.mfcl_mesh_collide_contacts.argtypes = [p, p, p, p, p, u64, i32, p, p]
@@ -201,6 +211,47 @@ def faces_from_triangles(tris, tri_off, tri_cnt):
     faces[:, 0] = 3
     faces[:, 1:] = tris
     return faces.reshape(-1), 4 * np.asarray(tri_off, np.uint64), np.asarray(tri_cnt, np.uint32)
+
+
+OBB_NODE_DTYPE = np.dtype([("ax", "<f4", 9), ("c", "<f4", 3), ("e", "<f4", 3), ("child", "<i4")])
+
+
+def _host_check(lib, rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {lib.mfcl_host_last_error().decode()}")
+
+
+def load_obj(path: str):
+    """Vertices (n x 3 float32) and triangles (m x 3 uint32, 0-based) of a
+    Wavefront OBJ file, read as the reference's test loader reads it."""
+    lib = load_library()
+    nv, nt = C.c_uint64(0), C.c_uint64(0)
+    _host_check(lib, lib.mfcl_load_obj(path.encode(), None, C.byref(nv), None, C.byref(nt)), "mfcl_load_obj")
+    v = np.zeros((nv.value, 3), np.float32)
+    t = np.zeros((nt.value, 3), np.uint32)
+    _host_check(lib, lib.mfcl_load_obj(path.encode(), _ptr(v), C.byref(nv), _ptr(t), C.byref(nt)), "mfcl_load_obj")
+    return v, t
+
+
+def build_mesh_file(path: str, verts, vert_off, vert_cnt, tris, tri_off, tri_cnt):
+    """OBB trees of a mesh set built on the host and written to a mesh-set
+    file (loadable with Context.load_meshes); no GPU needed."""
+    lib = load_library()
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    vo, vc = np.ascontiguousarray(vert_off, np.uint64), np.ascontiguousarray(vert_cnt, np.uint32)
+    to, tc = np.ascontiguousarray(tri_off, np.uint64), np.ascontiguousarray(tri_cnt, np.uint32)
+    _host_check(lib, lib.mfcl_build_mesh_file(path.encode(), _ptr(verts), _ptr(vo), _ptr(vc), _ptr(tris), _ptr(to),
+                                              _ptr(tc), len(vc)), "mfcl_build_mesh_file")
+
+
+def mesh_file_info(path: str):
+    """(meshes, triangles, nodes) of a mesh-set file after a full check."""
+    lib = load_library()
+    n, nt, nn = C.c_uint32(0), C.c_uint64(0), C.c_uint64(0)
+    _host_check(lib, lib.mfcl_mesh_file_info(path.encode(), C.byref(n), C.byref(nt), C.byref(nn)),
+                "mfcl_mesh_file_info")
+    return n.value, nt.value, nn.value
 
 
 def request(mode: int = MODE_COLLIDE, **kw) -> Request:
@@ -451,6 +502,26 @@ class Context:
         return hc
 
     # ---- meshes
+    def mesh_get_nodes(self, mesh: int) -> np.ndarray:
+        n = C.c_uint32(0)
+        self._check(self.lib.mfcl_mesh_num_nodes(self.h, mesh, C.byref(n)))
+        out = np.zeros(n.value, OBB_NODE_DTYPE)
+        self._check(self.lib.mfcl_mesh_get_nodes(self.h, mesh, _ptr(out)))
+        return out
+
+    def mesh_replace_vertices(self, mesh: int, verts, refit: bool = True):
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        self._check(self.lib.mfcl_mesh_replace_vertices(self.h, mesh, _ptr(verts), 1 if refit else 0))
+
+    def mesh_replace_vertices_dev(self, mesh: int, verts: DeviceArray, stream=None):
+        self._check(self.lib.mfcl_mesh_replace_vertices_dev(self.h, mesh, verts.ptr, stream))
+
+    def save_meshes(self, path: str):
+        self._check(self.lib.mfcl_save_meshes(self.h, path.encode()))
+
+    def load_meshes(self, path: str):
+        self._check(self.lib.mfcl_load_meshes(self.h, path.encode()))
+
     def upload_meshes(self, verts, vert_off, vert_cnt, tris, tri_off, tri_cnt):
         verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)

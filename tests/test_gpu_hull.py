@@ -165,3 +165,27 @@ def test_convex_points_rejects_bad_sets(gpu_ctx):
         off = np.array([0, len(good)], np.uint64)
         with pytest.raises(RuntimeError, match=msg):
             gpu_ctx.upload_convex_points(np.concatenate(sets), off, cnt)
+
+
+@pytest.mark.gpu
+def test_shared_vertices_need_shared_faces(gpu_ctx):
+    """The walk data lives per vertex: two walking hulls over one vertex
+    range must have one face list (the same one twice is fine)."""
+    import mfcl
+
+    rng = np.random.default_rng(54)
+    v, off, cnt, _ = W.ellipsoid_hulls(rng, 2, 40, 40)
+    P = v[:40]
+    _, t1 = W.hull_faces(P)
+    _, t2 = W.hull_faces(v[40:80])  # another 40-vertex triangulation
+    types = np.full(2, W.CONVEX, np.int32)
+    params = np.zeros((2, 4), np.float32)
+    voff = np.zeros(2, np.uint64)
+    vcnt = np.full(2, 40, np.uint32)
+    tris = np.concatenate([t1, t2])
+    gpu_ctx.upload_shapes(types, params, voff, vcnt, P,
+                          faces=mfcl.faces_from_triangles(tris, np.array([0, 0]), np.array([len(t1), len(t1)])))
+    with pytest.raises(RuntimeError, match="share vertices but not their faces"):
+        gpu_ctx.upload_shapes(types, params, voff, vcnt, P,
+                              faces=mfcl.faces_from_triangles(tris, np.array([0, len(t1)]),
+                                                              np.array([len(t1), len(t2)])))
