@@ -278,6 +278,28 @@ int mfcl_convex_collide_contacts_dev(mfcl_ctx* ctx, const uint32_t* pair_a, cons
                                      uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
                                      mfcl_contact* contacts, void* stream);
 
+/* fcl::collisionPenetration with CollisionPenetrationMode DirectedPenetration
+ * (include/fcl/narrowphase/collision_penetration-inl.h:196-244) over shape
+ * pairs: the boolean collide of req (penetration disabled), then for each
+ * colliding pair the MPR penetration search along dir3[i] -- a world unit
+ * vector, the direction shape b escapes along -- (MPR::
+ * RunDirectedPenetration, include/fcl/cvx_collide/mpr.hpp:496-704; MPR(128,
+ * req->epa_tolerance) as there).  The pair's contact: depth = the distance
+ * along dir3 that separates the shapes (-1 when the search fails), pos =
+ * the witness midpoint (0 on failure), normal = dir3 (computePenetrationMPR,
+ * :105-194); distance = -depth; epa_status = MPR::DirectedPenetrationStatus
+ * (0 OK, 1 Failed, 2 FailedNoIntersect, 3 FailedRefinementIterationLimit),
+ * epa_iters = its refinement iterations.  req->mode must be
+ * MFCL_MODE_COLLIDE.  Host pointers; the _dev form takes device pointers and
+ * needs mfcl_reserve(ctx, MFCL_WS_CONVEX, >= n). */
+int mfcl_convex_directed_penetration(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                                     const float* pose_a7, const float* pose_b7, const float* dir3, uint64_t n,
+                                     const mfcl_request* req, mfcl_convex_result* out);
+int mfcl_convex_directed_penetration_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                                         const float* pose_a7, const float* pose_b7, const float* dir3,
+                                         uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
+                                         void* stream);
+
 /* ---- convex hulls on the device ----------------------------------------- */
 /* The convex hull of each point set (what a caller of fcl::Convex computes
  * before constructing it from the hull's vertices and faces,

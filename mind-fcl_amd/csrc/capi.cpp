@@ -680,6 +680,74 @@ int mfcl_convex_collide_contacts(mfcl_ctx* ctx, const uint32_t* pa, const uint32
   return MFCL_OK;
 }
 
+int mfcl_convex_directed_penetration_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb,
+                                         const float* posea, const float* poseb, const float* dir3, uint64_t n,
+                                         const mfcl_request* req, mfcl_convex_result* out, void* stream) {
+  if (!ctx || !req) return MFCL_E_INVALID;
+  if (n && !dir3) {
+    ctx->err = "null device pointer";
+    return MFCL_E_INVALID;
+  }
+  if (req->mode != MFCL_MODE_COLLIDE) {
+    ctx->err = "directed penetration needs a collide request";
+    return MFCL_E_INVALID;
+  }
+  mfcl_request boolean = *req;
+  boolean.penetration = 0;
+  boolean.use_guess = 0;
+  // the boolean collide first (timed as the query's stages), then the pass
+  if (int rc = mfcl_convex_collide_contacts_dev(ctx, pa, pb, posea, poseb, nullptr, n, &boolean, out, nullptr, stream))
+    return rc;
+  if (n == 0) return MFCL_OK;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  // MPR(128, distanceTolerance) as collisionPenetrationMPR builds it (:208-210)
+  return directedLaunch(ctx, pa, pb, posea, poseb, dir3, n, 128, req->epa_tolerance, out, s);
+}
+
+int mfcl_convex_directed_penetration(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                                     const float* poseb, const float* dir3, uint64_t n, const mfcl_request* req,
+                                     mfcl_convex_result* out) {
+  if (!ctx || !req) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  if (n == 0) return MFCL_OK;
+  if (!pa || !pb || !posea || !poseb || !dir3 || !out) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (n > 0xFFFFFFFFull) {
+    ctx->err = "too many pairs in one query";
+    return MFCL_E_INVALID;
+  }
+  if (int rc = reserveConvex(ctx, n)) return rc;
+  for (uint64_t i = 0; i < n; ++i)
+    if (pa[i] >= ctx->n_shapes || pb[i] >= ctx->n_shapes) {
+      ctx->err = "pair " + std::to_string(i) + " references a shape id out of range";
+      return MFCL_E_INVALID;
+    }
+  const size_t b_ids = 4 * n, b_pose = 28 * n, b_dir = 12 * n, b_out = sizeof(mfcl_convex_result) * n;
+  if (int rc = ensure(ctx, ctx->ws_io, 2 * b_ids + 2 * b_pose + b_dir + b_out + 6 * 256)) return rc;
+  char* base = static_cast<char*>(ctx->ws_io.p);
+  auto carve = [&](size_t bytes) {
+    char* p = base;
+    base += (bytes + 255) & ~static_cast<size_t>(255);
+    return p;
+  };
+  uint32_t* dpa = reinterpret_cast<uint32_t*>(carve(b_ids));
+  uint32_t* dpb = reinterpret_cast<uint32_t*>(carve(b_ids));
+  float* dqa = reinterpret_cast<float*>(carve(b_pose));
+  float* dqb = reinterpret_cast<float*>(carve(b_pose));
+  float* dd = reinterpret_cast<float*>(carve(b_dir));
+  mfcl_convex_result* dout = reinterpret_cast<mfcl_convex_result*>(carve(b_out));
+  hipStream_t s = ctx->stream;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpa, pa, b_ids, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dpb, pb, b_ids, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqa, posea, b_pose, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqb, poseb, b_pose, hipMemcpyHostToDevice, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(dd, dir3, b_dir, hipMemcpyHostToDevice, s));
+  if (int rc = mfcl_convex_directed_penetration_dev(ctx, dpa, dpb, dqa, dqb, dd, n, req, dout, s)) return rc;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(out, dout, b_out, hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return MFCL_OK;
+}
+
 int mfcl_convex_hulls_dev(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off, const uint32_t* pt_cnt,
                           uint32_t n_sets, int32_t max_points, uint32_t* hull_cnt, uint32_t* hull_idx,
                           uint32_t* tri_cnt, int32_t* tris, int32_t* status, void* stream) {

@@ -1398,4 +1398,138 @@ MFCL_D int mprIntersect(MK& md, int max_iter, double tol) {
   return kMprFailed;
 }
 
+// MPR penetration along a unit direction d (shape-0 frame): the distance
+// along d from the origin to where the ray leaves the Minkowski difference,
+// by a portal search from a mock start -d (MPR::RunDirectedPenetration,
+// mpr.hpp:496-632), and the witness points as the barycentric blend of the
+// final portal's support points (finalizeDirectedPenetrationResult,
+// mpr.hpp:642-704).  Status codes follow MPR::DirectedPenetrationStatus.
+enum DirStatus : int { kDirOK = 0, kDirFailed = 1, kDirNoIntersect = 2, kDirIterLimit = 3 };
+struct DirOut {
+  double distance;
+  d3 p0, p1;
+  int iters;
+};
+
+template <class MK>
+MFCL_D void mprDirectedResult(MK& md, d3 d, d3 n, d3 v1, d3 v2, d3 v3, d3 d1, d3 d2, d3 d3v, const d3* v4,
+                              DirOut& o) {
+  const double nd = dot(d, n);
+  if (fabs(nd) <= 0) {
+    const double e1 = dot(v1, d), e2 = dot(v2, d), e3 = dot(v3, d);
+    double best = -INFINITY;
+    d3 bd = d1;
+    if (e1 > best) { best = e1; bd = d1; }
+    if (e2 > best) { best = e2; bd = d2; }
+    if (e3 > best) { best = e3; bd = d3v; }
+    o.distance = best;
+    o.p0 = md.support0(bd);
+    o.p1 = md.support1(-bd);
+    return;
+  }
+  const double to_portal = dot(v1, n) / nd;
+  o.distance = v4 ? dot(*v4, n) / nd : to_portal;
+  const d3 q = to_portal * d;
+  const d3 e12 = v2 - v1, e13 = v3 - v1;
+  const double area = norm(cross(e12, e13));
+  const double w2 = norm(cross(e13, v1 - q)) / area;
+  const double w3 = norm(cross(e12, v1 - q)) / area;
+  const double w1 = 1.0 - w2 - w3;
+  o.p0 = md.support0(d1) * w1 + md.support0(d2) * w2 + md.support0(d3v) * w3;
+  o.p1 = md.support1(-d1) * w1 + md.support1(-d2) * w2 + md.support1(-d3v) * w3;
+}
+
+template <class MK>
+MFCL_D int mprDirected(MK& md, d3 d, int max_iter, double tol, DirOut& o) {
+  constexpr double kDotEps = 2.220446049250313e-16;
+  o.iters = 0;
+  const d3 v0 = -d;
+  d3 d1 = d;
+  d3 v1 = mprSupport(md, d1);
+  if (dot(d1, v1) < 0) return kDirNoIntersect;
+  d3 d2 = cross(v0, v1);
+  if (absnorm(d2) <= absnorm(v1) * tol) {
+    o.distance = dot(v1, d);
+    o.p0 = md.support0(d1);
+    o.p1 = md.support1(-d1);
+    return kDirOK;
+  }
+  d3 v2 = mprSupport(md, d2);
+  if (dot(d2, v2) < 0) return kDirNoIntersect;
+  d3 d3v = cross(v1, v2);
+  if (dot(d3v, v0) > 0) {
+    swapd3(v1, v2);
+    swapd3(d1, d2);
+    d3v = d3v * -1.0;
+  }
+  d3 v3 = mprSupport(md, d3v);
+  if (dot(d3v, v3) < 0) return kDirNoIntersect;
+  const double m2 = fmax(sqnorm(v1), fmax(sqnorm(v2), sqnorm(v3)));
+  const d3 v0s = v0 * sqrt(m2);
+  // the portal search of mprIntersect, from v0s
+  const double v0n = absnorm(v0s);
+  int it = 0;
+  while (true) {
+    if (it >= max_iter) return kDirFailed;
+    ++it;
+    d3 a1 = v1 - v0s, a2 = v2 - v0s, a3 = v3 - v0s;
+    d3 n031 = cross(a3, a1), n012 = cross(a1, a2);
+    if (dot(a2, n031) < 0) {
+      swapd3(v2, v3);
+      swapd3(d2, d3v);
+      swapd3(a2, a3);
+      swapd3(n012, n031);
+      n031 = -n031;
+      n012 = -n012;
+    }
+    if (dot(v0s, n031) > kDotEps * v0n * absnorm(n031)) {
+      d3 sd = -n031;
+      v2 = mprSupport(md, sd);
+      d2 = sd;
+      if (dot(v2, sd) < 0) return kDirNoIntersect;
+      continue;
+    }
+    if (dot(v0s, n012) > kDotEps * v0n * absnorm(n012)) {
+      d3 sd = -n012;
+      v3 = mprSupport(md, sd);
+      d3v = sd;
+      if (dot(v3, sd) < 0) return kDirNoIntersect;
+      continue;
+    }
+    const d3 n023 = cross(a2, a3);
+    if (dot(v0s, n023) > kDotEps * v0n * absnorm(n023)) {
+      d3 sd = -n023;
+      v1 = mprSupport(md, sd);
+      d1 = sd;
+      if (dot(v1, sd) < 0) return kDirNoIntersect;
+      continue;
+    }
+    break;
+  }
+  for (it = 1; it <= max_iter; ++it) {
+    o.iters = it;
+    d3 n = cross(v2 - v1, v3 - v1);
+    if (dot(n, d) < 0) {
+      swapd3(v2, v3);
+      swapd3(d2, d3v);
+      n = n * -1.0;
+    }
+    const d3 v4 = mprSupport(md, n);
+    if (dot(v4, n) < 0) return kDirNoIntersect;
+    if (fabs(dot(v4 - v1, n)) < tol * absnorm(n)) {
+      mprDirectedResult(md, d, n, v1, v2, v3, d1, d2, d3v, &v4, o);
+      return kDirOK;
+    }
+    const d3 nn = cross(v4, v0s);
+    if (dot(v1, nn) > 0) {
+      if (dot(v2, nn) > 0) { v1 = v4; d1 = n; }
+      else { v3 = v4; d3v = n; }
+    } else {
+      if (dot(v3, nn) > 0) { v2 = v4; d2 = n; }
+      else { v1 = v4; d1 = n; }
+    }
+  }
+  return kDirIterLimit;
+}
+
 }  // namespace mfcl

@@ -598,6 +598,61 @@ __global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO
   if (lane == 0 && v) atomicAdd(scanned_total, v);
 }
 
+// Directed penetration (fcl::collisionPenetration in its directed MPR mode,
+// collision_penetration-inl.h:196-244): after the boolean collide, one
+// wavefront per pair it reported colliding runs the MPR search along the
+// pair's escape direction for shape b (world unit vector) and replaces the
+// pair's contact: depth = the distance along it (-1 when the search fails),
+// pos = the witness midpoint (zero on failure), normal = the direction
+// (computePenetrationMPR, :105-194).  epa_status / epa_iters hold the MPR
+// status and its refinement iterations.
+__global__ void __launch_bounds__(kBlock) directedKernel(ShapeTables T, PairIO io, const float* dirs, uint64_t n,
+                                                         int max_it, double tol) {
+  __shared__ MinkCfg cfgs[kBlock / 64];
+  const uint64_t i = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;  // uniform across the wavefront
+  mfcl_convex_result& r = io.out[i];
+  if (r.n_contacts <= 0) return;
+  MFCL_LDS MinkCfg* cfg = (MFCL_LDS MinkCfg*)(&cfgs[threadIdx.x >> 6]);
+  DevShape A, B;
+  tf3 ta, tb;
+  uint32_t ia, ib;
+  loadPair(T, nullptr, io, static_cast<uint32_t>(i), A, B, ta, tb, ia, ib);
+  writeCfg(cfg, A, ta, B, tb, d3{0, 0, 0}, d3{0, 0, 0});
+  MinkL<64> md{cfg, lane, 0};
+  const d3 dw = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+  DirOut o;
+  o.distance = 0;
+  o.p0 = o.p1 = d3{0, 0, 0};
+  const int st = mprDirected(md, tmul(ta.R, dw), max_it, tol, o);
+  if (lane != 0) return;
+  r.epa_status = static_cast<int8_t>(st);
+  r.epa_iters = sat16(o.iters);
+  r.normal[0] = dw.x; r.normal[1] = dw.y; r.normal[2] = dw.z;
+  if (st == kDirOK) {
+    const d3 pw = apply(ta, 0.5 * (o.p0 + o.p1));
+    r.depth = o.distance;
+    r.pos[0] = pw.x; r.pos[1] = pw.y; r.pos[2] = pw.z;
+  } else {
+    r.depth = -1;
+    r.pos[0] = r.pos[1] = r.pos[2] = 0;
+  }
+  r.distance = -r.depth;
+}
+
+int directedLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
+                   const float* dirs, uint64_t n, int max_it, double tol, mfcl_convex_result* out, hipStream_t s) {
+  ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
+                ctx->d_seed, ctx->d_wblk};
+  PairIO io{pa, pb, posea, poseb, nullptr, out, nullptr, 0};
+  const uint64_t threads = 64 * n;
+  directedKernel<<<static_cast<unsigned>((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(T, io, dirs, n, max_it,
+                                                                                          tol);
+  MFCL_HIP_TRY(ctx, hipGetLastError());
+  return MFCL_OK;
+}
+
 __global__ void zeroKernel(uint32_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;

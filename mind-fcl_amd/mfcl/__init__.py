@@ -166,6 +166,8 @@ def load_library() -> C.CDLL:
     lib.mfcl_convex_hulls.argtypes = [p, p, p, p, u32, p, p, p, p, p]
     lib.mfcl_convex_hulls_dev.argtypes = [p, p, p, p, u32, i32, p, p, p, p, p, p]
     lib.mfcl_upload_convex_points.argtypes = [p, p, p, p, u32, p]
+    lib.mfcl_convex_directed_penetration.argtypes = [p, p, p, p, p, p, u64, p, p]
+    lib.mfcl_convex_directed_penetration_dev.argtypes = [p, p, p, p, p, p, u64, p, p, p]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
     lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
     lib.mfcl_mesh_get_nodes.argtypes = [p, u32, p]
@@ -427,6 +429,21 @@ class Context:
                 self.h, _ptr(pair_a), _ptr(pair_b), _ptr(pose_a), _ptr(pose_b), _ptr(g), n, C.byref(req), _ptr(out)
             )
         )
+        return out
+
+    def convex_directed_penetration(self, pair_a, pair_b, pose_a, pose_b, dirs, req: Request) -> np.ndarray:
+        """collisionPenetration in the directed MPR mode: per pair the
+        boolean collide and, when colliding, the penetration depth along
+        dirs[i] (world unit vector, shape b's escape direction)."""
+        pair_a = np.ascontiguousarray(pair_a, np.uint32)
+        pair_b = np.ascontiguousarray(pair_b, np.uint32)
+        pose_a = np.ascontiguousarray(pose_a, np.float32).reshape(-1, 7)
+        pose_b = np.ascontiguousarray(pose_b, np.float32).reshape(-1, 7)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = len(pair_a)
+        out = np.zeros(n, CONVEX_RESULT_DTYPE)
+        self._check(self.lib.mfcl_convex_directed_penetration(self.h, _ptr(pair_a), _ptr(pair_b), _ptr(pose_a),
+                                                              _ptr(pose_b), _ptr(dirs), n, C.byref(req), _ptr(out)))
         return out
 
     def convex_collide_contacts(self, pair_a, pair_b, pose_a, pose_b, req: Request):

@@ -192,6 +192,52 @@ double orc_collide_batch(void* hs, const int* type, const double* params, const 
                                     penetration, sp6, guess3, n_threads, out, nullptr);
 }
 
+// fcl::collisionPenetration with the directed mode on shape pairs
+// (include/fcl/narrowphase/collision_penetration-inl.h:196-244): the
+// boolean collide (penetration disabled), and for its contact the MPR
+// penetration along dir3 (world unit vector, the escape direction of shape
+// b) with MPR(128, tol) (computePenetrationMPR, :105-194): depth = distance
+// along it or -1 when the search fails, pos = tf_a * the witness midpoint
+// (zero on failure), normal = dir3.  epa_status holds the MPR status,
+// epa_iters its refinement iterations.
+double orc_directed_penetration_batch(void* hs, const int* type, const double* params, const int* hull_idx,
+                                      const int* pa, const int* pb, const double* tfa12, const double* tfb12,
+                                      const double* dir3, int64_t n, int max_contacts, const double* sp6,
+                                      double mpr_tol, int n_threads, OrcCollideRec* out) {
+  ShapeTable st{static_cast<HullSet*>(hs), type, params, hull_idx};
+  const SolverParams sp = solverFrom(sp6);
+  const auto t0 = std::chrono::steady_clock::now();
+  parallelFor(n, n_threads, [&](int64_t i) {
+    const Shape a = st.get(pa[i]), b = st.get(pb[i]);
+    const Tf ta = tfFrom12(tfa12 + 12 * i), tb = tfFrom12(tfb12 + 12 * i);
+    CollideOut c;
+    collide(a, ta, b, tb, max_contacts, false, sp, c);
+    OrcCollideRec& r = out[i];
+    std::memset(&r, 0, sizeof(r));
+    r.hit = c.hit;
+    r.n = c.n;
+    r.method = c.method;
+    r.gjk_status = c.gjk_status;
+    r.epa_status = -1;
+    if (c.n == 0) return;
+    const V3 dw(dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]);
+    const MinkDiff md = makeMinkDiff(a, ta, b, tb);
+    DirectedOut o;
+    const DirectedStatus ds = mprDirected(md, transpose(ta.R) * dw, 128, mpr_tol, o);
+    r.epa_status = static_cast<int>(ds);
+    r.epa_iters = o.iterations;
+    r.normal[0] = dw.x; r.normal[1] = dw.y; r.normal[2] = dw.z;
+    if (ds == DirectedStatus::OK) {
+      r.depth = o.distance;
+      const V3 pw = ta * (0.5 * (o.p0 + o.p1));
+      r.pos[0] = pw.x; r.pos[1] = pw.y; r.pos[2] = pw.z;
+    } else {
+      r.depth = -1;
+    }
+  });
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // GJKSolver::shapeSignedDistance over a batch.  Returns wall seconds.
 double orc_signed_distance_batch(void* hs, const int* type, const double* params,
                                  const int* hull_idx, const int* pa, const int* pb,

@@ -1,6 +1,10 @@
 // CPU oracle -- TEST INFRASTRUCTURE ONLY (see omath.h).
 // MPR boolean: include/fcl/cvx_collide/mpr.hpp:34-185 (RunIntersect),
-// :188-328 (findPortal), :436-495 (updatePortal).
+// :188-328 (findPortal), :436-495 (updatePortal); directed penetration
+// :496-632 (RunDirectedPenetration), :642-704 (its result).
+#include <algorithm>
+#include <cmath>
+#include <limits>
 #include <utility>
 
 #include "narrowphase.h"
@@ -114,6 +118,92 @@ MPRStatus mprIntersect(const MinkDiff& md, int max_iter, double tol) {
     updatePortal(v0, v4, n, v1, v2, v3, d1, d2, d3);
   }
   return MPRStatus::Failed;
+}
+
+namespace {
+
+// mpr.hpp:642-704: the penetration distance along d through the portal
+// (v1, v2, v3) with normal n, and the witness points as the barycentric
+// blend of the portal vertices' support points.
+void directedResult(const MinkDiff& md, const V3& d, const V3& n, const V3& v1, const V3& v2, const V3& v3,
+                    const V3& d1, const V3& d2, const V3& d3, const V3* v4, DirectedOut& o) {
+  o.portal_normal = n;
+  const double nd = dot(d, n);
+  if (std::fabs(nd) <= 0) {
+    const double e[3] = {dot(v1, d), dot(v2, d), dot(v3, d)};
+    const V3* dirs[3] = {&d1, &d2, &d3};
+    double best = -std::numeric_limits<double>::infinity();
+    int bi = 0;
+    for (int i = 0; i < 3; ++i)
+      if (e[i] > best) {
+        best = e[i];
+        bi = i;
+      }
+    o.distance = best;
+    o.p0 = md.support0(*dirs[bi]);
+    o.p1 = md.support1(-*dirs[bi]);
+    return;
+  }
+  const double to_portal = dot(v1, n) / nd;
+  o.distance = v4 ? dot(*v4, n) / nd : to_portal;
+  const V3 q = to_portal * d;
+  const V3 e12 = v2 - v1, e13 = v3 - v1;
+  const double area = norm(cross(e12, e13));
+  const double w2 = norm(cross(e13, v1 - q)) / area;
+  const double w3 = norm(cross(e12, v1 - q)) / area;
+  const double w1 = 1.0 - w2 - w3;
+  o.p0 = md.support0(d1) * w1 + md.support0(d2) * w2 + md.support0(d3) * w3;
+  o.p1 = md.support1(-d1) * w1 + md.support1(-d2) * w2 + md.support1(-d3) * w3;
+}
+
+}  // namespace
+
+DirectedStatus mprDirected(const MinkDiff& md, const V3& d, int max_iter, double tol, DirectedOut& o) {
+  // the ray starts at a mock point -d (not necessarily inside the difference)
+  const V3 v0 = -d;
+  V3 d1 = d;
+  V3 v1 = support(md, d1);
+  if (dot(d1, v1) < 0) return DirectedStatus::FailedNoIntersect;
+  V3 d2 = cross(v0, v1);
+  if (absnorm(d2) <= absnorm(v1) * tol) {  // v1 on the ray
+    o.distance = dot(v1, d);
+    o.portal_normal = d1;
+    o.p0 = md.support0(d1);
+    o.p1 = md.support1(-d1);
+    return DirectedStatus::OK;
+  }
+  V3 v2 = support(md, d2);
+  if (dot(d2, v2) < 0) return DirectedStatus::FailedNoIntersect;
+  V3 d3 = cross(v1, v2);
+  if (dot(d3, v0) > 0) {
+    std::swap(v1, v2);
+    std::swap(d1, d2);
+    d3 = d3 * -1.0;
+  }
+  V3 v3 = support(md, d3);
+  if (dot(d3, v3) < 0) return DirectedStatus::FailedNoIntersect;
+  const double m2 = std::max(sqnorm(v1), std::max(sqnorm(v2), sqnorm(v3)));
+  const V3 v0s = v0 * std::sqrt(m2);
+  const Portal ps = findPortal(md, v0s, v1, v2, v3, d1, d2, d3, max_iter);
+  if (ps == Portal::IterationLimit) return DirectedStatus::Failed;
+  if (ps == Portal::Separated) return DirectedStatus::FailedNoIntersect;
+  for (int it = 1; it <= max_iter; ++it) {
+    o.iterations = it;
+    V3 n = cross(v2 - v1, v3 - v1);
+    if (dot(n, d) < 0) {
+      std::swap(v2, v3);
+      std::swap(d2, d3);
+      n = n * -1.0;
+    }
+    const V3 v4 = support(md, n);  // n is normalised in place
+    if (dot(v4, n) < 0) return DirectedStatus::FailedNoIntersect;
+    if (std::fabs(dot(v4 - v1, n)) < tol * absnorm(n)) {
+      directedResult(md, d, n, v1, v2, v3, d1, d2, d3, &v4, o);
+      return DirectedStatus::OK;
+    }
+    updatePortal(v0s, v4, n, v1, v2, v3, d1, d2, d3);
+  }
+  return DirectedStatus::FailedRefinementIterationLimit;
 }
 
 }  // namespace oracle

@@ -104,6 +104,19 @@ EPAStatus epaEvaluate(const Simplex& simplex, const MinkDiff& md, double tol, in
 enum class MPRStatus { Intersect, Separated, Failed };
 MPRStatus mprIntersect(const MinkDiff& md, int max_iter, double tol);
 
+// MPR penetration along a given unit direction d (shape-0 frame)
+// (MPR::RunDirectedPenetration, include/fcl/cvx_collide/mpr.hpp:496-632,
+// finalizeDirectedPenetrationResult :642-704); status codes follow
+// MPR::DirectedPenetrationStatus (mpr.h:45-50).
+enum class DirectedStatus { OK = 0, Failed = 1, FailedNoIntersect = 2, FailedRefinementIterationLimit = 3 };
+struct DirectedOut {
+  double distance{0};
+  V3 portal_normal;
+  V3 p0, p1;  // shape-0 frame
+  int iterations{0};  // portal refinement iterations
+};
+DirectedStatus mprDirected(const MinkDiff& md, const V3& d, int max_iter, double tol, DirectedOut& out);
+
 // ---------------------------------------------------------------------------
 // GJKSolver-level queries (include/fcl/narrowphase/detail/gjk_solver-inl.h).
 struct SolverParams {
