@@ -299,6 +299,23 @@ int mfcl_convex_directed_penetration_dev(mfcl_ctx* ctx, const uint32_t* pair_a, 
                                          const float* pose_a7, const float* pose_b7, const float* dir3,
                                          uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
                                          void* stream);
+/* The same with CollisionPenetrationMode IncrementalMinimumPenetration:
+ * from the hint hint3[i] (world unit vector) the MPR search refines the
+ * direction of minimum penetration (MPR::
+ * RunIncrementalMinimumPenetrationDistance, include/fcl/cvx_collide/
+ * mpr_incremental_penetration.hpp:10-337).  depth = the minimum
+ * penetration found, normal = its direction in the world (shape b's escape
+ * direction), pos = the witness midpoint; an iteration-limit result counts
+ * as found; failures as above.  epa_status = MPR::
+ * IncrementalPenetrationStatus (0 OK, 1 Failed, 2 FailedNoIntersect, 4
+ * IterationLimit), epa_iters = its outer iterations. */
+int mfcl_convex_incremental_penetration(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                                        const float* pose_a7, const float* pose_b7, const float* hint3, uint64_t n,
+                                        const mfcl_request* req, mfcl_convex_result* out);
+int mfcl_convex_incremental_penetration_dev(mfcl_ctx* ctx, const uint32_t* pair_a, const uint32_t* pair_b,
+                                            const float* pose_a7, const float* pose_b7, const float* hint3,
+                                            uint64_t n, const mfcl_request* req, mfcl_convex_result* out,
+                                            void* stream);
 
 /* ---- convex hulls on the device ----------------------------------------- */
 /* The convex hull of each point set (what a caller of fcl::Convex computes

@@ -680,16 +680,16 @@ int mfcl_convex_collide_contacts(mfcl_ctx* ctx, const uint32_t* pa, const uint32
   return MFCL_OK;
 }
 
-int mfcl_convex_directed_penetration_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb,
-                                         const float* posea, const float* poseb, const float* dir3, uint64_t n,
-                                         const mfcl_request* req, mfcl_convex_result* out, void* stream) {
+static int mprPenDev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
+                     const float* dir3, uint64_t n, const mfcl_request* req, mfcl_convex_result* out, void* stream,
+                     bool incremental) {
   if (!ctx || !req) return MFCL_E_INVALID;
   if (n && !dir3) {
     ctx->err = "null device pointer";
     return MFCL_E_INVALID;
   }
   if (req->mode != MFCL_MODE_COLLIDE) {
-    ctx->err = "directed penetration needs a collide request";
+    ctx->err = "MPR penetration needs a collide request";
     return MFCL_E_INVALID;
   }
   mfcl_request boolean = *req;
@@ -701,12 +701,12 @@ int mfcl_convex_directed_penetration_dev(mfcl_ctx* ctx, const uint32_t* pa, cons
   if (n == 0) return MFCL_OK;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   // MPR(128, distanceTolerance) as collisionPenetrationMPR builds it (:208-210)
-  return directedLaunch(ctx, pa, pb, posea, poseb, dir3, n, 128, req->epa_tolerance, out, s);
+  return mprPenLaunch(ctx, pa, pb, posea, poseb, dir3, n, incremental, 128, req->epa_tolerance, out, s);
 }
 
-int mfcl_convex_directed_penetration(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
-                                     const float* poseb, const float* dir3, uint64_t n, const mfcl_request* req,
-                                     mfcl_convex_result* out) {
+static int mprPenHost(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                      const float* poseb, const float* dir3, uint64_t n, const mfcl_request* req,
+                      mfcl_convex_result* out, bool incremental) {
   if (!ctx || !req) return MFCL_E_INVALID;
   MFCL_NOT_WHILE_CAPTURING(ctx);
   if (n == 0) return MFCL_OK;
@@ -742,10 +742,34 @@ int mfcl_convex_directed_penetration(mfcl_ctx* ctx, const uint32_t* pa, const ui
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqa, posea, b_pose, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dqb, poseb, b_pose, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dd, dir3, b_dir, hipMemcpyHostToDevice, s));
-  if (int rc = mfcl_convex_directed_penetration_dev(ctx, dpa, dpb, dqa, dqb, dd, n, req, dout, s)) return rc;
+  if (int rc = mprPenDev(ctx, dpa, dpb, dqa, dqb, dd, n, req, dout, s, incremental)) return rc;
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(out, dout, b_out, hipMemcpyDeviceToHost, s));
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
   return MFCL_OK;
+}
+
+int mfcl_convex_directed_penetration_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb,
+                                         const float* posea, const float* poseb, const float* dir3, uint64_t n,
+                                         const mfcl_request* req, mfcl_convex_result* out, void* stream) {
+  return mprPenDev(ctx, pa, pb, posea, poseb, dir3, n, req, out, stream, false);
+}
+
+int mfcl_convex_directed_penetration(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                                     const float* poseb, const float* dir3, uint64_t n, const mfcl_request* req,
+                                     mfcl_convex_result* out) {
+  return mprPenHost(ctx, pa, pb, posea, poseb, dir3, n, req, out, false);
+}
+
+int mfcl_convex_incremental_penetration_dev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb,
+                                            const float* posea, const float* poseb, const float* hint3, uint64_t n,
+                                            const mfcl_request* req, mfcl_convex_result* out, void* stream) {
+  return mprPenDev(ctx, pa, pb, posea, poseb, hint3, n, req, out, stream, true);
+}
+
+int mfcl_convex_incremental_penetration(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
+                                        const float* poseb, const float* hint3, uint64_t n, const mfcl_request* req,
+                                        mfcl_convex_result* out) {
+  return mprPenHost(ctx, pa, pb, posea, poseb, hint3, n, req, out, true);
 }
 
 int mfcl_convex_hulls_dev(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off, const uint32_t* pt_cnt,

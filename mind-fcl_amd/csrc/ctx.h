@@ -194,9 +194,11 @@ int uploadVec(mfcl_ctx* ctx, void** dptr, const void* src, size_t bytes);
 int uploadConvexFaces(mfcl_ctx* ctx, const int32_t* faces, const uint64_t* face_off, const uint32_t* num_faces,
                       uint64_t faces_len);
 int computeWalkSeeds(mfcl_ctx* ctx);
-// narrowphase.hip: the directed-penetration pass over a collide's results
-int directedLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
-                   const float* dirs, uint64_t n, int max_it, double tol, mfcl_convex_result* out, hipStream_t s);
+// narrowphase.hip: the MPR penetration pass (directed or incremental) over
+// a collide's results
+int mprPenLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
+                 const float* dirs, uint64_t n, bool incremental, int max_it, double tol, mfcl_convex_result* out,
+                 hipStream_t s);
 // capi.cpp: frees the shape table (convex tables included)
 void freeShapes(mfcl_ctx* ctx);
 // hull.hip

@@ -1532,4 +1532,176 @@ MFCL_D int mprDirected(MK& md, d3 d, int max_iter, double tol, DirOut& o) {
   return kDirIterLimit;
 }
 
+// MPR minimum penetration refined from a unit hint direction (shape-0
+// frame): directed portal searches that switch to the current portal's
+// normal whenever the portal proves that direction shorter, until a search
+// converges (MPR::RunIncrementalMinimumPenetrationDistance with
+// return_on_subroutine_converge, mpr_incremental_penetration.hpp:10-337).
+// Status codes follow MPR::IncrementalPenetrationStatus.
+enum IncStatus : int { kIncOK = 0, kIncFailed = 1, kIncNoIntersect = 2, kIncIterLimit = 4 };
+enum IncExplore : int { kExNewDir = 0, kExConverged = 1, kExDegenerated = 2, kExFailed = 3, kExNoIntersect = 4 };
+struct IncOut {
+  double depth;
+  d3 dir, p0, p1;
+  int iters;
+};
+
+template <class MK>
+MFCL_D void mprIncResult(MK& md, d3 d, d3 v1, d3 v2, d3 v3, d3 d1, d3 d2, d3 d3v, double lb, double ub, IncOut& o) {
+  o.depth = ub;
+  o.dir = d;
+  const d3 q = lb * d;
+  const d3 e12 = v2 - v1, e13 = v3 - v1;
+  const double area = norm(cross(e12, e13));
+  const double w2 = norm(cross(e13, v1 - q)) / area;
+  const double w3 = norm(cross(e12, v1 - q)) / area;
+  const double w1 = 1.0 - w2 - w3;
+  o.p0 = md.support0(d1) * w1 + md.support0(d2) * w2 + md.support0(d3v) * w3;
+  o.p1 = md.support1(-d1) * w1 + md.support1(-d2) * w2 + md.support1(-d3v) * w3;
+}
+
+template <class MK>
+MFCL_D int mprExplore(MK& md, d3 d, d3& v1, d3& v2, d3& v3, d3& d1, d3& d2, d3& d3v, double& lb, double& ub,
+                      d3& new_d, bool portal_valid, int max_iter, double tol) {
+  constexpr double kDotEps = 2.220446049250313e-16;
+  const d3 v0 = -d;
+  d3 sd0 = d;
+  const d3 s0 = mprSupport(md, sd0);
+  if (dot(d, s0) < 0) return kExNoIntersect;
+  if (!portal_valid) {
+    d1 = d;
+    v1 = s0;
+    d2 = cross(v0, v1);
+    if (absnorm(d2) <= absnorm(v1) * tol) {
+      lb = dot(v1, d);
+      ub = lb;
+      new_d = d;
+      return kExDegenerated;
+    }
+    v2 = mprSupport(md, d2);
+    if (dot(d2, v2) < 0) return kExNoIntersect;
+    d3v = cross(v1, v2);
+    if (dot(d3v, v0) > 0) {
+      swapd3(v1, v2);
+      swapd3(d1, d2);
+      d3v = d3v * -1.0;
+    }
+    v3 = mprSupport(md, d3v);
+    if (dot(d3v, v3) < 0) return kExNoIntersect;
+  }
+  const double m2 = fmax(sqnorm(v1), fmax(sqnorm(v2), sqnorm(v3)));
+  const d3 v0s = v0 * sqrt(m2);
+  const double v0n = absnorm(v0s);
+  int it = 0;
+  while (true) {
+    if (it >= max_iter) return kExFailed;
+    ++it;
+    d3 a1 = v1 - v0s, a2 = v2 - v0s, a3 = v3 - v0s;
+    d3 n031 = cross(a3, a1), n012 = cross(a1, a2);
+    if (dot(a2, n031) < 0) {
+      swapd3(v2, v3);
+      swapd3(d2, d3v);
+      swapd3(a2, a3);
+      swapd3(n012, n031);
+      n031 = -n031;
+      n012 = -n012;
+    }
+    if (dot(v0s, n031) > kDotEps * v0n * absnorm(n031)) {
+      d3 sd = -n031;
+      v2 = mprSupport(md, sd);
+      d2 = sd;
+      if (dot(v2, sd) < 0) return kExNoIntersect;
+      continue;
+    }
+    if (dot(v0s, n012) > kDotEps * v0n * absnorm(n012)) {
+      d3 sd = -n012;
+      v3 = mprSupport(md, sd);
+      d3v = sd;
+      if (dot(v3, sd) < 0) return kExNoIntersect;
+      continue;
+    }
+    const d3 n023 = cross(a2, a3);
+    if (dot(v0s, n023) > kDotEps * v0n * absnorm(n023)) {
+      d3 sd = -n023;
+      v1 = mprSupport(md, sd);
+      d1 = sd;
+      if (dot(v1, sd) < 0) return kExNoIntersect;
+      continue;
+    }
+    break;
+  }
+  for (it = 0; it < max_iter; ++it) {
+    d3 n = cross(v2 - v1, v3 - v1);
+    if (dot(n, d) < 0) {
+      swapd3(v2, v3);
+      swapd3(d2, d3v);
+      n = n * -1.0;
+    }
+    const d3 v4 = mprSupport(md, n);
+    const double ub_plane = dot(v4, n);
+    if (ub_plane < 0) return kExNoIntersect;
+    const double v1n = dot(v1, n), v4n = dot(v4, n);
+    const double gap = v4n - v1n;
+    const double nd = dot(d, n);
+    if (fabs(nd) <= 0) {
+      lb = fmax(dot(v1, d), fmax(dot(v2, d), dot(v3, d)));
+      ub = lb;
+    } else {
+      lb = v1n / nd;
+      ub = v4n / nd;
+    }
+    if (lb > ub_plane + tol) {
+      new_d = n;
+      return kExNewDir;
+    }
+    if (fabs(gap) <= tol) {
+      new_d = n;
+      return kExConverged;
+    }
+    const d3 nn = cross(v4, v0s);
+    if (dot(v1, nn) > 0) {
+      if (dot(v2, nn) > 0) { v1 = v4; d1 = n; }
+      else { v3 = v4; d3v = n; }
+    } else {
+      if (dot(v3, nn) > 0) { v2 = v4; d2 = n; }
+      else { v1 = v4; d1 = n; }
+    }
+  }
+  return kExFailed;
+}
+
+template <class MK>
+MFCL_D int mprIncremental(MK& md, d3 init, int max_iter, double tol, IncOut& o) {
+  d3 d = init;
+  d3 v1 = {0, 0, 0}, v2 = v1, v3 = v1, d1 = v1, d2 = v1, d3v = v1;
+  double prev_lb = -1, prev_ub = -1;
+  o.iters = 0;
+  for (int outer = 0; outer < max_iter; ++outer) {
+    o.iters = outer + 1;
+    d3 new_d = {0, 0, 0};
+    double lb = 0, ub = 0;
+    const int e = mprExplore(md, d, v1, v2, v3, d1, d2, d3v, lb, ub, new_d, outer >= 1, max_iter, tol);
+    if (e == kExNewDir) {
+      d = new_d;
+    } else if (e == kExConverged) {
+      mprIncResult(md, d, v1, v2, v3, d1, d2, d3v, lb, ub, o);
+      return kIncOK;
+    } else if (e == kExDegenerated) {
+      o.depth = ub;
+      o.dir = new_d;
+      o.p0 = md.support0(new_d);
+      o.p1 = md.support1(-new_d);
+      return kIncOK;
+    } else if (e == kExNoIntersect) {
+      return kIncNoIntersect;
+    } else {
+      return kIncFailed;
+    }
+    prev_lb = lb;
+    prev_ub = ub;
+  }
+  mprIncResult(md, d, v1, v2, v3, d1, d2, d3v, prev_lb, prev_ub, o);
+  return kIncIterLimit;
+}
+
 }  // namespace mfcl

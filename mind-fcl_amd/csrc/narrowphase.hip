@@ -598,16 +598,19 @@ __global__ void __launch_bounds__(kBlock, 2) epaWaveKernel(ShapeTables T, PairIO
   if (lane == 0 && v) atomicAdd(scanned_total, v);
 }
 
-// Directed penetration (fcl::collisionPenetration in its directed MPR mode,
-// collision_penetration-inl.h:196-244): after the boolean collide, one
-// wavefront per pair it reported colliding runs the MPR search along the
-// pair's escape direction for shape b (world unit vector) and replaces the
-// pair's contact: depth = the distance along it (-1 when the search fails),
-// pos = the witness midpoint (zero on failure), normal = the direction
-// (computePenetrationMPR, :105-194).  epa_status / epa_iters hold the MPR
-// status and its refinement iterations.
-__global__ void __launch_bounds__(kBlock) directedKernel(ShapeTables T, PairIO io, const float* dirs, uint64_t n,
-                                                         int max_it, double tol) {
+// MPR penetration (fcl::collisionPenetration in its DirectedPenetration or
+// IncrementalMinimumPenetration mode, collision_penetration-inl.h:196-244):
+// after the boolean collide, one wavefront per pair it reported colliding
+// runs the MPR search from the pair's direction (world unit vector: the
+// escape direction of shape b, or the hint) and replaces the pair's
+// contact (computePenetrationMPR, :105-194): depth = the distance along the
+// direction / the refined minimum penetration (-1 when the search fails),
+// pos = the witness midpoint (zero on failure), normal = the direction /
+// the refined direction in the world.  epa_status / epa_iters hold the MPR
+// status and its iterations.
+template <bool INC>
+__global__ void __launch_bounds__(kBlock) mprPenKernel(ShapeTables T, PairIO io, const float* dirs, uint64_t n,
+                                                       int max_it, double tol) {
   __shared__ MinkCfg cfgs[kBlock / 64];
   const uint64_t i = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -622,17 +625,44 @@ __global__ void __launch_bounds__(kBlock) directedKernel(ShapeTables T, PairIO i
   writeCfg(cfg, A, ta, B, tb, d3{0, 0, 0}, d3{0, 0, 0});
   MinkL<64> md{cfg, lane, 0};
   const d3 dw = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
-  DirOut o;
-  o.distance = 0;
-  o.p0 = o.p1 = d3{0, 0, 0};
-  const int st = mprDirected(md, tmul(ta.R, dw), max_it, tol, o);
+  const d3 dl = tmul(ta.R, dw);
+  int st, iters;
+  bool ok;
+  double depth = 0;
+  d3 p0 = {0, 0, 0}, p1 = p0, nw = dw;
+  if constexpr (INC) {
+    IncOut o;
+    o.depth = 0;
+    o.dir = o.p0 = o.p1 = d3{0, 0, 0};
+    st = mprIncremental(md, dl, max_it, tol, o);
+    iters = o.iters;
+    ok = st == kIncOK || st == kIncIterLimit;
+    if (ok) {
+      depth = o.depth;
+      p0 = o.p0;
+      p1 = o.p1;
+      nw = mul(ta.R, o.dir);
+    }
+  } else {
+    DirOut o;
+    o.distance = 0;
+    o.p0 = o.p1 = d3{0, 0, 0};
+    st = mprDirected(md, dl, max_it, tol, o);
+    iters = o.iters;
+    ok = st == kDirOK;
+    if (ok) {
+      depth = o.distance;
+      p0 = o.p0;
+      p1 = o.p1;
+    }
+  }
   if (lane != 0) return;
   r.epa_status = static_cast<int8_t>(st);
-  r.epa_iters = sat16(o.iters);
-  r.normal[0] = dw.x; r.normal[1] = dw.y; r.normal[2] = dw.z;
-  if (st == kDirOK) {
-    const d3 pw = apply(ta, 0.5 * (o.p0 + o.p1));
-    r.depth = o.distance;
+  r.epa_iters = sat16(iters);
+  r.normal[0] = nw.x; r.normal[1] = nw.y; r.normal[2] = nw.z;
+  if (ok) {
+    const d3 pw = apply(ta, 0.5 * (p0 + p1));
+    r.depth = depth;
     r.pos[0] = pw.x; r.pos[1] = pw.y; r.pos[2] = pw.z;
   } else {
     r.depth = -1;
@@ -641,14 +671,15 @@ __global__ void __launch_bounds__(kBlock) directedKernel(ShapeTables T, PairIO i
   r.distance = -r.depth;
 }
 
-int directedLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
-                   const float* dirs, uint64_t n, int max_it, double tol, mfcl_convex_result* out, hipStream_t s) {
+int mprPenLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
+                 const float* dirs, uint64_t n, bool incremental, int max_it, double tol, mfcl_convex_result* out,
+                 hipStream_t s) {
   ShapeTables T{ctx->d_type, ctx->d_param, ctx->d_voff, ctx->d_vcnt, ctx->d_verts, ctx->d_nbr, ctx->d_nbr_off,
                 ctx->d_seed, ctx->d_wblk};
   PairIO io{pa, pb, posea, poseb, nullptr, out, nullptr, 0};
-  const uint64_t threads = 64 * n;
-  directedKernel<<<static_cast<unsigned>((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(T, io, dirs, n, max_it,
-                                                                                          tol);
+  const unsigned blocks = static_cast<unsigned>((64 * n + kBlock - 1) / kBlock);
+  if (incremental) mprPenKernel<true><<<blocks, kBlock, 0, s>>>(T, io, dirs, n, max_it, tol);
+  else mprPenKernel<false><<<blocks, kBlock, 0, s>>>(T, io, dirs, n, max_it, tol);
   MFCL_HIP_TRY(ctx, hipGetLastError());
   return MFCL_OK;
 }

@@ -168,6 +168,8 @@ def load_library() -> C.CDLL:
     lib.mfcl_upload_convex_points.argtypes = [p, p, p, p, u32, p]
     lib.mfcl_convex_directed_penetration.argtypes = [p, p, p, p, p, p, u64, p, p]
     lib.mfcl_convex_directed_penetration_dev.argtypes = [p, p, p, p, p, p, u64, p, p, p]
+    lib.mfcl_convex_incremental_penetration.argtypes = [p, p, p, p, p, p, u64, p, p]
+    lib.mfcl_convex_incremental_penetration_dev.argtypes = [p, p, p, p, p, p, u64, p, p, p]
     lib.mfcl_upload_meshes.argtypes = [p, p, p, p, p, p, p, u32]
     lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
     lib.mfcl_mesh_get_nodes.argtypes = [p, u32, p]
@@ -431,10 +433,17 @@ class Context:
         )
         return out
 
-    def convex_directed_penetration(self, pair_a, pair_b, pose_a, pose_b, dirs, req: Request) -> np.ndarray:
+    def convex_incremental_penetration(self, pair_a, pair_b, pose_a, pose_b, hints, req: Request) -> np.ndarray:
+        """collisionPenetration in the incremental minimum-penetration MPR
+        mode, refined from hints[i] (world unit vectors)."""
+        return self.convex_directed_penetration(pair_a, pair_b, pose_a, pose_b, hints, req, incremental=True)
+
+    def convex_directed_penetration(self, pair_a, pair_b, pose_a, pose_b, dirs, req: Request,
+                                    incremental: bool = False) -> np.ndarray:
         """collisionPenetration in the directed MPR mode: per pair the
         boolean collide and, when colliding, the penetration depth along
         dirs[i] (world unit vector, shape b's escape direction)."""
+        fn = self.lib.mfcl_convex_incremental_penetration if incremental else self.lib.mfcl_convex_directed_penetration
         pair_a = np.ascontiguousarray(pair_a, np.uint32)
         pair_b = np.ascontiguousarray(pair_b, np.uint32)
         pose_a = np.ascontiguousarray(pose_a, np.float32).reshape(-1, 7)
@@ -442,8 +451,8 @@ class Context:
         dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
         n = len(pair_a)
         out = np.zeros(n, CONVEX_RESULT_DTYPE)
-        self._check(self.lib.mfcl_convex_directed_penetration(self.h, _ptr(pair_a), _ptr(pair_b), _ptr(pose_a),
-                                                              _ptr(pose_b), _ptr(dirs), n, C.byref(req), _ptr(out)))
+        self._check(fn(self.h, _ptr(pair_a), _ptr(pair_b), _ptr(pose_a), _ptr(pose_b), _ptr(dirs), n, C.byref(req),
+                       _ptr(out)))
         return out
 
     def convex_collide_contacts(self, pair_a, pair_b, pose_a, pose_b, req: Request):

@@ -192,18 +192,20 @@ double orc_collide_batch(void* hs, const int* type, const double* params, const 
                                     penetration, sp6, guess3, n_threads, out, nullptr);
 }
 
-// fcl::collisionPenetration with the directed mode on shape pairs
+// fcl::collisionPenetration with the MPR modes on shape pairs
 // (include/fcl/narrowphase/collision_penetration-inl.h:196-244): the
 // boolean collide (penetration disabled), and for its contact the MPR
-// penetration along dir3 (world unit vector, the escape direction of shape
-// b) with MPR(128, tol) (computePenetrationMPR, :105-194): depth = distance
-// along it or -1 when the search fails, pos = tf_a * the witness midpoint
-// (zero on failure), normal = dir3.  epa_status holds the MPR status,
-// epa_iters its refinement iterations.
-double orc_directed_penetration_batch(void* hs, const int* type, const double* params, const int* hull_idx,
-                                      const int* pa, const int* pb, const double* tfa12, const double* tfb12,
-                                      const double* dir3, int64_t n, int max_contacts, const double* sp6,
-                                      double mpr_tol, int n_threads, OrcCollideRec* out) {
+// search from dir3 (world unit vector, the escape direction of shape b)
+// with MPR(128, tol) (computePenetrationMPR, :105-194).  Directed
+// (incremental == 0): depth = distance along dir3, normal = dir3.
+// Incremental: depth = the refined minimum penetration, normal = its
+// direction in the world (an iteration-limit result counts as valid).
+// Failures: depth -1, pos 0, normal = dir3.  pos = tf_a * the witness
+// midpoint.  epa_status holds the MPR status, epa_iters its iterations.
+double orc_mpr_penetration_batch(void* hs, const int* type, const double* params, const int* hull_idx,
+                                 const int* pa, const int* pb, const double* tfa12, const double* tfb12,
+                                 const double* dir3, int64_t n, int max_contacts, const double* sp6, double mpr_tol,
+                                 int incremental, int n_threads, OrcCollideRec* out) {
   ShapeTable st{static_cast<HullSet*>(hs), type, params, hull_idx};
   const SolverParams sp = solverFrom(sp6);
   const auto t0 = std::chrono::steady_clock::now();
@@ -222,14 +224,38 @@ double orc_directed_penetration_batch(void* hs, const int* type, const double* p
     if (c.n == 0) return;
     const V3 dw(dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]);
     const MinkDiff md = makeMinkDiff(a, ta, b, tb);
-    DirectedOut o;
-    const DirectedStatus ds = mprDirected(md, transpose(ta.R) * dw, 128, mpr_tol, o);
-    r.epa_status = static_cast<int>(ds);
-    r.epa_iters = o.iterations;
-    r.normal[0] = dw.x; r.normal[1] = dw.y; r.normal[2] = dw.z;
-    if (ds == DirectedStatus::OK) {
-      r.depth = o.distance;
-      const V3 pw = ta * (0.5 * (o.p0 + o.p1));
+    const V3 dl = transpose(ta.R) * dw;
+    bool ok;
+    V3 p0, p1, nw = dw;
+    double depth = 0;
+    if (incremental) {
+      IncrementalOut o;
+      const IncrementalStatus s = mprIncremental(md, dl, 128, mpr_tol, o);
+      r.epa_status = static_cast<int>(s);
+      r.epa_iters = o.iterations;
+      ok = s == IncrementalStatus::OK || s == IncrementalStatus::IterationLimit;
+      if (ok) {
+        depth = o.depth;
+        p0 = o.p0;
+        p1 = o.p1;
+        nw = ta.R * o.direction;
+      }
+    } else {
+      DirectedOut o;
+      const DirectedStatus s = mprDirected(md, dl, 128, mpr_tol, o);
+      r.epa_status = static_cast<int>(s);
+      r.epa_iters = o.iterations;
+      ok = s == DirectedStatus::OK;
+      if (ok) {
+        depth = o.distance;
+        p0 = o.p0;
+        p1 = o.p1;
+      }
+    }
+    r.normal[0] = nw.x; r.normal[1] = nw.y; r.normal[2] = nw.z;
+    if (ok) {
+      r.depth = depth;
+      const V3 pw = ta * (0.5 * (p0 + p1));
       r.pos[0] = pw.x; r.pos[1] = pw.y; r.pos[2] = pw.z;
     } else {
       r.depth = -1;

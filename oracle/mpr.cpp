@@ -1,7 +1,8 @@
 // CPU oracle -- TEST INFRASTRUCTURE ONLY (see omath.h).
 // MPR boolean: include/fcl/cvx_collide/mpr.hpp:34-185 (RunIntersect),
 // :188-328 (findPortal), :436-495 (updatePortal); directed penetration
-// :496-632 (RunDirectedPenetration), :642-704 (its result).
+// :496-632 (RunDirectedPenetration), :642-704 (its result); incremental
+// minimum penetration: mpr_incremental_penetration.hpp:10-337.
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -204,6 +205,138 @@ DirectedStatus mprDirected(const MinkDiff& md, const V3& d, int max_iter, double
     updatePortal(v0s, v4, n, v1, v2, v3, d1, d2, d3);
   }
   return DirectedStatus::FailedRefinementIterationLimit;
+}
+
+namespace {
+
+enum class Explore { NewDirection, Converged, Degenerated, Failed, FailedNoIntersect };
+
+// mpr_incremental_penetration.hpp:10-34: the distance along d to the
+// portal plane (lb) and to the parallel plane through v4 (ub).
+void exploredBounds(const V3& d, const V3& v1, const V3& v2, const V3& v3, const V3& n, double v1n, double v4n,
+                    double& lb, double& ub) {
+  const double nd = dot(d, n);
+  if (std::fabs(nd) <= 0) {
+    lb = std::max(dot(v1, d), std::max(dot(v2, d), dot(v3, d)));
+    ub = lb;
+    return;
+  }
+  lb = v1n / nd;
+  ub = v4n / nd;
+}
+
+// mpr_incremental_penetration.hpp:36-186: a directed portal search along d
+// that stops early when the portal's normal is a better direction.
+Explore exploreDirection(const MinkDiff& md, const V3& d, V3& v1, V3& v2, V3& v3, V3& d1, V3& d2, V3& d3,
+                         double& lb, double& ub, V3& new_d, bool portal_valid, int max_iter, double tol) {
+  const V3 v0 = -d;
+  V3 sd = d;
+  const V3 s0 = support(md, sd);
+  if (dot(d, s0) < 0) return Explore::FailedNoIntersect;
+  if (!portal_valid) {
+    d1 = d;
+    v1 = s0;
+    d2 = cross(v0, v1);
+    if (absnorm(d2) <= absnorm(v1) * tol) {
+      lb = dot(v1, d);
+      ub = lb;
+      new_d = d;
+      return Explore::Degenerated;
+    }
+    v2 = support(md, d2);
+    if (dot(d2, v2) < 0) return Explore::FailedNoIntersect;
+    d3 = cross(v1, v2);
+    if (dot(d3, v0) > 0) {
+      std::swap(v1, v2);
+      std::swap(d1, d2);
+      d3 = d3 * -1.0;
+    }
+    v3 = support(md, d3);
+    if (dot(d3, v3) < 0) return Explore::FailedNoIntersect;
+  }
+  const double m2 = std::max(sqnorm(v1), std::max(sqnorm(v2), sqnorm(v3)));
+  const V3 v0s = v0 * std::sqrt(m2);
+  const Portal ps = findPortal(md, v0s, v1, v2, v3, d1, d2, d3, max_iter);
+  if (ps == Portal::IterationLimit) return Explore::Failed;
+  if (ps == Portal::Separated) return Explore::FailedNoIntersect;
+  for (int it = 0; it < max_iter; ++it) {
+    V3 n = cross(v2 - v1, v3 - v1);
+    if (dot(n, d) < 0) {
+      std::swap(v2, v3);
+      std::swap(d2, d3);
+      n = n * -1.0;
+    }
+    const V3 v4 = support(md, n);  // n is normalised in place
+    const double ub_plane = dot(v4, n);
+    if (ub_plane < 0) return Explore::FailedNoIntersect;
+    const double v1n = dot(v1, n), v4n = dot(v4, n);
+    const double gap = v4n - v1n;
+    exploredBounds(d, v1, v2, v3, n, v1n, v4n, lb, ub);
+    if (lb > ub_plane + tol) {
+      new_d = n;
+      return Explore::NewDirection;
+    }
+    if (std::fabs(gap) <= tol) {
+      new_d = n;
+      return Explore::Converged;
+    }
+    updatePortal(v0s, v4, n, v1, v2, v3, d1, d2, d3);
+  }
+  return Explore::Failed;
+}
+
+// mpr_incremental_penetration.hpp:298-334
+void incrementalResult(const MinkDiff& md, const V3& d, const V3& v1, const V3& v2, const V3& v3, const V3& d1,
+                       const V3& d2, const V3& d3, double lb, double ub, IncrementalOut& o) {
+  o.depth = ub;
+  o.direction = d;
+  const V3 q = lb * d;
+  const V3 e12 = v2 - v1, e13 = v3 - v1;
+  const double area = norm(cross(e12, e13));
+  const double w2 = norm(cross(e13, v1 - q)) / area;
+  const double w3 = norm(cross(e12, v1 - q)) / area;
+  const double w1 = 1.0 - w2 - w3;
+  o.p0 = md.support0(d1) * w1 + md.support0(d2) * w2 + md.support0(d3) * w3;
+  o.p1 = md.support1(-d1) * w1 + md.support1(-d2) * w2 + md.support1(-d3) * w3;
+}
+
+}  // namespace
+
+IncrementalStatus mprIncremental(const MinkDiff& md, const V3& init, int max_iter, double tol, IncrementalOut& o) {
+  V3 d = init;
+  V3 v1, v2, v3, d1, d2, d3;
+  V3 prev_dir = init;
+  double prev_lb = -1, prev_ub = -1;
+  for (int outer = 0; outer < max_iter; ++outer) {
+    o.iterations = outer + 1;
+    V3 new_d;
+    double lb = 0, ub = 0;
+    const Explore e = exploreDirection(md, d, v1, v2, v3, d1, d2, d3, lb, ub, new_d, outer >= 1, max_iter, tol);
+    switch (e) {
+      case Explore::NewDirection:
+        d = new_d;
+        break;
+      case Explore::Converged:
+        incrementalResult(md, d, v1, v2, v3, d1, d2, d3, lb, ub, o);
+        return IncrementalStatus::OK;
+      case Explore::Degenerated:
+        o.depth = ub;
+        o.direction = new_d;
+        o.p0 = md.support0(new_d);
+        o.p1 = md.support1(-new_d);
+        return IncrementalStatus::OK;
+      case Explore::FailedNoIntersect:
+        return IncrementalStatus::FailedNoIntersect;
+      default:
+        return IncrementalStatus::Failed;
+    }
+    prev_dir = new_d;
+    prev_lb = lb;
+    prev_ub = ub;
+  }
+  (void)prev_dir;
+  incrementalResult(md, d, v1, v2, v3, d1, d2, d3, prev_lb, prev_ub, o);
+  return IncrementalStatus::IterationLimit;
 }
 
 }  // namespace oracle

@@ -117,6 +117,21 @@ struct DirectedOut {
 };
 DirectedStatus mprDirected(const MinkDiff& md, const V3& d, int max_iter, double tol, DirectedOut& out);
 
+// MPR minimum penetration refined from a unit hint direction d (shape-0
+// frame) (MPR::RunIncrementalMinimumPenetrationDistance with
+// return_on_subroutine_converge, include/fcl/cvx_collide/
+// mpr_incremental_penetration.hpp:10-337); status codes follow
+// MPR::IncrementalPenetrationStatus (mpr.h:76-82).
+enum class IncrementalStatus { OK = 0, Failed = 1, FailedNoIntersect = 2, DirectedPenetrationFailed = 3,
+                               IterationLimit = 4 };
+struct IncrementalOut {
+  double depth{0};
+  V3 direction;  // penetration direction (shape-0 frame)
+  V3 p0, p1;     // shape-0 frame
+  int iterations{0};  // outer iterations
+};
+IncrementalStatus mprIncremental(const MinkDiff& md, const V3& d, int max_iter, double tol, IncrementalOut& out);
+
 // ---------------------------------------------------------------------------
 // GJKSolver-level queries (include/fcl/narrowphase/detail/gjk_solver-inl.h).
 struct SolverParams {

@@ -55,8 +55,8 @@ def lib() -> C.CDLL:
         L.orc_collide_batch.restype = d
         L.orc_collide_contacts_batch.argtypes = [p, p, p, p, p, p, p, p, i64, i32, i32, p, p, i32, p, p]
         L.orc_collide_contacts_batch.restype = d
-        L.orc_directed_penetration_batch.argtypes = [p, p, p, p, p, p, p, p, p, i64, i32, p, d, i32, p]
-        L.orc_directed_penetration_batch.restype = d
+        L.orc_mpr_penetration_batch.argtypes = [p, p, p, p, p, p, p, p, p, i64, i32, p, d, i32, i32, p]
+        L.orc_mpr_penetration_batch.restype = d
         L.orc_signed_distance_batch.argtypes = [p, p, p, p, p, p, p, p, i64, p, p, i32, p]
         L.orc_signed_distance_batch.restype = d
         L.orc_pair_algorithm.argtypes = [p, p, p, p, i32, i32, p, p, i32, i32, d, p, p]
@@ -185,10 +185,12 @@ class ShapeSet:
                                          1 if penetration else 0, _p(sp), None, threads, _p(out), _p(cont))
         return out, cont
 
-    def directed_penetration(self, pa, pb, tfa12, tfb12, dirs, max_contacts=1, tol=1e-6, solver=None, threads=1):
-        """collisionPenetration with the directed MPR mode: COLLIDE_REC per
-        pair (epa_status = MPR directed status, epa_iters its iterations;
-        depth -1 when it fails)."""
+    def directed_penetration(self, pa, pb, tfa12, tfb12, dirs, max_contacts=1, tol=1e-6, solver=None, threads=1,
+                             incremental=False):
+        """collisionPenetration with an MPR mode (directed, or incremental
+        minimum penetration from the hint dirs): COLLIDE_REC per pair
+        (epa_status = the MPR status, epa_iters its iterations; depth -1
+        when it fails)."""
         pa = np.ascontiguousarray(pa, np.int32)
         pb = np.ascontiguousarray(pb, np.int32)
         tfa12 = np.ascontiguousarray(tfa12, np.float64).reshape(-1, 12)
@@ -196,10 +198,14 @@ class ShapeSet:
         dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
         sp = None if solver is None else np.ascontiguousarray(solver, np.float64)
         out = np.zeros(len(pa), COLLIDE_REC)
-        secs = lib().orc_directed_penetration_batch(self.hs, _p(self.types), _p(self.params), _p(self.hull_idx),
-                                                    _p(pa), _p(pb), _p(tfa12), _p(tfb12), _p(dirs), len(pa),
-                                                    max_contacts, _p(sp), tol, threads, _p(out))
+        secs = lib().orc_mpr_penetration_batch(self.hs, _p(self.types), _p(self.params), _p(self.hull_idx), _p(pa),
+                                               _p(pb), _p(tfa12), _p(tfb12), _p(dirs), len(pa), max_contacts, _p(sp),
+                                               tol, 1 if incremental else 0, threads, _p(out))
         return out, secs
+
+    def incremental_penetration(self, pa, pb, tfa12, tfb12, hints, max_contacts=1, tol=1e-6, solver=None,
+                                threads=1):
+        return self.directed_penetration(pa, pb, tfa12, tfb12, hints, max_contacts, tol, solver, threads, True)
 
     def signed_distance(self, pa, pb, tfa12, tfb12, solver=None, guess=None, threads=1):
         pa = np.ascontiguousarray(pa, np.int32)

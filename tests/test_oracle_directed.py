@@ -139,3 +139,104 @@ def test_separated_pair(oracle):
     out, _ = S.directed_penetration([0], [1], oracle.pose12(_pose((0, 0, 0))[None]),
                                     oracle.pose12(_pose((1, 0, 0))[None]), np.array([[1.0, 0, 0]]))
     assert out["hit"][0] == 0 and out["n"][0] == 0 and out["epa_status"][0] == -1
+
+
+# ---- incremental minimum penetration (mpr_incremental_penetration.hpp) ----
+
+def test_incremental_spheres(oracle):
+    """From any hint with a positive component along the centre offset, the
+    refinement reaches the minimum penetration r_a + r_b - |c| along c."""
+    rng = np.random.default_rng(63)
+    n = 300
+    r = rng.uniform(0.1, 0.5, (n, 2))
+    types = np.array([SPHERE, SPHERE] * n, np.int32)
+    params = np.zeros((2 * n, 3))
+    params[:, 0] = r.reshape(-1)
+    S = oracle.ShapeSet(types, params)
+    ca = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    off = _unit(rng, n) * (rng.uniform(0.2, 0.9, (n, 1)) * r.sum(1, keepdims=True)).astype(np.float32)
+    cb = (ca + off).astype(np.float32)
+    c = cb.astype(np.float64) - ca.astype(np.float64)
+    cu = c / np.linalg.norm(c, axis=1, keepdims=True)
+    hint = cu + rng.normal(scale=0.5, size=(n, 3))
+    hint /= np.linalg.norm(hint, axis=1, keepdims=True)
+    out, _ = S.incremental_penetration(np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2),
+                                       oracle.pose12(np.stack([_pose(x) for x in ca])),
+                                       oracle.pose12(np.stack([_pose(x) for x in cb])), hint, threads=8)
+    assert np.all(out["hit"] == 1) and np.all(np.isin(out["epa_status"], [0, 4]))
+    t = r.sum(1) - np.linalg.norm(c, axis=1)
+    err = np.abs(out["depth"] - t)
+    ang = np.arccos(np.clip(np.sum(out["normal"] * cu, 1), -1, 1))
+    print(f"\n[incremental spheres] max |depth - (r_a + r_b - |c|)| {err.max():.2e}, max angle {ang.max():.2e}")
+    assert err.max() < 1e-4
+    assert np.all(out["depth"] >= t - 1e-9)
+
+
+def test_incremental_boxes(oracle):
+    """Axis-aligned boxes: every result is at least the minimum penetration
+    min_k (h_k - |c_k|) (no escape is shorter), and a hint near the best
+    axis reaches it."""
+    rng = np.random.default_rng(64)
+    n = 400
+    ha = rng.uniform(0.1, 0.5, (n, 3)).astype(np.float32)
+    hb = rng.uniform(0.1, 0.5, (n, 3)).astype(np.float32)
+    types = np.array([BOX, BOX] * n, np.int32)
+    params = np.zeros((2 * n, 3))
+    params[0::2] = 2 * ha
+    params[1::2] = 2 * hb
+    S = oracle.ShapeSet(types, params)
+    ca = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    cb = (ca + rng.uniform(-0.9, 0.9, (n, 3)) * (ha + hb)).astype(np.float32)
+    c = cb.astype(np.float64) - ca.astype(np.float64)
+    h = ha.astype(np.float64) + hb.astype(np.float64)
+    pen = h - np.abs(c)
+    best = pen.argmin(1)
+    tmin = pen.min(1)
+    axis = np.zeros((n, 3))
+    axis[np.arange(n), best] = np.sign(c[np.arange(n), best])
+    near = axis + rng.normal(scale=0.05, size=(n, 3))
+    near /= np.linalg.norm(near, axis=1, keepdims=True)
+    anyd = _unit(rng, n).astype(np.float64)
+    ta = oracle.pose12(np.stack([_pose(x) for x in ca]))
+    tb = oracle.pose12(np.stack([_pose(x) for x in cb]))
+    pa, pb = np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)
+    o_near, _ = S.incremental_penetration(pa, pb, ta, tb, near, threads=8)
+    o_any, _ = S.incremental_penetration(pa, pb, ta, tb, anyd, threads=8)
+    for o in (o_near, o_any):
+        ok = np.isin(o["epa_status"], [0, 4])
+        assert np.all(o["depth"][ok] >= tmin[ok] - 1e-9)
+    ok = np.isin(o_near["epa_status"], [0, 4])
+    err = np.abs(o_near["depth"] - tmin)[ok]
+    print(f"\n[incremental boxes] near-axis hints: {int(ok.sum())}/{n} converged, max |depth - min| {err.max():.2e}, "
+          f"exact on {int((err == 0).sum())}")
+    # the refinement stops once a step gains less than the tolerance (1e-6)
+    assert ok.sum() > 0.95 * n and err.max() < 2e-6 and (err == 0).mean() > 0.9
+
+
+def test_incremental_hulls_match_epa(oracle):
+    """Hull pairs, hinted with the EPA normal (the minimum-penetration
+    direction, EPA pinned separately): the refinement keeps EPA's depth."""
+    rng = np.random.default_rng(65)
+    n = 80
+    v, off, cnt, _ = W.ellipsoid_hulls(rng, 2 * n, 8, 40)
+    keep_v, keep_o, keep_c, tris, to, tc = [], [], [], [], [], []
+    o = q = 0
+    for i in range(2 * n):
+        P = v[off[i]:off[i] + cnt[i]]
+        k, t = W.hull_faces(P)
+        keep_v.append(P[k]); keep_o.append(o); keep_c.append(len(k)); o += len(k)
+        tris.append(t); to.append(q); tc.append(len(t)); q += len(t)
+    S = oracle.ShapeSet(np.full(2 * n, CONVEX, np.int32), np.zeros((2 * n, 3)), np.concatenate(keep_v), keep_o,
+                        keep_c, np.concatenate(tris), to, tc)
+    poses = W.poses(rng, 2 * n, 0.0, 0.1)
+    ta, tb = oracle.pose12(poses[0::2]), oracle.pose12(poses[1::2])
+    pa, pb = np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)
+    epa, _ = S.collide(pa, pb, ta, tb, 1, True, threads=8)
+    hit = epa["hit"] == 1
+    out, _ = S.incremental_penetration(pa, pb, ta, tb, epa["normal"], threads=8)
+    ok = hit & np.isin(out["epa_status"], [0, 4])
+    rel = np.abs(out["depth"] - epa["depth"])[ok] / np.maximum(epa["depth"][ok], 1e-3)
+    print(f"\n[incremental hulls] {int(ok.sum())}/{int(hit.sum())} colliding pairs refined, "
+          f"max relative |depth - EPA| {rel.max():.2e}")
+    assert ok.sum() > 0.9 * hit.sum() and hit.sum() > 30
+    assert rel.max() < 1e-4
