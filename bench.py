@@ -91,9 +91,9 @@ def cpu_baseline(h, geom, n_sample, threads, target_cpu_s=10.0):
 
 def measured_traffic(tuned):
     """HBM bytes per launch of the lane GJK kernel on this workload, from the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (each
-    scaled by the same counter's reading on a 1 GiB copy); None when the
-    group kernel was chosen (not profiled)."""
+    rocprofv3 passes summarised in profiles/ (read bytes from the L2's read
+    requests by size, written bytes from WRITE_SIZE); None when the group
+    kernel was chosen (not profiled)."""
     if tuned.get("gjk_group", 1) != 1:
         return None
     path = os.path.join(ROOT, "profiles", "lane_kernel_traffic.json")
