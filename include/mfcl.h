@@ -360,6 +360,22 @@ int mfcl_upload_meshes(mfcl_ctx* ctx, const float* verts_xyz, const uint64_t* ve
                        const uint32_t* tri_cnt, uint32_t n_meshes);
 /* Number of BVH nodes of mesh i (host query). */
 int mfcl_mesh_num_nodes(mfcl_ctx* ctx, uint32_t mesh, uint32_t* n_nodes);
+/* Batched fcl::collide(BVHModel<OBBRSS>, shape) with penetration disabled
+ * (OrientedNodeBVHSolver::MeshShapeIntersect, include/fcl/narrowphase/
+ * detail/traversal/collision/bvh_solver-inl.h:6-72): query i tests mesh
+ * mesh_ids[i] posed pose_mesh7 against shape shape_ids[i] of the shape
+ * table posed pose_shape7; leaf tests as GJKSolver::shapeTriangleIntersect
+ * (gjk_solver-inl.h:545-563: sphere and box closed forms, else MPR with
+ * req's GJK iteration cap and tolerance).  out: hit (-1: the traversal's
+ * stack overflowed), n_contacts = min(colliding triangles, max_contacts),
+ * bv_tests, leaf_tests; tri_ids (optional): n x max_contacts triangle ids
+ * (the reference's Contact::b1; b2 is Contact::NONE), ascending, -1 past
+ * n_contacts.  When more than max_contacts triangles collide, which ones are
+ * kept is not the reference's depth-first choice.  req->penetration must be
+ * 0 (MFCL_E_UNSUPPORTED otherwise).  Host pointers; synchronises. */
+int mfcl_mesh_shape_collide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* shape_ids,
+                            const float* pose_mesh7, const float* pose_shape7, uint64_t n,
+                            const mfcl_request* req, mfcl_mesh_result* out, int32_t* tri_ids);
 /* Copies mesh i's tree to host memory: mfcl_mesh_num_nodes() nodes of 64
  * bytes, breadth-first (float axes[9], column-major; float centre[3]; float
  * half extents[3]; int32 child: >= 0 first child, its sibling next; < 0 leaf

@@ -955,6 +955,22 @@ int mfcl_load_obj(const char* path, float* verts_xyz, uint64_t* n_verts, uint32_
   return MFCL_OK;
 }
 
+int mfcl_mesh_shape_collide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* shape_ids,
+                            const float* pose_mesh7, const float* pose_shape7, uint64_t n, const mfcl_request* req,
+                            mfcl_mesh_result* out, int32_t* tri_ids) {
+  if (!ctx || !req) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  if (n == 0) return MFCL_OK;
+  if (!mesh_ids || !shape_ids || !pose_mesh7 || !pose_shape7 || !out) return MFCL_E_INVALID;
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (n > 0xFFFFFFFFull) {
+    ctx->err = "too many queries in one call";
+    return MFCL_E_INVALID;
+  }
+  return meshShapeCollide(ctx, mesh_ids, shape_ids, pose_mesh7, pose_shape7, n, req, out, tri_ids);
+}
+
 static int meshCollideDev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,
                           const float* poseb, uint64_t n, int32_t max_contacts, mfcl_mesh_result* out,
                           int32_t* contacts, mfcl_mesh_contact* pen, void* stream) {

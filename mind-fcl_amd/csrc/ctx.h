@@ -199,6 +199,10 @@ int computeWalkSeeds(mfcl_ctx* ctx);
 int mprPenLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea, const float* poseb,
                  const float* dirs, uint64_t n, bool incremental, int max_it, double tol, mfcl_convex_result* out,
                  hipStream_t s);
+// meshshape.hip: mesh-shape collision over host arrays
+int meshShapeCollide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* shape_ids, const float* pose_m7,
+                     const float* pose_s7, uint64_t n, const mfcl_request* req, mfcl_mesh_result* out,
+                     int32_t* tri_ids);
 // capi.cpp: frees the shape table (convex tables included)
 void freeShapes(mfcl_ctx* ctx);
 // hull.hip

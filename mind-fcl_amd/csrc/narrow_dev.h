@@ -16,7 +16,7 @@
 
 namespace mfcl {
 
-enum ShapeType : int { kBox = 0, kSphere = 1, kCapsule = 3, kCylinder = 5, kConvex = 100 };
+enum ShapeType : int { kBox = 0, kSphere = 1, kCapsule = 3, kCylinder = 5, kConvex = 100, kTriangle = 101 };
 
 #define MFCL_LDS __attribute__((address_space(3)))
 
@@ -542,6 +542,15 @@ MFCL_D d3 primSupport(const DevShape& s, d3 dir) {
       if (zd == 0.0) return {0, 0, dir.z > 0 ? hh : -hh};
       const double k = r / zd;
       return {k * dir.x, k * dir.y, dir.z > 0 ? hh : -hh};
+    }
+    case kTriangle: {
+      // a mesh triangle (its three gathered vertices at verts), with the
+      // tie rule of triangleSupport (shape_gjk_interface-inl.h:17-38)
+      const float4 fa = s.verts[0], fb = s.verts[1], fc = s.verts[2];
+      const d3 a = {fa.x, fa.y, fa.z}, b = {fb.x, fb.y, fb.z}, c = {fc.x, fc.y, fc.z};
+      const double da = dot(dir, a), db = dot(dir, b), dc = dot(dir, c);
+      if (da > db) return dc > da ? c : a;
+      return dc > db ? c : b;
     }
     default:
       return {0, 0, 0};

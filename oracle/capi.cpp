@@ -337,6 +337,57 @@ void* orc_meshset_create(const float* xyz, const int64_t* vert_off, const int* v
   return ms;
 }
 void orc_meshset_destroy(void* m) { delete static_cast<MeshSet*>(m); }
+
+// Mesh-shape collision, penetration disabled (OrientedNodeBVHSolver::
+// MeshShapeIntersect, include/fcl/narrowphase/detail/traversal/collision/
+// bvh_solver-inl.h:6-72, leaf ShapeSimplexIntersect, shape_pair_intersect-
+// inl.h:133-172): the colliding triangles of mesh `mesh` (posed tfm12)
+// against shape `shape` (posed tfs12), by testing every triangle -- the set
+// does not depend on the tree -- in ascending id order.  Writes at most cap
+// ids, returns the count.
+int64_t orc_mesh_shape_all(void* ms, void* hs, const int* type, const double* params, const int* hull_idx,
+                           int mesh, int shape, const double* tfm12, const double* tfs12, const double* sp6,
+                           int* out, int64_t cap) {
+  const Mesh& m = static_cast<MeshSet*>(ms)->meshes[mesh];
+  ShapeTable st{static_cast<HullSet*>(hs), type, params, hull_idx};
+  const SolverParams sp = solverFrom(sp6);
+  const Shape sh = st.get(shape);
+  const Tf tm = tfFrom12(tfm12), ts = tfFrom12(tfs12);
+  int64_t n = 0;
+  const int nt = static_cast<int>(m.tris.size() / 3);
+  for (int t = 0; t < nt; ++t) {
+    const V3 tri[3] = {m.verts[m.tris[3 * t]], m.verts[m.tris[3 * t + 1]], m.verts[m.tris[3 * t + 2]]};
+    if (shapeTriangleHit(sh, ts, tri, tm, sp.gjk_max_iterations, sp.gjk_tolerance)) {
+      if (n < cap) out[n] = t;
+      ++n;
+    }
+  }
+  return n;
+}
+
+// The same over a batch of (mesh, shape) queries: hit flag and count.
+double orc_mesh_shape_batch(void* ms, void* hs, const int* type, const double* params, const int* hull_idx,
+                            const int* mesh, const int* shape, const double* tfm12, const double* tfs12, int64_t n,
+                            const double* sp6, int n_threads, int* hit, int64_t* count) {
+  const auto t0 = std::chrono::steady_clock::now();
+  parallelFor(n, n_threads, [&](int64_t i) {
+    count[i] = orc_mesh_shape_all(ms, hs, type, params, hull_idx, mesh[i], shape[i], tfm12 + 12 * i,
+                                  tfs12 + 12 * i, sp6, nullptr, 0);
+    hit[i] = count[i] > 0;
+  });
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// One leaf test (for pins): shape `shape` posed tfs12 against the triangle
+// P (9 doubles) posed tfm12.
+int orc_shape_triangle_hit(void* hs, const int* type, const double* params, const int* hull_idx, int shape,
+                           const double* tfs12, const double* P9, const double* tfm12, const double* sp6) {
+  ShapeTable st{static_cast<HullSet*>(hs), type, params, hull_idx};
+  const SolverParams sp = solverFrom(sp6);
+  const V3 tri[3] = {V3(P9[0], P9[1], P9[2]), V3(P9[3], P9[4], P9[5]), V3(P9[6], P9[7], P9[8])};
+  return shapeTriangleHit(st.get(shape), tfFrom12(tfs12), tri, tfFrom12(tfm12), sp.gjk_max_iterations,
+                          sp.gjk_tolerance);
+}
 int orc_mesh_num_nodes(void* m, int i) {
   return static_cast<int>(static_cast<MeshSet*>(m)->meshes[i].nodes.size());
 }

@@ -21,7 +21,7 @@
 namespace oracle {
 
 // Shape type codes follow cvx_collide::GJKShapeType (include/fcl/cvx_collide/gjk_shape.h:12-28).
-enum ShapeType : int { kBox = 0, kSphere = 1, kCapsule = 3, kCylinder = 5, kConvex = 100 };
+enum ShapeType : int { kBox = 0, kSphere = 1, kCapsule = 3, kCylinder = 5, kConvex = 100, kTriangle = 101 };
 
 // A convex hull: vertices plus the vertex adjacency used by the reference's
 // hill-climbing support search (include/fcl/geometry/shape/convex.h:212-259).
@@ -44,6 +44,7 @@ struct Shape {
   // box: full side lengths; sphere: radius; capsule/cylinder: radius, full length.
   double p[3]{0, 0, 0};
   const Hull* hull{nullptr};
+  V3 tri[3];  // triangle (TriangleP): its vertices
 };
 
 V3 shapeSupport(const Shape& s, const V3& d);
@@ -131,6 +132,18 @@ struct IncrementalOut {
   int iterations{0};  // outer iterations
 };
 IncrementalStatus mprIncremental(const MinkDiff& md, const V3& d, int max_iter, double tol, IncrementalOut& out);
+
+// Mesh-shape leaf tests (a triangle of a mesh posed by tf_tri against a
+// shape posed by tf_shape, penetration disabled): GJKSolver::
+// shapeTriangleIntersect (include/fcl/narrowphase/detail/gjk_solver-inl.h:
+// 545-563) -> sphereTriangleIntersect (primitive_shape_algorithm/
+// sphere_triangle-inl.h:107-175), boxTriangleIntersect (box_triangle-inl.h:
+// 72-137), else MPR on (shape, TriangleP) (gjk_solver-inl.h:452-480).
+bool sphereTriangleHit(double r, const Tf& tf_sphere, const V3& P1, const V3& P2, const V3& P3);
+bool boxTriangleHit(const double side[3], const Tf& tf_box, const V3& P1, const V3& P2, const V3& P3,
+                    const Tf& tf_tri);
+bool shapeTriangleHit(const Shape& s, const Tf& tf_shape, const V3 tri[3], const Tf& tf_tri, int max_iter,
+                      double tol);
 
 // ---------------------------------------------------------------------------
 // GJKSolver-level queries (include/fcl/narrowphase/detail/gjk_solver-inl.h).

@@ -474,4 +474,137 @@ double capsuleCapsuleDistance(double r1, double lz1, const Tf& tf1, double r2, d
   return norm(c1 - c2) - r1 - r2;
 }
 
+// ---- mesh-shape leaf tests -------------------------------------------------
+
+namespace {
+
+// sphere_triangle-inl.h:52-74: squared distance from p to segment
+// [from, to] and the nearest point.
+double segmentSqrDistance(const V3& from, const V3& to, const V3& p, V3& nearest) {
+  V3 diff = p - from;
+  const V3 v = to - from;
+  double t = dot(v, diff);
+  if (t > 0) {
+    const double vv = dot(v, v);
+    if (t < vv) {
+      t /= vv;
+      diff -= v * t;
+    } else {
+      t = 1;
+      diff -= v;
+    }
+  } else {
+    t = 0;
+  }
+  nearest = from + v * t;
+  return dot(diff, diff);
+}
+
+// sphere_triangle-inl.h:77-104
+bool projectInTriangle(const V3& p1, const V3& p2, const V3& p3, const V3& n, const V3& p) {
+  const V3 e1 = p2 - p1, e2 = p3 - p2, e3 = p1 - p3;
+  const double r1 = dot(cross(e1, n), p - p1);
+  const double r2 = dot(cross(e2, n), p - p2);
+  const double r3 = dot(cross(e3, n), p - p3);
+  return (r1 > 0 && r2 > 0 && r3 > 0) || (r1 <= 0 && r2 <= 0 && r3 <= 0);
+}
+
+// box_triangle-inl.h:9-69: is the box separated from the triangle along
+// box axis x / y / z crossed with the edge e0 (v0 on the edge, v1 off it)?
+bool separatedByEdgeAxes(const double h[3], const V3& e0, const V3& v0, const V3& v1) {
+  const double ax = std::fabs(e0.x), ay = std::fabs(e0.y), az = std::fabs(e0.z);
+  auto sep = [](double p0, double p1, double rad) {
+    if (p0 < p1) return p0 > rad || p1 < -rad;
+    return p1 > rad || p0 < -rad;
+  };
+  if (sep(v0.y * e0.z - v0.z * e0.y, v1.y * e0.z - v1.z * e0.y, h[1] * az + h[2] * ay)) return true;
+  if (sep(v0.x * e0.z - v0.z * e0.x, v1.x * e0.z - v1.z * e0.x, h[0] * az + h[2] * ax)) return true;
+  if (sep(v0.x * e0.y - v0.y * e0.x, v1.x * e0.y - v1.y * e0.x, h[0] * ay + h[1] * ax)) return true;
+  return false;
+}
+
+void minMax3(double a, double b, double c, double& mn, double& mx) {
+  mn = mx = a;
+  if (b < mn) mn = b;
+  if (b > mx) mx = b;
+  if (c < mn) mn = c;
+  if (c > mx) mx = c;
+}
+
+}  // namespace
+
+bool sphereTriangleHit(double radius, const Tf& tf_sphere, const V3& P1, const V3& P2, const V3& P3) {
+  V3 n = cross(P2 - P1, P3 - P1);
+  normalize(n);
+  const V3 c = tf_sphere.t;
+  const double rt = radius + std::numeric_limits<double>::epsilon();
+  double dist = dot(c - P1, n);
+  if (dist < 0) {
+    dist *= -1;
+    n *= -1;
+  }
+  if (!(dist < rt)) return false;
+  bool has = false;
+  V3 cp;
+  if (projectInTriangle(P1, P2, P3, n, c)) {
+    has = true;
+    cp = c - n * dist;
+  } else {
+    const double cr2 = rt * rt;
+    V3 q;
+    if (segmentSqrDistance(P1, P2, c, q) < cr2) { has = true; cp = q; }
+    if (segmentSqrDistance(P2, P3, c, q) < cr2) { has = true; cp = q; }
+    if (segmentSqrDistance(P3, P1, c, q) < cr2) { has = true; cp = q; }
+  }
+  if (!has) return false;
+  const V3 d = cp - c;
+  return dot(d, d) < rt * rt;
+}
+
+bool boxTriangleHit(const double side[3], const Tf& tf_box, const V3& P1, const V3& P2, const V3& P3,
+                    const Tf& tf_tri) {
+  const Tf t = inverse(tf_box) * tf_tri;
+  const V3 v0 = t * P1, v1 = t * P2, v2 = t * P3;
+  const double h[3] = {0.5 * side[0], 0.5 * side[1], 0.5 * side[2]};
+  double mn, mx;
+  minMax3(v0.x, v1.x, v2.x, mn, mx);
+  if (mx < -h[0] || mn > h[0]) return false;
+  minMax3(v0.y, v1.y, v2.y, mn, mx);
+  if (mx < -h[1] || mn > h[1]) return false;
+  minMax3(v0.z, v1.z, v2.z, mn, mx);
+  if (mx < -h[2] || mn > h[2]) return false;
+  const V3 e0 = v1 - v0;
+  if (separatedByEdgeAxes(h, e0, v0, v2)) return false;
+  const V3 e1 = v2 - v1;
+  if (separatedByEdgeAxes(h, e1, v1, v0)) return false;
+  const V3 e2 = v0 - v2;
+  if (separatedByEdgeAxes(h, e2, v0, v1)) return false;
+  const V3 n = cross(e0, e1);
+  const double nv[3] = {n.x, n.y, n.z}, pv[3] = {v0.x, v0.y, v0.z};
+  double pmin[3], pmax[3];
+  for (int i = 0; i < 3; ++i) {
+    if (nv[i] > 0.0) {
+      pmin[i] = -h[i] - pv[i];
+      pmax[i] = h[i] - pv[i];
+    } else {
+      pmin[i] = h[i] - pv[i];
+      pmax[i] = -h[i] - pv[i];
+    }
+  }
+  return dot(n, V3(pmin[0], pmin[1], pmin[2])) <= 0 && dot(n, V3(pmax[0], pmax[1], pmax[2])) >= 0;
+}
+
+bool shapeTriangleHit(const Shape& s, const Tf& tf_shape, const V3 tri[3], const Tf& tf_tri, int max_iter,
+                      double tol) {
+  if (s.type == kSphere) return sphereTriangleHit(s.p[0], tf_shape, tf_tri * tri[0], tf_tri * tri[1], tf_tri * tri[2]);
+  if (s.type == kBox) return boxTriangleHit(s.p, tf_shape, tri[0], tri[1], tri[2], tf_tri);
+  Shape t;
+  t.type = kTriangle;
+  t.tri[0] = tri[0];
+  t.tri[1] = tri[1];
+  t.tri[2] = tri[2];
+  const MinkDiff md = makeMinkDiff(s, tf_shape, t, tf_tri);
+  return mprIntersect(md, max_iter, tol) == MPRStatus::Intersect;
+}
+
 }  // namespace oracle

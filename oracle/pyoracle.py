@@ -57,6 +57,12 @@ def lib() -> C.CDLL:
         L.orc_collide_contacts_batch.restype = d
         L.orc_mpr_penetration_batch.argtypes = [p, p, p, p, p, p, p, p, p, i64, i32, p, d, i32, i32, p]
         L.orc_mpr_penetration_batch.restype = d
+        L.orc_mesh_shape_all.argtypes = [p, p, p, p, p, i32, i32, p, p, p, p, i64]
+        L.orc_mesh_shape_all.restype = i64
+        L.orc_mesh_shape_batch.argtypes = [p, p, p, p, p, p, p, p, p, i64, p, i32, p, p]
+        L.orc_mesh_shape_batch.restype = d
+        L.orc_shape_triangle_hit.argtypes = [p, p, p, p, i32, p, p, p, p]
+        L.orc_shape_triangle_hit.restype = i32
         L.orc_signed_distance_batch.argtypes = [p, p, p, p, p, p, p, p, i64, p, p, i32, p]
         L.orc_signed_distance_batch.restype = d
         L.orc_pair_algorithm.argtypes = [p, p, p, p, i32, i32, p, p, i32, i32, d, p, p]
@@ -206,6 +212,39 @@ class ShapeSet:
     def incremental_penetration(self, pa, pb, tfa12, tfb12, hints, max_contacts=1, tol=1e-6, solver=None,
                                 threads=1):
         return self.directed_penetration(pa, pb, tfa12, tfb12, hints, max_contacts, tol, solver, threads, True)
+
+    def triangle_hit(self, shape, tfs12, tri9, tfm12, solver=None) -> bool:
+        """Leaf test of mesh-shape collision: shape posed tfs12 vs the
+        triangle tri9 (3 x 3) posed tfm12."""
+        sp = None if solver is None else np.ascontiguousarray(solver, np.float64)
+        return bool(lib().orc_shape_triangle_hit(self.hs, _p(self.types), _p(self.params), _p(self.hull_idx),
+                                                 int(shape), _p(np.ascontiguousarray(tfs12, np.float64)),
+                                                 _p(np.ascontiguousarray(tri9, np.float64)),
+                                                 _p(np.ascontiguousarray(tfm12, np.float64)), _p(sp)))
+
+    def mesh_shape(self, M: "MeshSet", mesh_ids, shape_ids, tfm12, tfs12, solver=None, threads=1):
+        """Mesh-shape collision (penetration disabled): hit flags and the
+        number of colliding triangles per query."""
+        mesh_ids = np.ascontiguousarray(mesh_ids, np.int32)
+        shape_ids = np.ascontiguousarray(shape_ids, np.int32)
+        tfm12 = np.ascontiguousarray(tfm12, np.float64).reshape(-1, 12)
+        tfs12 = np.ascontiguousarray(tfs12, np.float64).reshape(-1, 12)
+        sp = None if solver is None else np.ascontiguousarray(solver, np.float64)
+        n = len(mesh_ids)
+        hit = np.zeros(n, np.int32)
+        cnt = np.zeros(n, np.int64)
+        lib().orc_mesh_shape_batch(M.h, self.hs, _p(self.types), _p(self.params), _p(self.hull_idx), _p(mesh_ids),
+                                   _p(shape_ids), _p(tfm12), _p(tfs12), n, _p(sp), threads, _p(hit), _p(cnt))
+        return hit, cnt
+
+    def mesh_shape_all(self, M: "MeshSet", mesh, shape, tfm12, tfs12, solver=None, cap=1 << 20):
+        """Every colliding triangle id (ascending) of one mesh-shape query."""
+        sp = None if solver is None else np.ascontiguousarray(solver, np.float64)
+        out = np.zeros(cap, np.int32)
+        n = lib().orc_mesh_shape_all(M.h, self.hs, _p(self.types), _p(self.params), _p(self.hull_idx), int(mesh),
+                                     int(shape), _p(np.ascontiguousarray(tfm12, np.float64)),
+                                     _p(np.ascontiguousarray(tfs12, np.float64)), _p(sp), _p(out), cap)
+        return out[:min(n, cap)], int(n)
 
     def signed_distance(self, pa, pb, tfa12, tfb12, solver=None, guess=None, threads=1):
         pa = np.ascontiguousarray(pa, np.int32)

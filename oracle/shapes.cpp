@@ -130,14 +130,22 @@ V3 shapeSupport(const Shape& s, const V3& dir) {
       const Hull& h = *s.hull;
       return h.verts[h.walk ? hullExtremeWalk(h, dir) : hullExtremeNaive(h, dir)];
     }
+    case kTriangle: {
+      // triangleSupport, shape_gjk_interface-inl.h:17-38 (its tie rule)
+      const double da = dot(dir, s.tri[0]), db = dot(dir, s.tri[1]), dc = dot(dir, s.tri[2]);
+      if (da > db) return dc > da ? s.tri[2] : s.tri[0];
+      return dc > db ? s.tri[2] : s.tri[1];
+    }
     default:
       return V3(0, 0, 0);
   }
 }
 
-// gjk_shape.hpp:28-32 (origin) and gjk_solver_cvx-inl.h:117-125 (hull mean).
+// gjk_shape.hpp:28-32 (origin), gjk_solver_cvx-inl.h:117-125 (hull mean),
+// shape_gjk_interface-inl.h:8-15 (triangle centroid).
 V3 shapeInterior(const Shape& s) {
   if (s.type == kConvex) return s.hull->interior;
+  if (s.type == kTriangle) return (s.tri[0] + s.tri[1] + s.tri[2]) / 3.0;
   return V3(0, 0, 0);
 }
 
