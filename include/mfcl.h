@@ -372,10 +372,30 @@ int mfcl_mesh_num_nodes(mfcl_ctx* ctx, uint32_t mesh, uint32_t* n_nodes);
  * (the reference's Contact::b1; b2 is Contact::NONE), ascending, -1 past
  * n_contacts.  When more than max_contacts triangles collide, which ones are
  * kept is not the reference's depth-first choice.  req->penetration must be
- * 0 (MFCL_E_UNSUPPORTED otherwise).  Host pointers; synchronises. */
+ * 0 (mfcl_mesh_shape_collide_contacts takes the penetration-enabled form).
+ * Host pointers; synchronises. */
 int mfcl_mesh_shape_collide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* shape_ids,
                             const float* pose_mesh7, const float* pose_shape7, uint64_t n,
                             const mfcl_request* req, mfcl_mesh_result* out, int32_t* tri_ids);
+/* The same with penetration enabled (req->penetration = 1; the same
+ * traversal, bvh_solver-inl.h:6-72, with the leaf test's contact outputs,
+ * shape_pair_intersect-inl.h:133-172): each colliding triangle adds one
+ * contact from GJKSolver::shapeTriangleIntersect (gjk_solver-inl.h:545-563):
+ * sphereTriangleIntersect's contact for spheres (sphere_triangle-inl.h:
+ * 107-187: pos = the nearest point found, depth = -(r - distance)), else GJK
+ * from -UnitX and EPA on (shape, triangle) with req's GJK and EPA caps and
+ * tolerances (gjk_solver-inl.h:452-527: a hit unless EPA fails; pos = the
+ * witness midpoint in the world, normal from the witnesses, depth = the
+ * EPA depth) -- boxes included, whose closed form is boolean only.  Each
+ * contact has b1 = the triangle, b2 = -1 and the normal reversed as the
+ * traversal writes it (ContactMeta::reverse_normal, bvh_solver-inl.h:
+ * 55-65).  contacts: n x max_contacts records, ascending b1, b1 = -1 past
+ * n_contacts; which contacts of a larger set are kept follows the traversal
+ * here.  Host pointers; synchronises. */
+int mfcl_mesh_shape_collide_contacts(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* shape_ids,
+                                     const float* pose_mesh7, const float* pose_shape7, uint64_t n,
+                                     const mfcl_request* req, mfcl_mesh_result* out,
+                                     mfcl_mesh_contact* contacts);
 /* Copies mesh i's tree to host memory: mfcl_mesh_num_nodes() nodes of 64
  * bytes, breadth-first (float axes[9], column-major; float centre[3]; float
  * half extents[3]; int32 child: >= 0 first child, its sibling next; < 0 leaf

@@ -968,7 +968,31 @@ int mfcl_mesh_shape_collide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint3
     ctx->err = "too many queries in one call";
     return MFCL_E_INVALID;
   }
-  return meshShapeCollide(ctx, mesh_ids, shape_ids, pose_mesh7, pose_shape7, n, req, out, tri_ids);
+  if (req->penetration) {
+    ctx->err = "mfcl_mesh_shape_collide takes penetration-disabled requests; use mfcl_mesh_shape_collide_contacts";
+    return MFCL_E_INVALID;
+  }
+  return meshShapeCollide(ctx, mesh_ids, shape_ids, pose_mesh7, pose_shape7, n, req, out, tri_ids, nullptr);
+}
+
+int mfcl_mesh_shape_collide_contacts(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* shape_ids,
+                                     const float* pose_mesh7, const float* pose_shape7, uint64_t n,
+                                     const mfcl_request* req, mfcl_mesh_result* out, mfcl_mesh_contact* contacts) {
+  if (!ctx || !req) return MFCL_E_INVALID;
+  MFCL_NOT_WHILE_CAPTURING(ctx);
+  if (n == 0) return MFCL_OK;
+  if (!mesh_ids || !shape_ids || !pose_mesh7 || !pose_shape7 || !out || !contacts) return MFCL_E_INVALID;
+  if (!req->penetration) {
+    ctx->err = "mfcl_mesh_shape_collide_contacts needs a request with penetration enabled";
+    return MFCL_E_INVALID;
+  }
+  MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (n > 0xFFFFFFFFull) {
+    ctx->err = "too many queries in one call";
+    return MFCL_E_INVALID;
+  }
+  return meshShapeCollide(ctx, mesh_ids, shape_ids, pose_mesh7, pose_shape7, n, req, out, nullptr, contacts);
 }
 
 static int meshCollideDev(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float* posea,

@@ -202,7 +202,7 @@ int mprPenLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const fl
 // meshshape.hip: mesh-shape collision over host arrays
 int meshShapeCollide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* shape_ids, const float* pose_m7,
                      const float* pose_s7, uint64_t n, const mfcl_request* req, mfcl_mesh_result* out,
-                     int32_t* tri_ids);
+                     int32_t* tri_ids, mfcl_mesh_contact* contacts);
 // capi.cpp: frees the shape table (convex tables included)
 void freeShapes(mfcl_ctx* ctx);
 // hull.hip

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "epa_dev.h"
 #include "narrow_dev.h"
 #include "obb_dev.h"
 
@@ -169,9 +170,16 @@ MFCL_D double segSqrDist(d3 from, d3 to, d3 p, d3& nearest) {
   return dot(diff, diff);
 }
 
-// sphere_triangle-inl.h:77-175 (boolean): sphere of radius r centred at c
-// against the world triangle (P1, P2, P3).
-MFCL_D bool sphereTriHit(double r, d3 c, d3 P1, d3 P2, d3 P3) {
+// sphere_triangle-inl.h:107-187: sphere of radius r centred at c against
+// the world triangle (P1, P2, P3).  c_out (optional) receives the contact as
+// the reference writes it: pos = the nearest point found, normal = that
+// point minus the centre, normalised (the flipped plane normal when they
+// coincide), depth = -(r - distance) (-r when they coincide).
+struct SphereTriContact {
+  d3 pos, normal;
+  double depth;
+};
+MFCL_D bool sphereTriContact(double r, d3 c, d3 P1, d3 P2, d3 P3, SphereTriContact* c_out) {
   d3 n = cross(P2 - P1, P3 - P1);
   const double nn = norm(n);
   if (nn > 0) n = n / nn;
@@ -199,8 +207,21 @@ MFCL_D bool sphereTriHit(double r, d3 c, d3 P1, d3 P2, d3 P3) {
   }
   if (!has) return false;
   const d3 d = cp - c;
-  return dot(d, d) < rt * rt;
+  const double d2 = dot(d, d);
+  if (!(d2 < rt * rt)) return false;
+  if (c_out) {
+    c_out->pos = cp;
+    if (d2 > 0) {
+      c_out->normal = normalized(d);
+      c_out->depth = -(r - sqrt(d2));
+    } else {
+      c_out->normal = n * -1.0;
+      c_out->depth = -r;
+    }
+  }
+  return true;
 }
+MFCL_D bool sphereTriHit(double r, d3 c, d3 P1, d3 P2, d3 P3) { return sphereTriContact(r, c, P1, P2, P3, nullptr); }
 
 // box_triangle-inl.h:9-69
 MFCL_D bool sepEdgeAxes(const double h[3], d3 e0, d3 v0, d3 v1) {
@@ -305,6 +326,156 @@ __global__ void __launch_bounds__(256) msLeafKernel(MeshShapeArgs A, uint64_t nc
     A.tri_out[q * static_cast<uint64_t>(A.max_contacts) + slot] = static_cast<int32_t>(t);
 }
 
+// Penetration enabled (shapeTriangleIntersect with contact outputs,
+// gjk_solver-inl.h:545-563): one wavefront per candidate, taken from a
+// shared counter.  Spheres take sphereTriContact; every other shape runs
+// GJK from -UnitX and EPA on (shape, triangle) (gjk_solver-inl.h:452-527),
+// the EPA polytope in the wavefront's LDS slice (cap_faces faces) and,
+// when it might outgrow it, in the wavefront's device-memory polytope of the
+// request's capacity.  A hit records the contact with the traversal's
+// normal reversal (bvh_solver-inl.h:55-65, shape_pair_intersect-inl.h:
+// 19-26): b1 = the triangle, b2 = -1.
+constexpr size_t kMsCfgBytes = (sizeof(MinkCfg) + 15) & ~static_cast<size_t>(15);
+constexpr size_t kMsRawBytes = (sizeof(RawF) + 15) & ~static_cast<size_t>(15);
+constexpr size_t kMsPolyBytes = (sizeof(Poly) + 15) & ~static_cast<size_t>(15);
+inline size_t msSliceBytes(int cap_faces) { return kMsCfgBytes + kMsRawBytes + kMsPolyBytes + arenaBytes(cap_faces); }
+
+struct MsPen {
+  mfcl_mesh_contact* contacts;  // n * max_contacts
+  int cap_faces;                // LDS polytope capacity
+  size_t slice_bytes;           // LDS bytes per wavefront
+  char* arena;                  // device-memory polytopes, one per wavefront of the grid
+  size_t arena_slice;
+  int epa_max_faces, epa_max_it;
+  double epa_tol;
+  uint32_t* next;               // candidate cursor
+};
+
+__global__ void __launch_bounds__(256, 2) msPenKernel(MeshShapeArgs A, MsPen E, uint64_t ncand) {
+  extern __shared__ __align__(16) char ms_lds[];
+  const int lane = threadIdx.x & 63;
+  const ArenaDims dims = arenaDims(E.cap_faces);
+  MFCL_LDS char* slice = (MFCL_LDS char*)(ms_lds) + static_cast<size_t>(threadIdx.x >> 6) * E.slice_bytes;
+  MFCL_LDS MinkCfg* cfg = (MFCL_LDS MinkCfg*)(slice);
+  MFCL_LDS RawF* raw = (MFCL_LDS RawF*)(slice + kMsCfgBytes);
+  MFCL_LDS char* base = slice + kMsCfgBytes + kMsRawBytes + kMsPolyBytes;
+  Poly& poly = *reinterpret_cast<Poly*>(ms_lds + static_cast<size_t>(threadIdx.x >> 6) * E.slice_bytes +
+                                        kMsCfgBytes + kMsRawBytes);
+  poly.V = (MFCL_LDS EVert*)(base);
+  poly.E = (MFCL_LDS EEdge*)(base + sizeof(EVert) * dims.nv);
+  poly.F = (MFCL_LDS EFace*)(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv);
+  poly.BL = (MFCL_LDS BorderRec*)(base + sizeof(EVert) * dims.nv + sizeof(EEdge) * dims.nv +
+                                  sizeof(EFace) * dims.nf);
+  poly.nv = dims.nv;
+  poly.nf = dims.nf;
+  const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const SpillTarget st{E.cap_faces < E.epa_max_faces ? E.arena + wave * E.arena_slice : nullptr, E.epa_max_faces};
+  for (;;) {
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(E.next, 1u);
+    k = __shfl(k, 0);
+    if (k >= ncand) break;
+    const uint64_t c = A.cand[k];
+    const uint64_t q = c >> 32;
+    const uint32_t t = static_cast<uint32_t>(c & 0xFFFFFFFFull);
+    const uint32_t mesh = A.mesh_ids[q], shp = A.shape_ids[q];
+    const tf3 tm = poseFromQuat(A.pose_m + 7 * q), ts = poseFromQuat(A.pose_s + 7 * q);
+    const float4* tri = A.tris + 3 * (static_cast<uint64_t>(A.tri_base[mesh]) + t);
+    const d3 P1 = {tri[0].x, tri[0].y, tri[0].z}, P2 = {tri[1].x, tri[1].y, tri[1].z},
+             P3 = {tri[2].x, tri[2].y, tri[2].z};
+    const DevShape S = loadShape(A.T, shp);
+    bool hit = false;
+    d3 pos = {0, 0, 0}, nrm = {0, 0, 0};
+    double depth = 0;
+    if (S.type == kSphere) {
+      SphereTriContact sc;
+      hit = sphereTriContact(S.p0, ts.t, apply(tm, P1), apply(tm, P2), apply(tm, P3), &sc);
+      pos = sc.pos;
+      nrm = sc.normal;
+      depth = sc.depth;
+    } else {
+      DevShape B;
+      B.type = kTriangle;
+      B.p0 = B.p1 = B.p2 = 0;
+      B.verts = tri;
+      B.nv = 3;
+      B.rbound = 0.f;
+      B.lv = nullptr;
+      B.ls = 0;
+      B.nbr = nullptr;
+      B.seed = nullptr;
+      B.wblk = nullptr;
+      writeCfg(cfg, S, ts, B, tm, d3{0, 0, 0}, d3{0, 0, 0});
+      MinkL<64> md{cfg, lane, 0};
+      Smp smp;
+      GjkOut g;
+      gjkEvaluate(md, smp, d3{-1, 0, 0}, A.gjk_tol, A.gjk_max_it, false, g);
+      if (g.status == kGjkIntersect) {
+        SmpD sd;
+        sd.rank = smp.rank;
+        storeMVD(sd.a, smp.a.v, slotDir(cfg, smp.a.id));
+        storeMVD(sd.b, smp.b.v, slotDir(cfg, smp.b.id));
+        storeMVD(sd.c, smp.c.v, slotDir(cfg, smp.c.id));
+        storeMVD(sd.e, smp.e.v, slotDir(cfg, smp.e.id));
+        EpaOut eo;
+        if (st.base) epaEvaluate<true>(poly, md, sd, E.epa_tol, E.epa_max_it, eo, *raw, st);
+        else epaEvaluate<false>(poly, md, sd, E.epa_tol, E.epa_max_it, eo, *raw, st);
+        if (eo.status != kEpaFailed) {
+          hit = true;
+          d3 n1 = eo.p0 - eo.p1;
+          n1 = sqnorm(n1) <= 0 ? d3{0, 0, 1} : normalized(n1);
+          pos = apply(ts, 0.5 * (eo.p0 + eo.p1));
+          nrm = mul(ts.R, n1);
+          depth = eo.depth;
+        }
+      }
+    }
+    if (lane != 0 || !hit) continue;
+    const uint32_t slot = atomicAdd(&A.counts[q], 1u);
+    if (static_cast<int>(slot) < A.max_contacts) {
+      mfcl_mesh_contact& o = E.contacts[q * static_cast<uint64_t>(A.max_contacts) + slot];
+      o.b1 = static_cast<int32_t>(t);
+      o.b2 = -1;
+      o.pos[0] = pos.x; o.pos[1] = pos.y; o.pos[2] = pos.z;
+      o.normal[0] = -nrm.x; o.normal[1] = -nrm.y; o.normal[2] = -nrm.z;
+      o.depth = depth;
+    }
+  }
+}
+
+// Result headers of a penetration-enabled query; the kept contacts sorted
+// by triangle id.
+__global__ void msFinishPenKernel(MeshShapeArgs A, mfcl_mesh_contact* contacts, mfcl_mesh_result* out) {
+  const uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= A.n) return;
+  const uint32_t cnt = A.counts[q];
+  const int kept = static_cast<int>(cnt) < A.max_contacts ? static_cast<int>(cnt) : A.max_contacts;
+  mfcl_mesh_result r;
+  r.hit = A.overflow[q] ? -1 : (cnt > 0 ? 1 : 0);
+  r.n_contacts = kept;
+  r.bv_tests = static_cast<int32_t>(A.bv_tests[q]);
+  r.leaf_tests = static_cast<int32_t>(A.leaves[q]);
+  out[q] = r;
+  mfcl_mesh_contact* o = contacts + q * static_cast<uint64_t>(A.max_contacts);
+  for (int i = 1; i < kept; ++i) {
+    const mfcl_mesh_contact key = o[i];
+    int j = i - 1;
+    while (j >= 0 && o[j].b1 > key.b1) {
+      o[j + 1] = o[j];
+      --j;
+    }
+    o[j + 1] = key;
+  }
+  for (int i = kept; i < A.max_contacts; ++i) {
+    mfcl_mesh_contact z;
+    z.b1 = z.b2 = -1;
+    z.pos[0] = z.pos[1] = z.pos[2] = 0;
+    z.normal[0] = z.normal[1] = z.normal[2] = 0;
+    z.depth = 0;
+    o[i] = z;
+  }
+}
+
 __global__ void msFinishKernel(MeshShapeArgs A, mfcl_mesh_result* out) {
   const uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= A.n) return;
@@ -354,18 +525,24 @@ struct DevAllocs {
 
 int meshShapeCollide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* shape_ids, const float* pose_m7,
                      const float* pose_s7, uint64_t n, const mfcl_request* req, mfcl_mesh_result* out,
-                     int32_t* tri_ids) {
+                     int32_t* tri_ids, mfcl_mesh_contact* contacts) {
   if (ctx->n_meshes == 0 || ctx->n_shapes == 0) {
     ctx->err = "mesh-shape collision needs uploaded meshes and shapes";
     return MFCL_E_STATE;
   }
-  if (req->penetration) {
-    ctx->err = "mesh-shape contacts with penetration are not built (boolean and triangle ids only)";
-    return MFCL_E_UNSUPPORTED;
+  const bool pen = req->penetration != 0;
+  if (pen != (contacts != nullptr)) {
+    ctx->err = pen ? "a penetration-enabled mesh-shape query writes contacts: pass a contact array"
+                   : "a contact array needs a request with penetration enabled";
+    return MFCL_E_INVALID;
   }
   const int maxc = req->max_contacts;
   if (maxc < 1 || maxc > 1024) {
     ctx->err = "max_contacts must be in [1, 1024]";
+    return MFCL_E_INVALID;
+  }
+  if (pen && (req->epa_max_faces < 4 || req->epa_max_faces > 8192)) {
+    ctx->err = "epa_max_faces out of range [4, 8192]";
     return MFCL_E_INVALID;
   }
   for (uint64_t i = 0; i < n; ++i) {
@@ -381,7 +558,7 @@ int meshShapeCollide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* sh
   auto* dpm = static_cast<float*>(m.get(28 * n));
   auto* dps = static_cast<float*>(m.get(28 * n));
   auto* dcnt = static_cast<uint32_t*>(m.get(4 * 4 * n));  // counts, leaves, bv tests, overflow
-  auto* dcc = static_cast<unsigned long long*>(m.get(8));
+  auto* dcc = static_cast<unsigned long long*>(m.get(16));  // candidate count, EPA cursor
   auto* dout = static_cast<mfcl_mesh_result*>(m.get(sizeof(mfcl_mesh_result) * n));
   auto* dtri = static_cast<int32_t*>(m.get(4 * n * static_cast<uint64_t>(maxc)));
   if (!dm || !ds || !dpm || !dps || !dcnt || !dcc || !dout || !dtri) {
@@ -437,13 +614,54 @@ int meshShapeCollide(mfcl_ctx* ctx, const uint32_t* mesh_ids, const uint32_t* sh
     ctx->err = "mesh-shape candidate count changed between runs";
     return MFCL_E_STATE;
   }
-  if (ncand)
-    msLeafKernel<<<static_cast<unsigned>((ncand * kMsGroup + 255) / 256), 256, 0, s>>>(A, ncand);
-  msFinishKernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(A, dout);
+  if (!pen) {
+    if (ncand)
+      msLeafKernel<<<static_cast<unsigned>((ncand * kMsGroup + 255) / 256), 256, 0, s>>>(A, ncand);
+    msFinishKernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(A, dout);
+    MFCL_HIP_TRY(ctx, hipGetLastError());
+    MFCL_HIP_TRY(ctx, hipMemcpyAsync(out, dout, sizeof(mfcl_mesh_result) * n, hipMemcpyDeviceToHost, s));
+    if (tri_ids)
+      MFCL_HIP_TRY(ctx, hipMemcpyAsync(tri_ids, dtri, 4 * n * static_cast<uint64_t>(maxc), hipMemcpyDeviceToHost, s));
+    MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return MFCL_OK;
+  }
+  // penetration enabled: GJK + EPA per candidate, 4 wavefronts per block
+  if (ncand > 0xF0000000ull) {
+    ctx->err = "too many mesh-shape candidates for one penetration-enabled call";
+    return MFCL_E_INVALID;
+  }
+  MsPen E;
+  E.epa_max_faces = req->epa_max_faces;
+  E.epa_max_it = req->epa_max_iterations;
+  E.epa_tol = req->epa_tolerance;
+  int lds_faces = std::min(E.epa_max_faces, 64);
+  while (lds_faces > 4 && 4 * msSliceBytes(lds_faces) > static_cast<size_t>(ctx->max_lds_per_block)) lds_faces /= 2;
+  E.cap_faces = lds_faces;
+  E.slice_bytes = msSliceBytes(lds_faces);
+  const uint64_t blocks64 = std::min<uint64_t>((ncand + 3) / 4, 2ull * static_cast<uint64_t>(ctx->n_cu));
+  const unsigned blocks = static_cast<unsigned>(std::max<uint64_t>(blocks64, 1));
+  E.arena = nullptr;
+  E.arena_slice = arenaBytes(E.epa_max_faces);
+  if (lds_faces < E.epa_max_faces) {
+    E.arena = static_cast<char*>(m.get(E.arena_slice * 4 * blocks));
+    if (!E.arena) {
+      ctx->err = "out of device memory for the mesh-shape EPA polytopes";
+      return MFCL_E_NOMEM;
+    }
+  }
+  E.contacts = static_cast<mfcl_mesh_contact*>(m.get(sizeof(mfcl_mesh_contact) * n * static_cast<uint64_t>(maxc)));
+  E.next = reinterpret_cast<uint32_t*>(dcc) + 2;  // the candidate counter's third word
+  if (!E.contacts) {
+    ctx->err = "out of device memory for the mesh-shape contacts";
+    return MFCL_E_NOMEM;
+  }
+  msZeroKernel<<<1, 64, 0, s>>>(E.next, 1);
+  if (ncand) msPenKernel<<<blocks, 256, 4 * E.slice_bytes, s>>>(A, E, ncand);
+  msFinishPenKernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(A, E.contacts, dout);
   MFCL_HIP_TRY(ctx, hipGetLastError());
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(out, dout, sizeof(mfcl_mesh_result) * n, hipMemcpyDeviceToHost, s));
-  if (tri_ids)
-    MFCL_HIP_TRY(ctx, hipMemcpyAsync(tri_ids, dtri, 4 * n * static_cast<uint64_t>(maxc), hipMemcpyDeviceToHost, s));
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(contacts, E.contacts, sizeof(mfcl_mesh_contact) * n * static_cast<uint64_t>(maxc),
+                                   hipMemcpyDeviceToHost, s));
   MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
   return MFCL_OK;
 }

@@ -174,6 +174,7 @@ def load_library() -> C.CDLL:
     lib.mfcl_mesh_num_nodes.argtypes = [p, u32, C.POINTER(u32)]
     lib.mfcl_mesh_get_nodes.argtypes = [p, u32, p]
     lib.mfcl_mesh_shape_collide.argtypes = [p, p, p, p, p, u64, p, p, p]
+    lib.mfcl_mesh_shape_collide_contacts.argtypes = [p, p, p, p, p, u64, p, p, p]
     lib.mfcl_mesh_replace_vertices.argtypes = [p, u32, p, i32]
     lib.mfcl_mesh_replace_vertices_dev.argtypes = [p, u32, p, p]
     lib.mfcl_save_meshes.argtypes = [p, C.c_char_p]
@@ -550,6 +551,22 @@ class Context:
         self._check(self.lib.mfcl_mesh_shape_collide(self.h, _ptr(mesh_ids), _ptr(shape_ids), _ptr(pose_mesh),
                                                      _ptr(pose_shape), n, C.byref(req), _ptr(out), _ptr(tri)))
         return out, tri
+
+    def mesh_shape_collide_contacts(self, mesh_ids, shape_ids, pose_mesh, pose_shape, req: Request):
+        """Mesh-shape collision with penetration enabled (req.penetration
+        = 1): (results of MESH_RESULT_DTYPE, contacts n x max_contacts of
+        MESH_CONTACT_DTYPE, ascending b1, b1 = -1 past n_contacts)."""
+        mesh_ids = np.ascontiguousarray(mesh_ids, np.uint32)
+        shape_ids = np.ascontiguousarray(shape_ids, np.uint32)
+        pose_mesh = np.ascontiguousarray(pose_mesh, np.float32).reshape(-1, 7)
+        pose_shape = np.ascontiguousarray(pose_shape, np.float32).reshape(-1, 7)
+        n = len(mesh_ids)
+        out = np.zeros(n, MESH_RESULT_DTYPE)
+        cont = np.zeros((n, max(1, req.max_contacts)), MESH_CONTACT_DTYPE)
+        self._check(self.lib.mfcl_mesh_shape_collide_contacts(self.h, _ptr(mesh_ids), _ptr(shape_ids),
+                                                              _ptr(pose_mesh), _ptr(pose_shape), n, C.byref(req),
+                                                              _ptr(out), _ptr(cont)))
+        return out, cont
 
     def mesh_replace_vertices(self, mesh: int, verts, refit: bool = True):
         verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)

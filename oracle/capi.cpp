@@ -461,6 +461,59 @@ int64_t orc_mesh_collide_all_pen(void* m, int ia, int ib, const double* tfa12, c
   return static_cast<int64_t>(o.pen.size());
 }
 
+// Penetration-enabled mesh-shape collide, every colliding triangle of one
+// query in ascending order (the set does not depend on the tree): b1 = the
+// triangle, b2 = -1 (Contact::NONE), the leaf contact with its normal
+// reversed (ContactMeta::reverse_normal, bvh_solver-inl.h:55-65,
+// shape_pair_intersect-inl.h:19-26).  Up to cap records; returns the count.
+int64_t orc_mesh_shape_contacts_all(void* ms, void* hs, const int* type, const double* params, const int* hull_idx,
+                                    int mesh, int shape, const double* tfm12, const double* tfs12,
+                                    const double* sp6, OrcMeshContact* out, int64_t cap) {
+  const Mesh& m = static_cast<MeshSet*>(ms)->meshes[mesh];
+  ShapeTable st{static_cast<HullSet*>(hs), type, params, hull_idx};
+  const SolverParams sp = solverFrom(sp6);
+  const Shape sh = st.get(shape);
+  const Tf tm = tfFrom12(tfm12), ts = tfFrom12(tfs12);
+  int64_t n = 0;
+  const int nt = static_cast<int>(m.tris.size() / 3);
+  for (int t = 0; t < nt; ++t) {
+    const V3 tri[3] = {m.verts[m.tris[3 * t]], m.verts[m.tris[3 * t + 1]], m.verts[m.tris[3 * t + 2]]};
+    ContactPt c;
+    if (!shapeTriangleContact(sh, ts, tri, tm, sp, c)) continue;
+    if (n < cap) {
+      MeshContact mc;
+      mc.b1 = t;
+      mc.b2 = -1;
+      mc.pos = c.pos;
+      mc.normal = -c.normal;
+      mc.depth = c.depth;
+      putContact(out[n], mc);
+    }
+    ++n;
+  }
+  return n;
+}
+
+// One leaf test with penetration (for pins): returns 1 on a hit and writes
+// pos (3), normal (3, before the traversal's reversal), depth, GJK and EPA
+// status into r9.
+int orc_shape_triangle_contact(void* hs, const int* type, const double* params, const int* hull_idx, int shape,
+                               const double* tfs12, const double* P9, const double* tfm12, const double* sp6,
+                               double* r9) {
+  ShapeTable st{static_cast<HullSet*>(hs), type, params, hull_idx};
+  const SolverParams sp = solverFrom(sp6);
+  const V3 tri[3] = {V3(P9[0], P9[1], P9[2]), V3(P9[3], P9[4], P9[5]), V3(P9[6], P9[7], P9[8])};
+  ContactPt c;
+  int gs = -1, es = -1;
+  const bool hit = shapeTriangleContact(st.get(shape), tfFrom12(tfs12), tri, tfFrom12(tfm12), sp, c, &gs, &es);
+  r9[0] = c.pos.x; r9[1] = c.pos.y; r9[2] = c.pos.z;
+  r9[3] = c.normal.x; r9[4] = c.normal.y; r9[5] = c.normal.z;
+  r9[6] = c.depth;
+  r9[7] = gs;
+  r9[8] = es;
+  return hit ? 1 : 0;
+}
+
 // One triangle pair (Q given in its own frame, mapped by R, T):
 // intersect-inl.h:594-613 with contacts.  Returns 1 when they intersect.
 int orc_tri_tri_contacts(const double* P9, const double* Q9, const double* R9, const double* T3, double* pts6,

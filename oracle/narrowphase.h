@@ -178,6 +178,15 @@ struct CollideOut {
 };
 void collide(const Shape& a, const Tf& tfa, const Shape& b, const Tf& tfb, int max_contacts,
              bool penetration, const SolverParams& sp, CollideOut& out);
+// Mesh-shape leaf tests with penetration enabled: sphereTriangleIntersect's
+// contact (sphere_triangle-inl.h:107-187, triangle in the world) and the
+// generic GJK + EPA contact on (shape, TriangleP) (gjk_solver-inl.h:452-527,
+// 545-563).  The contact is before the traversal's normal reversal.
+bool sphereTriangleContact(double r, const Tf& tf_sphere, const V3& P1, const V3& P2, const V3& P3,
+                           ContactPt* c);
+bool shapeTriangleContact(const Shape& s, const Tf& tf_shape, const V3 tri[3], const Tf& tf_tri,
+                          const SolverParams& sp, ContactPt& c, int* gjk_status = nullptr,
+                          int* epa_status = nullptr);
 
 // shapeSignedDistance semantics (gjk_solver-inl.h:820-874): returns false on
 // failure.  distance > 0 separated (witnesses in world), < 0 = -depth.

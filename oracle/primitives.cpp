@@ -533,12 +533,17 @@ void minMax3(double a, double b, double c, double& mn, double& mx) {
 
 }  // namespace
 
-bool sphereTriangleHit(double radius, const Tf& tf_sphere, const V3& P1, const V3& P2, const V3& P3) {
+// sphere_triangle-inl.h:107-187.  c (optional) receives the contact as the
+// reference writes it: pos = the nearest point found, normal = that point
+// minus the centre, normalised (the flipped plane normal when they
+// coincide), depth = -(radius - distance) (-radius when they coincide).
+bool sphereTriangleContact(double radius, const Tf& tf_sphere, const V3& P1, const V3& P2, const V3& P3,
+                           ContactPt* c) {
   V3 n = cross(P2 - P1, P3 - P1);
   normalize(n);
-  const V3 c = tf_sphere.t;
+  const V3 ctr = tf_sphere.t;
   const double rt = radius + std::numeric_limits<double>::epsilon();
-  double dist = dot(c - P1, n);
+  double dist = dot(ctr - P1, n);
   if (dist < 0) {
     dist *= -1;
     n *= -1;
@@ -546,19 +551,37 @@ bool sphereTriangleHit(double radius, const Tf& tf_sphere, const V3& P1, const V
   if (!(dist < rt)) return false;
   bool has = false;
   V3 cp;
-  if (projectInTriangle(P1, P2, P3, n, c)) {
+  if (projectInTriangle(P1, P2, P3, n, ctr)) {
     has = true;
-    cp = c - n * dist;
+    cp = ctr - n * dist;
   } else {
     const double cr2 = rt * rt;
     V3 q;
-    if (segmentSqrDistance(P1, P2, c, q) < cr2) { has = true; cp = q; }
-    if (segmentSqrDistance(P2, P3, c, q) < cr2) { has = true; cp = q; }
-    if (segmentSqrDistance(P3, P1, c, q) < cr2) { has = true; cp = q; }
+    if (segmentSqrDistance(P1, P2, ctr, q) < cr2) { has = true; cp = q; }
+    if (segmentSqrDistance(P2, P3, ctr, q) < cr2) { has = true; cp = q; }
+    if (segmentSqrDistance(P3, P1, ctr, q) < cr2) { has = true; cp = q; }
   }
   if (!has) return false;
-  const V3 d = cp - c;
-  return dot(d, d) < rt * rt;
+  const V3 d = cp - ctr;
+  const double d2 = dot(d, d);
+  if (!(d2 < rt * rt)) return false;
+  if (c) {
+    if (d2 > 0) {
+      const double l = std::sqrt(d2);
+      c->normal = normalized(d);
+      c->pos = cp;
+      c->depth = -(radius - l);
+    } else {
+      c->normal = -n;
+      c->pos = cp;
+      c->depth = -radius;
+    }
+  }
+  return true;
+}
+
+bool sphereTriangleHit(double radius, const Tf& tf_sphere, const V3& P1, const V3& P2, const V3& P3) {
+  return sphereTriangleContact(radius, tf_sphere, P1, P2, P3, nullptr);
 }
 
 bool boxTriangleHit(const double side[3], const Tf& tf_box, const V3& P1, const V3& P2, const V3& P3,
@@ -605,6 +628,47 @@ bool shapeTriangleHit(const Shape& s, const Tf& tf_shape, const V3 tri[3], const
   t.tri[2] = tri[2];
   const MinkDiff md = makeMinkDiff(s, tf_shape, t, tf_tri);
   return mprIntersect(md, max_iter, tol) == MPRStatus::Intersect;
+}
+
+// The leaf test with penetration enabled (GJKSolver::shapeTriangleIntersect
+// with contact outputs, gjk_solver-inl.h:545-563): the sphere's closed form
+// with its contact, else GJK from -guess (guess = UnitX) and EPA on
+// (shape, TriangleP) (shapeTransformedTriangleIntersectIndepImplRun,
+// gjk_solver-inl.h:452-527: boxes too -- their closed form is boolean only):
+// a hit when GJK intersects and EPA does not fail; pos = tf_shape * the
+// witness midpoint, normal = tf_shape.R * normalize(p0 - p1) (UnitZ when
+// zero), depth = the EPA depth.  The contact is before the mesh-shape
+// traversal's normal reversal.
+bool shapeTriangleContact(const Shape& s, const Tf& tf_shape, const V3 tri[3], const Tf& tf_tri,
+                          const SolverParams& sp, ContactPt& c, int* gjk_status, int* epa_status) {
+  if (gjk_status) *gjk_status = -1;
+  if (epa_status) *epa_status = -1;
+  if (s.type == kSphere)
+    return sphereTriangleContact(s.p[0], tf_shape, tf_tri * tri[0], tf_tri * tri[1], tf_tri * tri[2], &c);
+  Shape t;
+  t.type = kTriangle;
+  t.tri[0] = tri[0];
+  t.tri[1] = tri[1];
+  t.tri[2] = tri[2];
+  const MinkDiff md = makeMinkDiff(s, tf_shape, t, tf_tri);
+  const V3 guess = sp.guess_valid ? sp.guess : V3(1, 0, 0);
+  Simplex smp;
+  const GJKStatus g = gjkEvaluate(md, smp, -guess, sp.gjk_tolerance, sp.gjk_max_iterations, nullptr);
+  if (gjk_status) *gjk_status = static_cast<int>(g);
+  if (g != GJKStatus::Intersect) return false;
+  double depth = 0;
+  V3 p0, p1;
+  const EPAStatus e = epaEvaluate(smp, md, sp.epa_tolerance, sp.epa_max_face_num, sp.epa_max_iterations, &depth,
+                                  &p0, &p1);
+  if (epa_status) *epa_status = static_cast<int>(e);
+  if (e == EPAStatus::Failed) return false;
+  V3 n1 = p0 - p1;
+  if (sqnorm(n1) <= 0) n1 = V3(0, 0, 1);
+  else normalize(n1);
+  c.pos = tf_shape * (0.5 * (p0 + p1));
+  c.normal = tf_shape.R * n1;
+  c.depth = depth;
+  return true;
 }
 
 }  // namespace oracle

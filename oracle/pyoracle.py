@@ -63,6 +63,10 @@ def lib() -> C.CDLL:
         L.orc_mesh_shape_batch.restype = d
         L.orc_shape_triangle_hit.argtypes = [p, p, p, p, i32, p, p, p, p]
         L.orc_shape_triangle_hit.restype = i32
+        L.orc_mesh_shape_contacts_all.argtypes = [p, p, p, p, p, i32, i32, p, p, p, p, i64]
+        L.orc_mesh_shape_contacts_all.restype = i64
+        L.orc_shape_triangle_contact.argtypes = [p, p, p, p, i32, p, p, p, p, p]
+        L.orc_shape_triangle_contact.restype = i32
         L.orc_signed_distance_batch.argtypes = [p, p, p, p, p, p, p, p, i64, p, p, i32, p]
         L.orc_signed_distance_batch.restype = d
         L.orc_pair_algorithm.argtypes = [p, p, p, p, i32, i32, p, p, i32, i32, d, p, p]
@@ -244,6 +248,29 @@ class ShapeSet:
         n = lib().orc_mesh_shape_all(M.h, self.hs, _p(self.types), _p(self.params), _p(self.hull_idx), int(mesh),
                                      int(shape), _p(np.ascontiguousarray(tfm12, np.float64)),
                                      _p(np.ascontiguousarray(tfs12, np.float64)), _p(sp), _p(out), cap)
+        return out[:min(n, cap)], int(n)
+
+    def triangle_contact(self, shape, tfs12, tri9, tfm12, solver=None):
+        """Leaf test of mesh-shape collision with penetration enabled: (hit,
+        pos, normal before the traversal's reversal, depth, GJK status, EPA
+        status)."""
+        sp = None if solver is None else np.ascontiguousarray(solver, np.float64)
+        r = np.zeros(9)
+        hit = lib().orc_shape_triangle_contact(self.hs, _p(self.types), _p(self.params), _p(self.hull_idx),
+                                               int(shape), _p(np.ascontiguousarray(tfs12, np.float64)),
+                                               _p(np.ascontiguousarray(tri9, np.float64)),
+                                               _p(np.ascontiguousarray(tfm12, np.float64)), _p(sp), _p(r))
+        return bool(hit), r[0:3].copy(), r[3:6].copy(), float(r[6]), int(r[7]), int(r[8])
+
+    def mesh_shape_contacts_all(self, M: "MeshSet", mesh, shape, tfm12, tfs12, solver=None, cap=1 << 16):
+        """Every contact of one penetration-enabled mesh-shape query
+        (MESH_CONTACT_REC, ascending triangle id, normals reversed as the
+        traversal writes them) and the total count."""
+        sp = None if solver is None else np.ascontiguousarray(solver, np.float64)
+        out = np.zeros(cap, MESH_CONTACT_REC)
+        n = lib().orc_mesh_shape_contacts_all(M.h, self.hs, _p(self.types), _p(self.params), _p(self.hull_idx),
+                                              int(mesh), int(shape), _p(np.ascontiguousarray(tfm12, np.float64)),
+                                              _p(np.ascontiguousarray(tfs12, np.float64)), _p(sp), _p(out), cap)
         return out[:min(n, cap)], int(n)
 
     def signed_distance(self, pa, pb, tfa12, tfb12, solver=None, guess=None, threads=1):

@@ -1,11 +1,13 @@
-"""GPU parity: mesh-shape collision (mfcl_mesh_shape_collide;
+"""GPU parity: mesh-shape collision (mfcl_mesh_shape_collide and, with
+penetration enabled, mfcl_mesh_shape_collide_contacts;
 OrientedNodeBVHSolver::MeshShapeIntersect, bvh_solver-inl.h:6-72, leaf
 tests as GJKSolver::shapeTriangleIntersect, gjk_solver-inl.h:545-563)
 against the oracle, which tests every triangle (the colliding set does not
 depend on the tree).  Meshes of 300-3,000 triangles against boxes, spheres,
 capsules, cylinders and convex hulls (walking and scanning) near their
 surfaces: every boolean equal, every complete colliding set equal when it
-fits the contact cap, no stack overflow."""
+fits the contact cap, no stack overflow; with penetration, every contact
+(triangle, position, normal, depth) bit-identical."""
 import numpy as np
 import pytest
 
@@ -51,10 +53,9 @@ def _shapes(rng, k):
             np.concatenate(ht), to, tc, oracle_hulls)
 
 
-@pytest.mark.gpu
-def test_mesh_shape_bit_exact_sets(oracle, gpu_ctx):
-    rng = np.random.default_rng(90)
-    pool = W.mesh_pool(8, 91, 300, 3000)
+def _setup(oracle, gpu_ctx, seed, n):
+    rng = np.random.default_rng(seed)
+    pool = W.mesh_pool(8, seed + 1, 300, 3000)
     types, params, vo, vc, verts, tris, to, tc, oh = _shapes(rng, 16)
     ns = len(types)
     # the oracle's hull table holds the hulls only, mapped from their shapes
@@ -64,7 +65,6 @@ def test_mesh_shape_bit_exact_sets(oracle, gpu_ctx):
     gpu_ctx.upload_shapes(types, params, vo, vc, verts, faces=mfcl.faces_from_triangles(tris, to, tc))
     gpu_ctx.upload_meshes(pool["verts"], pool["vert_off"], pool["vert_cnt"], pool["tris"], pool["tri_off"],
                           pool["tri_cnt"])
-    n = 2048
     mesh_ids = rng.integers(0, len(pool["vert_cnt"]), n).astype(np.uint32)
     shape_ids = rng.integers(0, ns, n).astype(np.uint32)
     pm = W.poses(rng, n, -1.0, 1.0)
@@ -73,6 +73,13 @@ def test_mesh_shape_bit_exact_sets(oracle, gpu_ctx):
     u /= np.linalg.norm(u, axis=1, keepdims=True)
     r = pool["radius"][mesh_ids] * rng.uniform(0.4, 1.8, n)
     ps[:, :3] = (pm[:, :3] + u * r[:, None]).astype(np.float32)
+    return S, M, types, mesh_ids, shape_ids, pm, ps
+
+
+@pytest.mark.gpu
+def test_mesh_shape_bit_exact_sets(oracle, gpu_ctx):
+    n = 2048
+    S, M, types, mesh_ids, shape_ids, pm, ps = _setup(oracle, gpu_ctx, 90, n)
     cap = 64
     req = mfcl.request(mfcl.MODE_COLLIDE, max_contacts=cap, penetration=0)
     out, got = gpu_ctx.mesh_shape_collide(mesh_ids, shape_ids, pm, ps, req)
@@ -106,3 +113,42 @@ def test_mesh_shape_rejects_penetration(gpu_ctx):
         gpu_ctx.mesh_shape_collide([0], [0], W.poses(np.random.default_rng(0), 1, 0, 0),
                                    W.poses(np.random.default_rng(1), 1, 0, 0),
                                    mfcl.request(mfcl.MODE_COLLIDE, max_contacts=4, penetration=1))
+
+
+@pytest.mark.gpu
+def test_mesh_shape_contacts_bit_exact(oracle, gpu_ctx):
+    """Penetration enabled: every query's hit flag and contact count equal
+    the oracle's; when the colliding triangles fit the cap, every contact
+    (b1, b2 = -1, pos, reversed normal, depth) is bit-identical to the
+    oracle's (spheres by the closed form, every other shape by GJK + EPA with
+    the triangle as a shape)."""
+    n = 768
+    S, M, types, mesh_ids, shape_ids, pm, ps = _setup(oracle, gpu_ctx, 94, n)
+    cap = 128
+    req = mfcl.request(mfcl.MODE_COLLIDE, max_contacts=cap, penetration=1)
+    out, got = gpu_ctx.mesh_shape_collide_contacts(mesh_ids, shape_ids, pm, ps, req)
+    tm, ts = oracle.pose12(pm), oracle.pose12(ps)
+    assert np.all(out["hit"] >= 0), "traversal stack overflow"
+    n_eq = n_hit = 0
+    per_type = {}
+    for i in range(n):
+        ref, cnt = S.mesh_shape_contacts_all(M, int(mesh_ids[i]), int(shape_ids[i]), tm[i], ts[i])
+        assert out["hit"][i] == (1 if cnt else 0), i
+        assert out["n_contacts"][i] == min(cnt, cap), i
+        if cnt == 0:
+            continue
+        n_hit += 1
+        t = int(types[shape_ids[i]])
+        per_type[t] = per_type.get(t, 0) + 1
+        if cnt > cap:
+            continue
+        g = got[i, :cnt]
+        assert np.array_equal(g["b1"], ref["b1"]) and np.all(g["b2"] == -1), i
+        for f in ("pos", "normal", "depth"):
+            assert np.array_equal(g[f], ref[f]), (i, f)
+        assert np.all(got[i, cnt:]["b1"] == -1)
+        n_eq += 1
+    print(f"\n[mesh-shape contacts] queries={n} colliding={n_hit} per shape type {per_type}; "
+          f"contacts bit-identical on {n_eq}")
+    assert all(per_type.get(t, 0) > 10 for t in (BOX, SPHERE, CAPSULE, CYLINDER, CONVEX))
+    assert n_eq > 300

@@ -203,3 +203,128 @@ def test_capsule_triangle_mpr(oracle):
         n_hit += got
         n += 1
     assert n_hit > 200 and n - n_hit > 200
+
+
+# ---------------------------------------------------------------------------
+# Penetration enabled (GJKSolver::shapeTriangleIntersect with contact
+# outputs): the sphere's closed-form contact, and GJK + EPA on every other
+# shape, pinned by what the geometry fixes.
+
+def test_sphere_triangle_contact(oracle):
+    """Sphere contact (sphere_triangle-inl.h:107-187): a hit when the
+    exact centre-to-triangle distance is below the radius; pos is a point of
+    the triangle within the radius of the centre; normal = (pos - centre)
+    normalised and depth = -(r - |pos - centre|); when the centre projects
+    inside the triangle, pos is the exact closest point.  A centre lying on
+    the triangle gives the flipped plane normal and depth -r."""
+    rng = np.random.default_rng(84)
+    r = 0.25
+    S = oracle.ShapeSet(np.array([SPHERE], np.int32), np.array([[r, 0, 0]]))
+    n_hit = n_in = 0
+    for Rs, ts, Rm, tm, tri in _cases(rng, 3000):
+        P = tri @ Rm.T + tm
+        q = _closest_on_triangle(ts, *P)
+        d = np.linalg.norm(q - ts)
+        if abs(d - r) < 1e-9:
+            continue
+        hit, pos, nrm, depth, _, _ = S.triangle_contact(0, _tf(Rs, ts), tri.reshape(9), _tf(Rm, tm))
+        assert hit == (d < r)
+        if not hit:
+            continue
+        n_hit += 1
+        assert np.linalg.norm(_closest_on_triangle(pos, *P) - pos) < 1e-12
+        dist = np.linalg.norm(pos - ts)
+        assert dist < r + 1e-12
+        assert abs(depth + (r - dist)) < 1e-12
+        assert np.allclose(nrm, (pos - ts) / dist, atol=1e-12)
+        n = np.cross(P[1] - P[0], P[2] - P[0])
+        n /= np.linalg.norm(n)
+        proj = ts - n * (n @ (ts - P[0]))
+        if np.linalg.norm(_closest_on_triangle(proj, *P) - proj) < 1e-13:
+            n_in += 1
+            assert np.linalg.norm(pos - q) < 1e-12
+    assert n_hit > 300 and n_in > 100
+    tri0 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64)
+    eye = np.eye(3)
+    hit, pos, nrm, depth, _, _ = S.triangle_contact(0, _tf(eye, np.array([0.25, 0.25, 0.0])), tri0.reshape(9),
+                                                    _tf(eye, np.zeros(3)))
+    assert hit and depth == -r and np.array_equal(nrm, [0, 0, -1.0]) and np.array_equal(pos, [0.25, 0.25, 0.0])
+
+
+def _mtd(A, B, axes):
+    """Minimum overlap of point sets A and B over the unit axes (the
+    penetration depth of their hulls when the axes are all facet normals of
+    the Minkowski difference) and the axis that attains it."""
+    best, arg = np.inf, None
+    for a in axes:
+        l = np.linalg.norm(a)
+        if l < 1e-9:
+            continue
+        a = a / l
+        pa, pb = A @ a, B @ a
+        o = min(pa.max() - pb.min(), pb.max() - pa.min())
+        if o < best:
+            best, arg = o, a
+    return best, arg
+
+
+def _overlap_along(A, B, a):
+    pa, pb = A @ a, B @ a
+    return min(pa.max() - pb.min(), pb.max() - pa.min())
+
+
+def _check_epa_contact(S, shape, Rs, ts, Rm, tm, tri, A, faces_n, edges):
+    """EPA contact of shape (world vertices A, face normals, edge vectors)
+    against the triangle: depth equals the minimum overlap over the facet
+    normals of the Minkowski difference (shape faces, the triangle's normal,
+    edge x edge), the contact normal is a direction of that overlap, pos is
+    the witness midpoint between the two sets.  Returns whether it hit."""
+    P = tri @ Rm.T + tm
+    hit, pos, nrm, depth, gs, es = S.triangle_contact(shape, _tf(Rs, ts), tri.reshape(9), _tf(Rm, tm))
+    if not hit:
+        return False
+    te = [P[1] - P[0], P[2] - P[1], P[0] - P[2]]
+    axes = list(faces_n) + [np.cross(te[0], te[1])] + [np.cross(e, f) for e in edges for f in te]
+    mtd, _ = _mtd(A, P, axes)
+    scale = max(np.abs(A).max(), np.abs(P).max())
+    assert mtd > -1e-9
+    assert abs(depth - mtd) <= 1e-5 * scale + 1e-9, (depth, mtd)
+    assert abs(np.linalg.norm(nrm) - 1) < 1e-12
+    assert abs(_overlap_along(A, P, nrm) - depth) <= 1e-5 * scale + 1e-9
+    return True
+
+
+def test_box_triangle_contact_depth(oracle):
+    rng = np.random.default_rng(85)
+    side = np.array([0.5, 0.3, 0.2])
+    S = oracle.ShapeSet(np.array([BOX], np.int32), side[None])
+    corners = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]) * side / 2
+    n_hit = 0
+    for Rs, ts, Rm, tm, tri in _cases(rng, 1500):
+        A = corners @ Rs.T + ts
+        n_hit += _check_epa_contact(S, 0, Rs, ts, Rm, tm, tri, A, list(Rs.T), list(Rs.T))
+    assert n_hit > 200
+
+
+def test_hull_triangle_contact_depth(oracle):
+    rng = np.random.default_rng(86)
+    v, off, cnt, _ = W.ellipsoid_hulls(rng, 6, 12, 48)
+    hv, ho, hc, ht, hto, htc = [], [], [], [], [], []
+    o = q = 0
+    for i in range(6):
+        Pp = v[off[i]:off[i] + cnt[i]]
+        keep, t = W.hull_faces(Pp)
+        hv.append(Pp[keep]); ho.append(o); hc.append(len(keep)); o += len(keep)
+        ht.append(t); hto.append(q); htc.append(len(t)); q += len(t)
+    S = oracle.ShapeSet(np.full(6, CONVEX, np.int32), np.zeros((6, 3)), np.concatenate(hv), ho, hc,
+                        np.concatenate(ht), hto, htc)
+    n_hit = 0
+    for k, (Rs, ts, Rm, tm, tri) in enumerate(_cases(rng, 600)):
+        h = k % 6
+        Vw = hv[h].astype(np.float64) @ Rs.T + ts
+        F = ht[h]
+        fn = [np.cross(Vw[f[1]] - Vw[f[0]], Vw[f[2]] - Vw[f[0]]) for f in F]
+        ed = {tuple(sorted((int(f[i]), int(f[(i + 1) % 3])))) for f in F for i in range(3)}
+        edges = [Vw[b] - Vw[a] for a, b in ed]
+        n_hit += _check_epa_contact(S, h, Rs, ts, Rm, tm, tri, Vw, fn, edges)
+    assert n_hit > 60
