@@ -160,6 +160,8 @@ int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n);
                                          the default guess where the MLP predicts penetration */
 #define MFCL_TUNE_EPA_SPILL 7         /* 1 (default): a pair that outgrows the LDS polytope continues in a
                                          device-memory polytope of full capacity; 0: it reruns afterwards */
+#define MFCL_TUNE_MLP_WAVES 8         /* wavefronts per block of the fused warm-start inference (4, 8 or 16),
+                                         sharing one LDS copy of the weights; one block per CU */
 int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value);
 
 /* Workspace sizing (no counterpart in the reference, whose calls allocate

@@ -109,6 +109,7 @@ struct mfcl_ctx {
     int epa_overflow_waves{512};   // persistent wavefronts (one per block) for overflow pairs
     int mesh_blocks_per_cu{3};  // persistent mesh-traversal blocks (4 waves, one query each) per CU
     int mesh_stack{16384};      // node-pair stack entries per wavefront
+    int mlp_waves{8};           // wavefronts per block of the fused warm-start inference
   } tune;
   std::string err;
   hipStream_t stream{nullptr};

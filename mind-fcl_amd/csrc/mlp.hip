@@ -344,7 +344,11 @@ __device__ __forceinline__ void denseGelu(const MFCL_LDS bf16x8* W, const bf16x8
   }
 }
 
-__global__ void __launch_bounds__(kBlk) mlpGuessFused(const uint32_t* pa, const uint32_t* pb, const float* posea,
+// BLK threads per block share the block's LDS copy of the weights (one
+// block per CU): more wavefronts per block hide the feature loads and the
+// MFMA chains behind each other, within BLK's register budget.
+template <int BLK>
+__global__ void __launch_bounds__(BLK) mlpGuessFused(const uint32_t* pa, const uint32_t* pb, const float* posea,
                                                       const float* poseb, const float* desc, uint32_t n_shapes,
                                                       uint64_t n, const uint4* pack_g, const float* bias_g, int policy,
                                                       float* guess3, float* dist) {
@@ -676,12 +680,23 @@ int mlpGuess(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, const float*
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[8], s));
   const size_t lds = kPackBytes + kPackBiasFloats * 4;
   const uint64_t ntiles = (n + 31) / 32;
-  const uint64_t want = (ntiles + 3) / 4;
-  const unsigned grid = static_cast<unsigned>(want < 1024 ? want : 1024);  // persistent: 4 waves per block
-  mlpGuessFused<<<grid, kBlk, lds, s>>>(pa, pb, posea, poseb, ctx->d_desc, ctx->n_shapes, n,
-                                        static_cast<const uint4*>(ctx->mlp.pack),
-                                        reinterpret_cast<const float*>(static_cast<char*>(ctx->mlp.pack) + kPackBytes),
-                                        ctx->tune.guess_policy, guess3, dist);
+  const int waves = ctx->tune.mlp_waves;
+  // persistent: at most one block per CU (the weights fill most of its LDS)
+  const uint64_t want = (ntiles + waves - 1) / waves;
+  const uint64_t cap = static_cast<uint64_t>(ctx->n_cu);
+  const unsigned grid = static_cast<unsigned>(want < cap ? want : cap);
+  const uint4* pk = static_cast<const uint4*>(ctx->mlp.pack);
+  const float* pb4 = reinterpret_cast<const float*>(static_cast<char*>(ctx->mlp.pack) + kPackBytes);
+  const int pol = ctx->tune.guess_policy;
+  if (waves == 16)
+    mlpGuessFused<1024><<<grid, 1024, lds, s>>>(pa, pb, posea, poseb, ctx->d_desc, ctx->n_shapes, n, pk, pb4, pol,
+                                                guess3, dist);
+  else if (waves == 8)
+    mlpGuessFused<512><<<grid, 512, lds, s>>>(pa, pb, posea, poseb, ctx->d_desc, ctx->n_shapes, n, pk, pb4, pol,
+                                              guess3, dist);
+  else
+    mlpGuessFused<256><<<grid, 256, lds, s>>>(pa, pb, posea, poseb, ctx->d_desc, ctx->n_shapes, n, pk, pb4, pol,
+                                              guess3, dist);
   if (ctx->timing) MFCL_HIP_TRY(ctx, hipEventRecord(ctx->ev[9], s));
   MFCL_HIP_TRY(ctx, hipGetLastError());
   return MFCL_OK;

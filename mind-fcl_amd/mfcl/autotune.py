@@ -22,6 +22,7 @@ MESH_STACK = 4
 EPA_LDS_FACES = 5
 GUESS_POLICY = 6
 EPA_SPILL = 7
+MLP_WAVES = 8
 
 
 def set_knob(ctx, knob: int, value: int) -> None:
