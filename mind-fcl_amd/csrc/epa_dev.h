@@ -302,13 +302,18 @@ struct PolyT {
       const int ok = __shfl_xor(bk, off), os = __shfl_xor(bs, off), oi = __shfl_xor(bi, off);
       if (better(od, ok, os, bd, bk, bs)) { bd = od; bk = ok; bs = os; bi = oi; }
     }
+    // identical in every lane after the butterfly: taken as scalars, so
+    // the callers branch on them with scalar branches
+    const int ubk = __builtin_amdgcn_readfirstlane(bk);
+    const int ubi = __builtin_amdgcn_readfirstlane(bi);
+    const int fin = __builtin_amdgcn_readfirstlane(bd < INFINITY ? 1 : 0);
     // only finite candidates beat the reference's initial INFINITY
-    if (bk == 4 || !(bd < INFINITY)) {
+    if (ubk == 4 || !fin) {
       kind = 0;
       idx = kNil;
     } else {
-      kind = bk;
-      idx = static_cast<int16_t>(bi);
+      kind = ubk;
+      idx = static_cast<int16_t>(ubi);
     }
   }
   // epa_polytope.hpp:347-405
@@ -1073,7 +1078,9 @@ MFCL_D void epaLoop(PT& P, MK& md, double tol, int max_iter, EpaOut& o, MFCL_LDS
       nv = d3{0, 0, 0};
       start = kNil;
       nh0 = nh1 = -1;
-      const int ds = nextDir(P, md, kind, fi, tol, nd, nv, start, nh0, nh1);
+      // statuses and the start face are the same in every lane: scalars
+      const int ds = __builtin_amdgcn_readfirstlane(nextDir(P, md, kind, fi, tol, nd, nv, start, nh0, nh1));
+      start = static_cast<int16_t>(__builtin_amdgcn_readfirstlane(start));
       if (ds == 1) { o.status = kEpaFailed; return; }
       if (ds == 2) {
         assignPair(md, nd, nh0, nh1, raw, o);
@@ -1090,7 +1097,7 @@ MFCL_D void epaLoop(PT& P, MK& md, double tol, int max_iter, EpaOut& o, MFCL_LDS
       }
     }
     pending = false;
-    const int ex = P.expand(nv, nd, nh0, nh1, start, SPILL && st.base != nullptr);
+    const int ex = __builtin_amdgcn_readfirstlane(P.expand(nv, nd, nh0, nh1, start, SPILL && st.base != nullptr));
     if constexpr (SPILL && std::is_same<PT, Poly>::value) {
       if (ex == 3) {
         o = epaContinue(P, md, tol, max_iter, o, raw, it, nv, nd, nh0, nh1, start, st);
