@@ -15,6 +15,11 @@ namespace mfcl {
 
 constexpr int16_t kNil = -1;
 
+// A condition that every lane of the wavefront evaluates on identical
+// values (the polytope code runs one pair per wavefront in lock step), taken
+// as a scalar so that the branch on it is a scalar branch.
+MFCL_D bool ub(bool c) { return __builtin_amdgcn_readfirstlane(c ? 1 : 0) != 0; }
+
 struct EVert {
   double vx, vy, vz;  // vertex on the Minkowski difference
   double dx, dy, dz;  // direction that produced it
@@ -246,10 +251,10 @@ struct PolyT {
     const int16_t es[3] = {e1, e2, e3};
     for (int k = 0; k < 3; ++k) {
       auto& ek = E[es[k]];
-      if (ek.f0 == kNil) {
+      if (ub(ek.f0 == kNil)) {
         ek.f0 = i;
       } else {
-        if (ek.f1 != kNil) return kNil;
+        if (ub(ek.f1 != kNil)) return kNil;
         ek.f1 = i;
       }
       M::sync();
@@ -324,10 +329,10 @@ struct PolyT {
     const d3 n = cross(a - b, b - c);
     const double nn = norm(n);
     if (area) *area = 0.5 * nn;
-    if (nn <= 0.0) return false;
+    if (ub(nn <= 0.0)) return false;
     const d3 dir = n / nn;
     const double od = dot(a, dir);
-    if (fabs(od) >= 1e-4) {
+    if (ub(fabs(od) >= 1e-4)) {
       normal = od > 0 ? dir : -dir;
       return true;
     }
@@ -353,8 +358,8 @@ struct PolyT {
   MFCL_D bool outside(int16_t fi, d3 p) const {
     d3 n;
     double area = 0;
-    if (!faceNormalOut(fi, n, &area)) return area <= 0.0;
-    return dot(n, p - faceAnchor(fi)) >= 0.0;
+    if (!faceNormalOut(fi, n, &area)) return ub(area <= 0.0);
+    return ub(dot(n, p - faceAnchor(fi)) >= 0.0);
   }
   MFCL_D bool anyLane(bool b) const { return __ballot(b) != 0ull; }
 
@@ -520,6 +525,7 @@ struct PolyT {
     int free_e = 0, free_f = 0;
     for (int base = 0; base < nv; base += 64) free_e += __popcll(__ballot(base + l < nv && E[base + l].stamp < 0));
     for (int base = 0; base < nf; base += 64) free_f += __popcll(__ballot(base + l < nf && F[base + l].stamp < 0));
+    new_edges = __builtin_amdgcn_readfirstlane(new_edges);
     if (new_edges > free_e || nb > free_f) return 2;
     // the free slot lists (before any slot is taken)
     {
@@ -901,22 +907,22 @@ MFCL_D int nextDir(const PT& P, MK& md, int kind, int16_t fi, double tol, d3& nd
     if (!P.faceNormalOut(f, fn, nullptr)) return 1;
     const d3 on = P.faceAnchor(f);
     const auto& F = P.F[f];
-    if (try_w && F.d2 > kOuter) {
+    if (try_w && ub(F.d2 > kOuter)) {
       nd = normalized(d3{F.wx, F.wy, F.wz});
       nv = md.supportFrom(nd, -1, -1, nh0, nh1);
-      if (dot(fn, nv - on) >= 0.0) { start = f; return 0; }
+      if (ub(dot(fn, nv - on) >= 0.0)) { start = f; return 0; }
     }
     nd = fn;
     nv = md.supportFrom(nd, -1, -1, nh0, nh1);
-    if (dot(fn, nv - on) > tol) { start = f; return 0; }
+    if (ub(dot(fn, nv - on) > tol)) { start = f; return 0; }
     return 2;
   };
   if (kind == 3) return face_cand(fi, true);
   if (kind == 2) {
     const auto& e = P.E[fi];
     const int16_t f0 = e.f0, f1 = e.f1;
-    if (f0 == kNil || f1 == kNil) return 1;  // open edge: malformed polytope
-    if (e.d2 > kOuter) {
+    if (ub(f0 == kNil || f1 == kNil)) return 1;  // open edge: malformed polytope
+    if (ub(e.d2 > kOuter)) {
       nd = normalized(d3{e.wx, e.wy, e.wz});
       nv = md.supportFrom(nd, -1, -1, nh0, nh1);
       if (P.outside(f0, nv)) { start = f0; return 0; }
@@ -1067,9 +1073,9 @@ MFCL_D void epaLoop(PT& P, MK& md, double tol, int max_iter, EpaOut& o, MFCL_LDS
         P.minToOrigin(true, k2, i2);
         if (k2 == 0) { o.status = kEpaFailed; return; }
         if (k2 == 2) {
-          if (kRatio * P.E[i2].d2 < vd2) { kind = 2; fi = i2; }
+          if (ub(kRatio * P.E[i2].d2 < vd2)) { kind = 2; fi = i2; }
         } else if (k2 == 3) {
-          if (kRatio * P.F[i2].d2 < vd2) { kind = 3; fi = i2; }
+          if (ub(kRatio * P.F[i2].d2 < vd2)) { kind = 3; fi = i2; }
         }
       }
       if (kind != 2 && kind != 3) { o.status = kEpaFailed; return; }
@@ -1089,7 +1095,7 @@ MFCL_D void epaLoop(PT& P, MK& md, double tol, int max_iter, EpaOut& o, MFCL_LDS
         return;
       }
       if (start == kNil) { o.status = kEpaFailed; return; }
-      if (epaTerminate(raw, nv, tol)) {
+      if (ub(epaTerminate(raw, nv, tol))) {
         assignPair(md, nd, nh0, nh1, raw, o);
         o.iters = it;
         o.status = kEpaOK;
