@@ -629,6 +629,23 @@ struct MinkCfg {
 
 MFCL_D void reloadFence() { asm volatile("" ::: "memory"); }
 
+// Wavefront-uniform copies of values that every lane holds identically.
+template <class T>
+MFCL_D T* uptr(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return reinterpret_cast<T*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+MFCL_D void uniformShape(DevShape& s) {
+  s.type = __builtin_amdgcn_readfirstlane(s.type);
+  s.nv = __builtin_amdgcn_readfirstlane(s.nv);
+  s.verts = uptr(s.verts);
+  s.nbr = uptr(s.nbr);
+  s.seed = uptr(s.seed);
+  s.wblk = uptr(s.wblk);
+}
+
 MFCL_D void storeM33(MFCL_LDS double* o, const m33& A) {
   o[0] = A.m00; o[1] = A.m01; o[2] = A.m02;
   o[3] = A.m10; o[4] = A.m11; o[5] = A.m12;
@@ -681,6 +698,7 @@ struct MinkL {
     s.verts = c->v0; s.nv = c->nv0; s.rbound = c->rb0;
     s.lv = c->lv0; s.ls = c->ls0;
     s.nbr = c->nbr0; s.seed = c->seed0; s.wblk = c->wblk0;
+    if constexpr (G == 64) uniformShape(s);  // one pair per wavefront
     return s;
   }
   MFCL_D DevShape sh1() const {
@@ -689,6 +707,7 @@ struct MinkL {
     s.verts = c->v1; s.nv = c->nv1; s.rbound = c->rb1;
     s.lv = c->lv1; s.ls = c->ls1;
     s.nbr = c->nbr1; s.seed = c->seed1; s.wblk = c->wblk1;
+    if constexpr (G == 64) uniformShape(s);  // one pair per wavefront
     return s;
   }
   MFCL_D m33 rot1() const { return loadM33(c->r1); }
