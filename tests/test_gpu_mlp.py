@@ -152,3 +152,35 @@ def test_fused_guess_matches_layered_forward(gpu_ctx):
     assert np.all(np.abs(np.linalg.norm(g, axis=1) - 1) < 1e-5)
     assert cos.min() >= 0.999
     assert err_d.max() <= 1e-2 and err_ref.max() <= 1e-2
+
+
+@pytest.mark.gpu
+def test_fused_guess_same_for_every_block_shape(gpu_ctx):
+    """The fused inference's block shape (4, 8 or 16 wavefronts sharing one
+    LDS copy of the weights) changes how tiles are spread, not what a tile
+    computes: guesses and distance estimates are bit-identical."""
+    from mfcl import autotune
+    from mfcl import workloads as W
+
+    h = W.hull_pairs(5000, 11, nmin=8, nmax=64)  # not a multiple of 32 pairs
+    gpu_ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"])
+    p, _, _ = _problem(4, 2)
+    gpu_ctx.mlp_set_params(p)
+    n = len(h["pair_a"])
+    dpa, dpb = gpu_ctx.to_device(h["pair_a"]), gpu_ctx.to_device(h["pair_b"])
+    dqa, dqb = gpu_ctx.to_device(h["pose_a"]), gpu_ctx.to_device(h["pose_b"])
+    dg, dd = gpu_ctx.alloc(n * 12), gpu_ctx.alloc(n * 4)
+    outs = []
+    try:
+        for waves in (4, 8, 16):
+            autotune.set_knob(gpu_ctx, autotune.MLP_WAVES, waves)
+            gpu_ctx.mlp_guess_dev(dpa, dpb, dqa, dqb, n, dg, dd)
+            g, d = np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+            dg.download(g)
+            dd.download(d)
+            outs.append((g, d))
+    finally:
+        autotune.set_knob(gpu_ctx, autotune.MLP_WAVES, 8)
+    for g, d in outs[1:]:
+        assert np.array_equal(g, outs[0][0]) and np.array_equal(d, outs[0][1])
+    assert np.all(np.isfinite(outs[0][0]))
