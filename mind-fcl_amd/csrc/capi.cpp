@@ -797,16 +797,11 @@ int mfcl_convex_hulls(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off
   if (n_sets == 0) return MFCL_OK;
   if (!points || !pt_off || !pt_cnt || !hull_cnt || !hull_idx || !tri_cnt || !tris || !status) return MFCL_E_INVALID;
   uint64_t total = 0;
-  uint32_t maxp = 4;
-  for (uint32_t i = 0; i < n_sets; ++i) {
-    total = std::max<uint64_t>(total, pt_off[i] + pt_cnt[i]);
-    maxp = std::max(maxp, pt_cnt[i]);
-  }
-  maxp = std::min<uint32_t>(maxp, 768);
+  for (uint32_t i = 0; i < n_sets; ++i) total = std::max<uint64_t>(total, pt_off[i] + pt_cnt[i]);
   MFCL_HIP_TRY(ctx, hipSetDevice(ctx->device));
   const size_t b_pts = 12 * total, b_off = 8 * static_cast<size_t>(n_sets), b_cnt = 4 * static_cast<size_t>(n_sets),
                b_idx = 4 * total, b_tri = 24 * total;
-  if (int rc = ensure(ctx, ctx->ws_io, b_pts + b_off + 4 * b_cnt + b_idx + b_tri + 8 * 256)) return rc;
+  if (int rc = ensure(ctx, ctx->ws_io, b_pts + b_off + 5 * b_cnt + b_idx + b_tri + 9 * 256)) return rc;
   char* base = static_cast<char*>(ctx->ws_io.p);
   auto carve = [&](size_t bytes) {
     char* p = base;
@@ -821,11 +816,12 @@ int mfcl_convex_hulls(mfcl_ctx* ctx, const float* points, const uint64_t* pt_off
   int32_t* dst = reinterpret_cast<int32_t*>(carve(b_cnt));
   uint32_t* dhi = reinterpret_cast<uint32_t*>(carve(b_idx));
   int32_t* dtr = reinterpret_cast<int32_t*>(carve(b_tri));
+  uint32_t* dord = reinterpret_cast<uint32_t*>(carve(b_cnt));
   hipStream_t s = ctx->stream;
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dp, points, b_pts, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(doff, pt_off, b_off, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dcnt, pt_cnt, b_cnt, hipMemcpyHostToDevice, s));
-  if (int rc = hullsLaunch(ctx, dp, doff, dcnt, n_sets, static_cast<int>(maxp), dhc, dhi, dtc, dtr, dst, s)) return rc;
+  if (int rc = hullsLaunchClassed(ctx, dp, doff, dcnt, pt_cnt, n_sets, dord, dhc, dhi, dtc, dtr, dst, s)) return rc;
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(hull_cnt, dhc, b_cnt, hipMemcpyDeviceToHost, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(tri_cnt, dtc, b_cnt, hipMemcpyDeviceToHost, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(status, dst, b_cnt, hipMemcpyDeviceToHost, s));

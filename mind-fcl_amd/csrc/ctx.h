@@ -212,6 +212,9 @@ int uploadConvexPoints(mfcl_ctx* ctx, const float* points, const uint64_t* pt_of
 int hullsLaunch(mfcl_ctx* ctx, const float* pts, const uint64_t* pt_off, const uint32_t* pt_cnt, uint32_t n_sets,
                 int max_points, uint32_t* hull_cnt, uint32_t* hull_idx, uint32_t* tri_cnt, int32_t* tris,
                 int32_t* status, hipStream_t s);
+int hullsLaunchClassed(mfcl_ctx* ctx, const float* pts, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                       const uint32_t* pt_cnt_host, uint32_t n_sets, uint32_t* order_dev, uint32_t* hull_cnt,
+                       uint32_t* hull_idx, uint32_t* tri_cnt, int32_t* tris, int32_t* status, hipStream_t s);
 // arm.hip
 size_t armWsBytes(const mfcl_arm* arm, uint64_t n_cfg, uint64_t* max_pairs_out);
 int armQueryLaunch(mfcl_ctx* ctx, const mfcl_arm* arm, const float* q, uint64_t n_cfg, float margin,

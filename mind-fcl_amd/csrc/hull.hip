@@ -122,16 +122,20 @@ MFCL_D void waveArgmax(double& val, int& idx) {
   }
 }
 
-// One wavefront per set.  Outputs (per set i, points at pt_off[i]):
+// One wavefront per set (set order[b] for block b when order is given, else
+// set b), in an LDS slice for sets of up to cap points; the slots a set's
+// faces take do not depend on cap, so neither does its result.  Outputs
+// (per set i, points at pt_off[i]):
 // hull_idx[pt_off + k] = index (within the set) of the k-th hull vertex in
 // ascending order, hull_cnt[i]; tris[6 pt_off + 3 t + j] = triangle t over
 // those k-numbered vertices (outward orientation), tri_cnt[i]; status[i].
 __global__ void __launch_bounds__(64) hullKernel(const float* pts, const uint64_t* pt_off, const uint32_t* pt_cnt,
-                                                 uint32_t n_sets, int cap, uint32_t* hull_cnt, uint32_t* hull_idx,
-                                                 uint32_t* tri_cnt, int32_t* tris, int32_t* status) {
+                                                 const uint32_t* order, uint32_t n_sets, int cap, uint32_t* hull_cnt,
+                                                 uint32_t* hull_idx, uint32_t* tri_cnt, int32_t* tris,
+                                                 int32_t* status) {
   extern __shared__ __align__(16) char hull_lds[];
-  const uint32_t set = blockIdx.x;
-  if (set >= n_sets) return;
+  if (blockIdx.x >= n_sets) return;
+  const uint32_t set = order ? order[blockIdx.x] : blockIdx.x;
   const int l = laneId64();
   const int n = static_cast<int>(pt_cnt[set]);
   const uint64_t o = pt_off[set];
@@ -445,9 +449,50 @@ int hullsLaunch(mfcl_ctx* ctx, const float* pts, const uint64_t* pt_off, const u
     ctx->err = "hull workspace exceeds the LDS of a workgroup";
     return MFCL_E_UNSUPPORTED;
   }
-  hullKernel<<<n_sets, 64, lds, s>>>(pts, pt_off, pt_cnt, n_sets, max_points, hull_cnt, hull_idx, tri_cnt, tris,
-                                     status);
+  hullKernel<<<n_sets, 64, lds, s>>>(pts, pt_off, pt_cnt, nullptr, n_sets, max_points, hull_cnt, hull_idx, tri_cnt,
+                                     tris, status);
   MFCL_HIP_TRY(ctx, hipGetLastError());
+  return MFCL_OK;
+}
+
+// The same with the point counts also on the host: sets are grouped by size
+// class (up to 32, 64, 128, 256, 512, 768 points) and each class runs in LDS
+// slices of its own size, so small sets are not held to the largest set's
+// slice (more wavefronts per CU).  order_dev: n_sets device words for the
+// grouped set list (filled here, read by the kernels).
+int hullsLaunchClassed(mfcl_ctx* ctx, const float* pts, const uint64_t* pt_off, const uint32_t* pt_cnt,
+                       const uint32_t* pt_cnt_host, uint32_t n_sets, uint32_t* order_dev, uint32_t* hull_cnt,
+                       uint32_t* hull_idx, uint32_t* tri_cnt, int32_t* tris, int32_t* status, hipStream_t s) {
+  if (n_sets == 0) return MFCL_OK;
+  constexpr int kClasses = 6;
+  const int caps[kClasses] = {32, 64, 128, 256, 512, kHullMaxPoints};
+  std::vector<uint32_t> order(n_sets);
+  uint32_t count[kClasses] = {}, start[kClasses] = {};
+  auto cls = [&](uint32_t n) {
+    int c = 0;
+    while (c < kClasses - 1 && static_cast<uint32_t>(caps[c]) < n) ++c;
+    return c;
+  };
+  for (uint32_t i = 0; i < n_sets; ++i) ++count[cls(pt_cnt_host[i])];
+  for (int c = 1; c < kClasses; ++c) start[c] = start[c - 1] + count[c - 1];
+  uint32_t fill[kClasses];
+  for (int c = 0; c < kClasses; ++c) fill[c] = start[c];
+  // largest classes first, so the longest sets do not start last
+  for (uint32_t i = 0; i < n_sets; ++i) order[fill[cls(pt_cnt_host[i])]++] = i;
+  MFCL_HIP_TRY(ctx, hipMemcpyAsync(order_dev, order.data(), 4 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice, s));
+  for (int c = kClasses - 1; c >= 0; --c) {
+    if (count[c] == 0) continue;
+    const size_t lds = hullLdsBytes(caps[c]);
+    if (lds > static_cast<size_t>(ctx->max_lds_per_block)) {
+      ctx->err = "hull workspace exceeds the LDS of a workgroup";
+      return MFCL_E_UNSUPPORTED;
+    }
+    hullKernel<<<count[c], 64, lds, s>>>(pts, pt_off, pt_cnt, order_dev + start[c], count[c], caps[c], hull_cnt,
+                                         hull_idx, tri_cnt, tris, status);
+  }
+  MFCL_HIP_TRY(ctx, hipGetLastError());
+  // the order list is read by the kernels: keep it until they have run
+  MFCL_HIP_TRY(ctx, hipStreamSynchronize(s));
   return MFCL_OK;
 }
 
@@ -672,11 +717,12 @@ int uploadConvexPoints(mfcl_ctx* ctx, const float* points, const uint64_t* pt_of
   int32_t* dst = rc ? nullptr : tmp.get<int32_t>(ctx, n_sets, rc);
   uint32_t* dhi = rc ? nullptr : tmp.get<uint32_t>(ctx, total, rc);
   int32_t* dtr = rc ? nullptr : tmp.get<int32_t>(ctx, 6 * total, rc);
+  uint32_t* dord = rc ? nullptr : tmp.get<uint32_t>(ctx, n_sets, rc);
   if (rc) return rc;
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dp, points, 12 * total, hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(doff, pt_off, 8 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice, s));
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(dcnt, pt_cnt, 4 * static_cast<size_t>(n_sets), hipMemcpyHostToDevice, s));
-  if ((rc = hullsLaunch(ctx, dp, doff, dcnt, n_sets, static_cast<int>(maxp), dhc, dhi, dtc, dtr, dst, s))) return rc;
+  if ((rc = hullsLaunchClassed(ctx, dp, doff, dcnt, pt_cnt, n_sets, dord, dhc, dhi, dtc, dtr, dst, s))) return rc;
   std::vector<uint32_t> hc(n_sets), tc(n_sets);
   std::vector<int32_t> st(n_sets);
   MFCL_HIP_TRY(ctx, hipMemcpyAsync(hc.data(), dhc, 4 * static_cast<size_t>(n_sets), hipMemcpyDeviceToHost, s));
