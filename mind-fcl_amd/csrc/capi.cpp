@@ -287,6 +287,10 @@ int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value) {
       if (value < 0 || value > 1) break;
       ctx->tune.epa_spill = value;
       return MFCL_OK;
+    case MFCL_TUNE_LANE_REFILL:
+      if (value < 0 || value > 63) break;
+      ctx->tune.lane_refill = value;
+      return MFCL_OK;
     case MFCL_TUNE_MLP_WAVES:
       if (value != 4 && value != 8 && value != 16) break;
       ctx->tune.mlp_waves = value;

@@ -318,144 +318,169 @@ enum LaneState : int {
 };
 
 // gjkEvaluate + gjkMinDistance of narrow_dev.h, one lane per pair, every
-// branch and operation in the same order (the results are bit-identical).
-// All 64 lanes of the wavefront must call this; lanes with active == false
-// only take part in the support stages.
-MFCL_D void laneGjk(const MinkLane& md, bool active, d3 guess, double tol, int max_iter, bool want_distance,
-                    GjkOut& o, Smp& s, uint64_t& scanned, int& walked) {
-  smpInit(s);
-  o.status = -1;
-  o.sep_valid = false;
-  o.iters = 0;
-  o.dist_iters = 0;
-  o.p0 = d3{0, 0, 0};
-  o.p1 = d3{0, 0, 0};
-  d3 dir = guess;
-  if (sqnorm(dir) <= 0.0) dir = d3{1, 0, 0};
-  dir = normalized(dir);
-  const double tol2 = tol * tol;
-  int st = active ? kLsB0 : kLsDone;
-  bool req = active;
-  int rqid = 0, it = 0, ext = 0;
-  double book = 0;
-  d3 cur = {0, 0, 0};
+// branch and operation in the same order (the results are bit-identical),
+// as a resumable machine: laneInit starts a lane's pair, laneRun advances
+// the lanes of the wavefront together, laneFinish extracts a finished
+// lane's witnesses.  All 64 lanes of the wavefront must call laneRun; lanes
+// without a pair (or done) only take part in the support stages.
+struct LaneM {
+  int st;
+  bool req;
+  int rqid, it, ext;
+  double book;
+  d3 cur, dir;
   MV v;
-  mvZero(v);
-  while (__ballot(st != kLsDone)) {
-    laneSupport(req, md, dir, rqid, v, scanned, walked);
-    req = false;
-    if (st == kLsB0) {
-      s.rank = -1;
-      smpAdd(s, v);
-      if (sqnorm(v.v) <= tol2) {
-        o.status = kGjkIntersect;
-        st = kLsDone;
-      } else if (dot(v.v, dir) < 0) {
-        st = kLsSep;
+  Smp s;
+  GjkOut o;
+};
+
+MFCL_D void laneInit(LaneM& M, bool active, d3 guess) {
+  smpInit(M.s);
+  M.o.status = -1;
+  M.o.sep_valid = false;
+  M.o.iters = 0;
+  M.o.dist_iters = 0;
+  M.o.p0 = d3{0, 0, 0};
+  M.o.p1 = d3{0, 0, 0};
+  M.dir = guess;
+  if (sqnorm(M.dir) <= 0.0) M.dir = d3{1, 0, 0};
+  M.dir = normalized(M.dir);
+  M.st = active ? kLsB0 : kLsDone;
+  M.req = active;
+  M.rqid = 0;
+  M.it = 0;
+  M.ext = 0;
+  M.book = 0;
+  M.cur = d3{0, 0, 0};
+  mvZero(M.v);
+}
+
+// Advances the machine until at most min_running lanes of the wavefront are
+// still running (0: until every lane is done).  A lane's pending support
+// request carries over to the next call.
+MFCL_D void laneRun(LaneM& M, const MinkLane& md, double tol, int max_iter, bool want_distance, uint64_t& scanned,
+                    int& walked, int min_running) {
+  const double tol2 = tol * tol;
+  while (true) {
+    const unsigned long long running = __ballot(M.st != kLsDone);
+    if (running == 0ull || __popcll(running) <= min_running) break;
+    laneSupport(M.req, md, M.dir, M.rqid, M.v, scanned, walked);
+    M.req = false;
+    if (M.st == kLsB0) {
+      M.s.rank = -1;
+      smpAdd(M.s, M.v);
+      if (sqnorm(M.v.v) <= tol2) {
+        M.o.status = kGjkIntersect;
+        M.st = kLsDone;
+      } else if (dot(M.v.v, M.dir) < 0) {
+        M.st = kLsSep;
       } else {
-        dir = -dir;
-        o.status = kGjkIterLimit;
-        it = 0;
-        if (it < max_iter) {
-          o.iters = 1;
-          rqid = freeSlot(s);
-          req = true;
-          st = kLsB1;
+        M.dir = -M.dir;
+        M.o.status = kGjkIterLimit;
+        M.it = 0;
+        if (M.it < max_iter) {
+          M.o.iters = 1;
+          M.rqid = freeSlot(M.s);
+          M.req = true;
+          M.st = kLsB1;
         } else {
-          st = kLsNoSep;
+          M.st = kLsNoSep;
         }
       }
-    } else if (st == kLsB1) {
-      if (dot(v.v, dir) < 0) {
-        st = kLsSep;
-      } else if (smpDup(s, v.v, tol2)) {
-        o.status = kGjkNoProgress;
-        st = kLsNoSep;
-      } else if (sqnorm(v.v) <= tol2) {
-        smpAdd(s, v);
-        o.status = kGjkIntersect;
-        st = kLsDone;
+    } else if (M.st == kLsB1) {
+      if (dot(M.v.v, M.dir) < 0) {
+        M.st = kLsSep;
+      } else if (smpDup(M.s, M.v.v, tol2)) {
+        M.o.status = kGjkNoProgress;
+        M.st = kLsNoSep;
+      } else if (sqnorm(M.v.v) <= tol2) {
+        smpAdd(M.s, M.v);
+        M.o.status = kGjkIntersect;
+        M.st = kLsDone;
       } else {
-        smpAdd(s, v);
-        const int p = gjkProject(s, dir, tol);
+        smpAdd(M.s, M.v);
+        const int p = gjkProject(M.s, M.dir, tol);
         if (p == kProjFailed) {
-          o.status = kGjkFailed;
-          st = kLsDone;
+          M.o.status = kGjkFailed;
+          M.st = kLsDone;
         } else if (p == kProjIntersect) {
-          o.status = kGjkIntersect;
-          st = kLsDone;
+          M.o.status = kGjkIntersect;
+          M.st = kLsDone;
         } else if (p == kProjZeroVolume) {
-          o.status = kGjkNoProgress;
-          st = kLsNoSep;
-        } else if (++it < max_iter) {
-          o.iters = it + 1;
-          rqid = freeSlot(s);
-          req = true;
+          M.o.status = kGjkNoProgress;
+          M.st = kLsNoSep;
+        } else if (++M.it < max_iter) {
+          M.o.iters = M.it + 1;
+          M.rqid = freeSlot(M.s);
+          M.req = true;
         } else {
-          st = kLsNoSep;  // status stays IterLimit
+          M.st = kLsNoSep;  // status stays IterLimit
         }
       }
-    } else if (st == kLsD1) {
-      if (dot(dir, v.v - cur) < tol || smpDup(s, v.v, tol2)) {
-        st = kLsExtract;
+    } else if (M.st == kLsD1) {
+      if (dot(M.dir, M.v.v - M.cur) < tol || smpDup(M.s, M.v.v, tol2)) {
+        M.st = kLsExtract;
       } else {
-        smpAdd(s, v);
-        const int u = distUpdate(s, cur, tol);
+        smpAdd(M.s, M.v);
+        const int u = distUpdate(M.s, M.cur, tol);
         if (u == kNoImprovement) {
-          st = kLsExtract;
+          M.st = kLsExtract;
         } else if (u != kUpdOK) {
-          st = kLsDone;  // sep_valid stays false
+          M.st = kLsDone;  // sep_valid stays false
         } else {
-          const double nm = norm(cur);
-          if (book - nm < tol || nm < tol) {
-            st = kLsExtract;
+          const double nm = norm(M.cur);
+          if (M.book - nm < tol || nm < tol) {
+            M.st = kLsExtract;
           } else {
-            book = nm;
-            dir = -cur / book;
-            if (++it < max_iter) {
-              o.dist_iters++;
-              rqid = freeSlot(s);
-              req = true;
+            M.book = nm;
+            M.dir = -M.cur / M.book;
+            if (++M.it < max_iter) {
+              M.o.dist_iters++;
+              M.rqid = freeSlot(M.s);
+              M.req = true;
             } else {
-              st = kLsDone;
+              M.st = kLsDone;
             }
           }
         }
       }
     }
-    if (st == kLsSep) {
-      o.status = kGjkSeparated;
+    if (M.st == kLsSep) {
+      M.o.status = kGjkSeparated;
       if (!want_distance) {
-        st = kLsDone;
+        M.st = kLsDone;
       } else {
         // gjkMinDistance from the one-vertex simplex {v}: its closest point is v
-        s.rank = -1;
-        smpAdd(s, v);
-        cur = v.v;
-        book = norm(cur);
-        if (book <= tol) {
-          st = kLsExtract;
+        M.s.rank = -1;
+        smpAdd(M.s, M.v);
+        M.cur = M.v.v;
+        M.book = norm(M.cur);
+        if (M.book <= tol) {
+          M.st = kLsExtract;
         } else {
-          dir = -cur / book;
-          it = 0;
-          if (it < max_iter) {
-            o.dist_iters++;
-            rqid = freeSlot(s);
-            req = true;
-            st = kLsD1;
+          M.dir = -M.cur / M.book;
+          M.it = 0;
+          if (M.it < max_iter) {
+            M.o.dist_iters++;
+            M.rqid = freeSlot(M.s);
+            M.req = true;
+            M.st = kLsD1;
           } else {
-            st = kLsDone;
+            M.st = kLsDone;
           }
         }
       }
     }
-    if (st == kLsExtract) ext = 1;
-    if (st == kLsNoSep && want_distance) ext = 2;
-    if (st == kLsExtract || st == kLsNoSep) st = kLsDone;
+    if (M.st == kLsExtract) M.ext = 1;
+    if (M.st == kLsNoSep && want_distance) M.ext = 2;
+    if (M.st == kLsExtract || M.st == kLsNoSep) M.st = kLsDone;
   }
-  // witnesses after the loop, where the machine's state is no longer live
-  if (ext == 1) o.sep_valid = extractNoSub(md, s, o.p0, o.p1);
-  if (ext == 2) o.sep_valid = extractTrySub(md, s, o.p0, o.p1);
+}
+
+// Witnesses of a finished lane, where the machine's state is no longer live.
+MFCL_D void laneFinish(LaneM& M, const MinkLane& md) {
+  if (M.ext == 1) M.o.sep_valid = extractNoSub(md, M.s, M.o.p0, M.o.p1);
+  if (M.ext == 2) M.o.sep_valid = extractTrySub(md, M.s, M.o.p0, M.o.p1);
 }
 
 }  // namespace mfcl

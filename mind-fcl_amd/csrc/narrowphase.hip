@@ -372,6 +372,93 @@ __global__ void __launch_bounds__(kBlock) narrowKernel(ShapeTables T, const doub
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(scanned_total, v);
 }
 
+// Lane-kernel pieces.  laneSetup: pair i's closed form or invalid ids are
+// finished at once (false); otherwise the lane's Minkowski difference is set
+// up and GJK's start direction returned (true).
+template <int MODE>
+MFCL_D bool laneSetup(const ShapeTables& T, const PairIO& io, const Params& P, uint32_t i, MinkLane& md,
+                      d3& guess) {
+  guess = d3{1, 0, 0};
+  mfcl_convex_result r;
+  clearResult(r);
+  ResAcc acc;
+  acc.normal = acc.pos = acc.pa = acc.pb = d3{0, 0, 0};
+  const uint32_t ia = io.pa[i], ib = io.pb[i];
+  if (ia >= P.n_shapes || ib >= P.n_shapes) {
+    r.ok = -1;  // shape id out of range
+  } else {
+    const int ta_ = T.type[ia], tb_ = T.type[ib];
+    if (MODE == kModePen && P.max_contacts <= 0) {
+      // the reference warns and adds nothing (collision-inl.h:81-84)
+    } else if (MODE == kModePen && closedForm(ta_, tb_)) {
+      const tf3 ta = poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i));
+      const tf3 tb = poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i));
+      PContacts pc;
+      pc.n = 0;
+      const bool hit = runClosedForm(loadShape(T, ia), ta, loadShape(T, ib), tb, &pc);
+      r.hit = hit;
+      r.ok = 1;
+      if (hit) finishClosedForm(pc, P, io, i, r, acc, true);
+    } else {
+      md.ia = ia;
+      md.ib = ib;
+      md.type0 = ta_;
+      md.type1 = tb_;
+      md.setTf0(relTf(poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i)),
+                      poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i))));
+      if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
+      return true;
+    }
+  }
+  storeResult(&io.out[i], r, acc);
+  return false;
+}
+
+// laneStore: pair i's result after its GJK (and its EPA queue entry when it
+// intersects; the EPA stage completes those).
+template <int MODE>
+MFCL_D void laneStore(const PairIO& io, uint32_t i, const MinkLane& md, const LaneM& M, QEntry* queue,
+                      uint32_t* qcount) {
+  const GjkOut& g = M.o;
+  const Smp& s = M.s;
+  mfcl_convex_result r;
+  clearResult(r);
+  ResAcc acc;
+  acc.normal = acc.pos = acc.pa = acc.pb = d3{0, 0, 0};
+  r.method = 1;
+  r.gjk_status = static_cast<int8_t>(g.status);
+  bool queued = false;
+  if constexpr (MODE == kModeSd) {
+    r.gjk_iters = sat16(g.iters + g.dist_iters);
+    if (g.status == kGjkSeparated && g.sep_valid) {
+      const tf3 ta = poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i));
+      acc.pa = apply(ta, g.p0);
+      acc.pb = apply(ta, g.p1);
+      r.distance = norm(g.p0 - g.p1);
+      r.ok = 1;
+    } else if (g.status == kGjkIntersect) {
+      queued = true;
+    } else {
+      r.distance = -1;
+    }
+  } else {
+    r.gjk_iters = sat16(g.iters);
+    r.ok = 1;
+    queued = g.status == kGjkIntersect;
+  }
+  if (queued) {
+    QEntry* qe = &queue[atomicAdd(qcount, 1u)];
+    qe->s.rank = s.rank;
+    storeMVD(qe->s.a, s.a.v, md.dirOf(static_cast<int>(s.a.id)));
+    storeMVD(qe->s.b, s.b.v, md.dirOf(static_cast<int>(s.b.id)));
+    storeMVD(qe->s.c, s.c.v, md.dirOf(static_cast<int>(s.c.id)));
+    storeMVD(qe->s.e, s.e.v, md.dirOf(static_cast<int>(s.e.id)));
+    qe->pair = i;
+  }
+  // queued pairs keep their GJK status here; the EPA stage completes them
+  storeResult(&io.out[i], r, acc);
+}
+
 // One lane per pair (penetration and signed-distance modes): closed forms
 // per lane, GJK through the lane machine of gjk_lane.h.  Results, EPA queue
 // entries and scan counts equal those of narrowKernel.  dir_ws holds
@@ -396,83 +483,81 @@ __global__ void __launch_bounds__(kBlock) laneKernel(ShapeTables T, PairIO io, c
     // the bucketed order runs from cheap to costly pairs: the grid takes it
     // from the costly end so that the largest pairs start first
     i = order[n - 1 - slot];
-    mfcl_convex_result r;
-    clearResult(r);
-    ResAcc acc;
-    acc.normal = acc.pos = acc.pa = acc.pb = d3{0, 0, 0};
-    const uint32_t ia = io.pa[i], ib = io.pb[i];
-    if (ia >= P.n_shapes || ib >= P.n_shapes) {
-      r.ok = -1;  // shape id out of range
-    } else {
-      const int ta_ = T.type[ia], tb_ = T.type[ib];
-      if (MODE == kModePen && P.max_contacts <= 0) {
-        // the reference warns and adds nothing (collision-inl.h:81-84)
-      } else if (MODE == kModePen && closedForm(ta_, tb_)) {
-        const tf3 ta = poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i));
-        const tf3 tb = poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i));
-        PContacts pc;
-        pc.n = 0;
-        const bool hit = runClosedForm(loadShape(T, ia), ta, loadShape(T, ib), tb, &pc);
-        r.hit = hit;
-        r.ok = 1;
-        if (hit) finishClosedForm(pc, P, io, i, r, acc, true);
-      } else {
-        md.ia = ia;
-        md.ib = ib;
-        md.type0 = ta_;
-        md.type1 = tb_;
-        md.setTf0(relTf(poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i)),
-                        poseFromQuat(io.poseb + 7 * static_cast<uint64_t>(i))));
-        if (P.use_guess) guess = d3{io.guess[3 * i], io.guess[3 * i + 1], io.guess[3 * i + 2]};
-        run_gjk = true;
-      }
-    }
-    if (!run_gjk) storeResult(&io.out[i], r, acc);
+    run_gjk = laneSetup<MODE>(T, io, P, i, md, guess);
   }
-  GjkOut g;
-  Smp s;
   uint64_t scanned = 0;
   int walked = 0;
-  laneGjk(md, run_gjk, -guess, P.gjk_tol, P.gjk_max_it, MODE == kModeSd, g, s, scanned, walked);
-  if (run_gjk) {
-    mfcl_convex_result r;
-    clearResult(r);
-    ResAcc acc;
-    acc.normal = acc.pos = acc.pa = acc.pb = d3{0, 0, 0};
-    r.method = 1;
-    r.gjk_status = static_cast<int8_t>(g.status);
-    bool queued = false;
-    if constexpr (MODE == kModeSd) {
-      r.gjk_iters = sat16(g.iters + g.dist_iters);
-      if (g.status == kGjkSeparated && g.sep_valid) {
-        const tf3 ta = poseFromQuat(io.posea + 7 * static_cast<uint64_t>(i));
-        acc.pa = apply(ta, g.p0);
-        acc.pb = apply(ta, g.p1);
-        r.distance = norm(g.p0 - g.p1);
-        r.ok = 1;
-      } else if (g.status == kGjkIntersect) {
-        queued = true;
-      } else {
-        r.distance = -1;
-      }
-    } else {
-      r.gjk_iters = sat16(g.iters);
-      r.ok = 1;
-      queued = g.status == kGjkIntersect;
-    }
-    if (queued) {
-      QEntry* qe = &queue[atomicAdd(qcount, 1u)];
-      qe->s.rank = s.rank;
-      storeMVD(qe->s.a, s.a.v, md.dirOf(static_cast<int>(s.a.id)));
-      storeMVD(qe->s.b, s.b.v, md.dirOf(static_cast<int>(s.b.id)));
-      storeMVD(qe->s.c, s.c.v, md.dirOf(static_cast<int>(s.c.id)));
-      storeMVD(qe->s.e, s.e.v, md.dirOf(static_cast<int>(s.e.id)));
-      qe->pair = i;
-    }
-    // queued pairs keep their GJK status here; the EPA stage completes them
-    storeResult(&io.out[i], r, acc);
-  }
+  LaneM M;
+  laneInit(M, run_gjk, -guess);
+  laneRun(M, md, P.gjk_tol, P.gjk_max_it, MODE == kModeSd, scanned, walked, 0);
+  laneFinish(M, md);
+  if (run_gjk) laneStore<MODE>(io, i, md, M, queue, qcount);
   // cooperative scans are counted once per wavefront, walks per lane
+  unsigned long long v = static_cast<unsigned long long>(walked);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  v += scanned;
+  if (md.lane == 0 && v) atomicAdd(scanned_total, v);
+}
+
+// The same with refill: a persistent grid whose lanes take pairs from a
+// shared cursor (the costly end of the bucketed order first).  When at most
+// refill_at lanes of a wavefront are still running, the finished lanes write
+// their results and take new pairs, so a wavefront does not wait on its
+// slowest pair.  Each pair's result and EPA queue entry equal laneKernel's
+// (a pair's computation does not depend on its lane or wavefront).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) laneKernelR(ShapeTables T, PairIO io, const uint32_t* order, uint64_t n,
+                                                      Params P, QEntry* queue, uint32_t* qcount,
+                                                      unsigned long long* scanned_total, double* dir_ws,
+                                                      uint32_t* cursor, int refill_at) {
+  __shared__ LaneWs ws[kBlock / 64];
+  const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  MinkLane md;
+  md.T = T;
+  md.w = (MFCL_LDS LaneWs*)(&ws[threadIdx.x >> 6]);
+  md.dirs = dir_ws + wave * kLaneDirDoubles;
+  md.lane = static_cast<int>(threadIdx.x & 63);
+  md.ia = md.ib = 0;
+  md.type0 = md.type1 = kSphere;
+  const unsigned long long below = (1ull << md.lane) - 1;
+  uint64_t scanned = 0;
+  int walked = 0;
+  LaneM M;
+  laneInit(M, false, d3{1, 0, 0});
+  uint32_t i = 0;
+  bool have = false;       // the lane holds a pair in GJK
+  bool exhausted = false;  // the cursor has passed n (wavefront-uniform)
+  while (true) {
+    // lanes without a pair take the next ones; closed forms finish at once
+    // and their lanes take again
+    while (!exhausted) {
+      const unsigned long long need = __ballot(!have);
+      if (need == 0ull) break;
+      const uint32_t cnt = static_cast<uint32_t>(__popcll(need));
+      uint32_t base = 0;
+      if (md.lane == 0) base = atomicAdd(cursor, cnt);
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (static_cast<uint64_t>(base) + cnt >= n) exhausted = true;
+      if (!have) {
+        const uint64_t k = static_cast<uint64_t>(base) + static_cast<uint64_t>(__popcll(need & below));
+        if (k < n) {
+          i = order[n - 1 - k];
+          d3 guess;
+          if (laneSetup<MODE>(T, io, P, i, md, guess)) {
+            laneInit(M, true, -guess);
+            have = true;
+          }
+        }
+      }
+    }
+    if (__ballot(have) == 0ull) break;
+    laneRun(M, md, P.gjk_tol, P.gjk_max_it, MODE == kModeSd, scanned, walked, exhausted ? 0 : refill_at);
+    if (have && M.st == kLsDone) {
+      laneFinish(M, md);
+      laneStore<MODE>(io, i, md, M, queue, qcount);
+      have = false;
+    }
+  }
   unsigned long long v = static_cast<unsigned long long>(walked);
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   v += scanned;
@@ -944,10 +1029,22 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
     if (G == 1) {
       const unsigned lb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
       double* dws = ws.lane_dirs;
-      if (mode == kModeSd)
+      // signed distance only: the penetration-mode refill kernel holds more
+      // registers than two wavefronts per SIMD allow and measures slower
+      const int refill = mode == kModeSd ? ctx->tune.lane_refill : 0;
+      if (refill > 0) {
+        // persistent: two blocks (8 wavefronts) per CU, at most one lane per pair
+        const unsigned rb = static_cast<unsigned>(std::min<uint64_t>(lb, 2ull * static_cast<uint64_t>(ctx->n_cu)));
+        uint32_t* cursor = qcount + 8;  // zeroed with the histogram
+        if (mode == kModeSd)
+          laneKernelR<kModeSd><<<rb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws, cursor, refill);
+        else
+          laneKernelR<kModePen><<<rb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws, cursor, refill);
+      } else if (mode == kModeSd) {
         laneKernel<kModeSd><<<lb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws);
-      else
+      } else {
         laneKernel<kModePen><<<lb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws);
+      }
     } else switch (G) {
       case 32: MFCL_NARROW_MODES(32); break;
       case 64: MFCL_NARROW_MODES(64); break;

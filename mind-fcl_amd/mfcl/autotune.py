@@ -23,6 +23,7 @@ EPA_LDS_FACES = 5
 GUESS_POLICY = 6
 EPA_SPILL = 7
 MLP_WAVES = 8
+LANE_REFILL = 9
 
 
 def set_knob(ctx, knob: int, value: int) -> None:
