@@ -245,7 +245,8 @@ def main():
             "launch": tuned,
         },
         "roofline": {
-            "kernel": "laneKernel (GJK stage)" if tuned.get("gjk_group", 1) == 1 else "narrowKernel (GJK stage)",
+            "kernel": "laneKernelR (GJK stage, lanes refilled)" if tuned.get("gjk_group", 1) == 1
+                      else "narrowKernel (GJK stage)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
