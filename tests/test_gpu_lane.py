@@ -109,3 +109,36 @@ def test_hull_signed_distance_bit_exact_vs_naive_oracle(oracle, gpu_ctx, group):
     print(f"\n[naive oracle, group {group}] pairs={len(out)} intersecting={int((ref['distance'] < 0).sum())} "
           f"differing per field {diff}")
     assert all(v == 0 for v in diff.values())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("walking,guess", [(False, False), (False, True), (True, False)])
+def test_lane_refill_thresholds_identical(gpu_ctx, walking, guess):
+    """Signed distance on the refilled lane grid (finished lanes take new
+    pairs once at most k lanes of their wavefront still run): every result
+    record is byte-identical for k = 1, 48, 63 and without refill (one pair
+    per lane), on scanning and on walking hulls; so are the work counters."""
+    if walking:
+        h = W.hull_pairs(3000, 24, faces=True)
+        gpu_ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"],
+                              faces=mfcl.faces_from_triangles(h["tris"], h["tri_off"], h["tri_cnt"]))
+        w = dict(pair_a=h["pair_a"], pair_b=h["pair_b"], pose_a=h["pose_a"], pose_b=h["pose_b"],
+                 guess=np.zeros((len(h["pair_a"]), 3), np.float32))
+    else:
+        w = _mixed(23)
+        gpu_ctx.upload_shapes(w["types"], w["params"], w["vert_off"], w["vert_cnt"], w["verts"])
+    req = mfcl.request(mfcl.MODE_SIGNED_DISTANCE, use_guess=int(guess))
+    runs = []
+    try:
+        for k in (0, 1, 48, 63):
+            autotune.set_knob(gpu_ctx, autotune.LANE_REFILL, k)
+            runs.append(_run(gpu_ctx, w, req, 1, guess))
+    finally:
+        autotune.set_knob(gpu_ctx, autotune.LANE_REFILL, 48)
+    ref, cref = runs[0]
+    print(f"\n[lane refill walking={walking} guess={guess}] pairs={len(ref)} "
+          f"EPA queued {int((ref['epa_status'] >= 0).sum())} counters {cref[:5].tolist()}")
+    for out, cnt in runs[1:]:
+        assert out.tobytes() == ref.tobytes()
+        assert cnt[0] == cref[0] and cnt[2] == cref[2] and cnt[3] == cref[3] and cnt[4] == cref[4]
+        assert abs(int(cnt[1]) - int(cref[1])) <= 1e-4 * int(cref[1])
