@@ -245,7 +245,7 @@ def main():
             "launch": tuned,
         },
         "roofline": {
-            "kernel": "laneKernelR (GJK stage, lanes refilled)" if tuned.get("gjk_group", 1) == 1
+            "kernel": "laneKernelRSd (GJK stage, lanes refilled)" if tuned.get("gjk_group", 1) == 1
                       else "narrowKernel (GJK stage)",
             "bound": "hbm",
             "achieved": achieved,

@@ -162,10 +162,10 @@ int mfcl_last_counters_n(mfcl_ctx* ctx, uint64_t* c, int32_t n);
                                          device-memory polytope of full capacity; 0: it reruns afterwards */
 #define MFCL_TUNE_MLP_WAVES 8         /* wavefronts per block of the fused warm-start inference (4, 8 or 16),
                                          sharing one LDS copy of the weights; one block per CU */
-#define MFCL_TUNE_LANE_REFILL 9       /* GJK lane kernel of signed-distance queries: 0 one pair per lane over a
-                                         grid of all pairs; k in 1..63 (default 48) a persistent grid whose
-                                         finished lanes take new pairs once at most k lanes of their
-                                         wavefront still run */
+#define MFCL_TUNE_LANE_REFILL 9       /* GJK lane kernel (signed distance and penetration): 0 one pair per
+                                         lane over a grid of all pairs; k in 1..63 (default 48) a persistent
+                                         grid whose finished lanes take new pairs once at most k lanes of
+                                         their wavefront still run */
 int mfcl_tune(mfcl_ctx* ctx, int32_t knob, int32_t value);
 
 /* Workspace sizing (no counterpart in the reference, whose calls allocate

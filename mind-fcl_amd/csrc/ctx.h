@@ -110,7 +110,7 @@ struct mfcl_ctx {
     int mesh_blocks_per_cu{3};  // persistent mesh-traversal blocks (4 waves, one query each) per CU
     int mesh_stack{16384};      // node-pair stack entries per wavefront
     int mlp_waves{8};           // wavefronts per block of the fused warm-start inference
-    int lane_refill{48};        // signed-distance lane kernel: 0 one pair per lane; k > 0 refill lanes when <= k run
+    int lane_refill{48};        // lane kernel: 0 one pair per lane; k > 0 refill lanes when <= k run
   } tune;
   std::string err;
   hipStream_t stream{nullptr};

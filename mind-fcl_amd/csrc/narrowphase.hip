@@ -499,22 +499,23 @@ __global__ void __launch_bounds__(kBlock) laneKernel(ShapeTables T, PairIO io, c
   if (md.lane == 0 && v) atomicAdd(scanned_total, v);
 }
 
-// The same with refill: a persistent grid whose lanes take pairs from a
-// shared cursor (the costly end of the bucketed order first).  When at most
-// refill_at lanes of a wavefront are still running, the finished lanes write
-// their results and take new pairs, so a wavefront does not wait on its
-// slowest pair.  Each pair's result and EPA queue entry equal laneKernel's
-// (a pair's computation does not depend on its lane or wavefront).
+// The same with refill: a persistent grid (two wavefronts per SIMD) whose
+// lanes take pairs from a shared cursor (the costly end of the bucketed
+// order first).  When at most refill_at lanes of a wavefront are still
+// running, the finished lanes write their results and take new pairs, so a
+// wavefront does not wait on its slowest pair.  Each pair's result and EPA
+// queue entry equal laneKernel's (a pair's computation does not depend on
+// its lane or wavefront).  The penetration-mode kernel is bounded to two
+// wavefronts per SIMD (it spills a little to fit); the signed-distance one
+// fits without the bound.
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) laneKernelR(ShapeTables T, PairIO io, const uint32_t* order, uint64_t n,
-                                                      Params P, QEntry* queue, uint32_t* qcount,
-                                                      unsigned long long* scanned_total, double* dir_ws,
-                                                      uint32_t* cursor, int refill_at) {
-  __shared__ LaneWs ws[kBlock / 64];
+__device__ __attribute__((always_inline)) inline void laneRefillBody(
+    ShapeTables T, PairIO io, const uint32_t* order, uint64_t n, Params P, QEntry* queue, uint32_t* qcount,
+    unsigned long long* scanned_total, double* dir_ws, uint32_t* cursor, int refill_at, MFCL_LDS LaneWs* ws) {
   const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   MinkLane md;
   md.T = T;
-  md.w = (MFCL_LDS LaneWs*)(&ws[threadIdx.x >> 6]);
+  md.w = ws + (threadIdx.x >> 6);
   md.dirs = dir_ws + wave * kLaneDirDoubles;
   md.lane = static_cast<int>(threadIdx.x & 63);
   md.ia = md.ib = 0;
@@ -562,6 +563,23 @@ __global__ void __launch_bounds__(kBlock) laneKernelR(ShapeTables T, PairIO io, 
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   v += scanned;
   if (md.lane == 0 && v) atomicAdd(scanned_total, v);
+}
+
+__global__ void __launch_bounds__(kBlock) laneKernelRSd(ShapeTables T, PairIO io, const uint32_t* order, uint64_t n,
+                                                        Params P, QEntry* queue, uint32_t* qcount,
+                                                        unsigned long long* scanned_total, double* dir_ws,
+                                                        uint32_t* cursor, int refill_at) {
+  __shared__ LaneWs ws[kBlock / 64];
+  laneRefillBody<kModeSd>(T, io, order, n, P, queue, qcount, scanned_total, dir_ws, cursor, refill_at,
+                          (MFCL_LDS LaneWs*)(&ws[0]));
+}
+__global__ void __launch_bounds__(kBlock, 2) laneKernelRPen(ShapeTables T, PairIO io, const uint32_t* order,
+                                                            uint64_t n, Params P, QEntry* queue, uint32_t* qcount,
+                                                            unsigned long long* scanned_total, double* dir_ws,
+                                                            uint32_t* cursor, int refill_at) {
+  __shared__ LaneWs ws[kBlock / 64];
+  laneRefillBody<kModePen>(T, io, order, n, P, queue, qcount, scanned_total, dir_ws, cursor, refill_at,
+                           (MFCL_LDS LaneWs*)(&ws[0]));
 }
 
 // EPA stage: one wavefront per queued pair; the 64 lanes share every
@@ -1029,17 +1047,15 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
     if (G == 1) {
       const unsigned lb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
       double* dws = ws.lane_dirs;
-      // signed distance only: the penetration-mode refill kernel holds more
-      // registers than two wavefronts per SIMD allow and measures slower
-      const int refill = mode == kModeSd ? ctx->tune.lane_refill : 0;
+      const int refill = ctx->tune.lane_refill;
       if (refill > 0) {
         // persistent: two blocks (8 wavefronts) per CU, at most one lane per pair
         const unsigned rb = static_cast<unsigned>(std::min<uint64_t>(lb, 2ull * static_cast<uint64_t>(ctx->n_cu)));
         uint32_t* cursor = qcount + 8;  // zeroed with the histogram
         if (mode == kModeSd)
-          laneKernelR<kModeSd><<<rb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws, cursor, refill);
+          laneKernelRSd<<<rb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws, cursor, refill);
         else
-          laneKernelR<kModePen><<<rb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws, cursor, refill);
+          laneKernelRPen<<<rb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws, cursor, refill);
       } else if (mode == kModeSd) {
         laneKernel<kModeSd><<<lb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws);
       } else {

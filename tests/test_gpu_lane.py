@@ -112,12 +112,14 @@ def test_hull_signed_distance_bit_exact_vs_naive_oracle(oracle, gpu_ctx, group):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("walking,guess", [(False, False), (False, True), (True, False)])
-def test_lane_refill_thresholds_identical(gpu_ctx, walking, guess):
-    """Signed distance on the refilled lane grid (finished lanes take new
-    pairs once at most k lanes of their wavefront still run): every result
-    record is byte-identical for k = 1, 48, 63 and without refill (one pair
-    per lane), on scanning and on walking hulls; so are the work counters."""
+@pytest.mark.parametrize("walking,guess,pen", [(False, False, False), (False, True, False), (True, False, False),
+                                              (False, True, True), (True, False, True)])
+def test_lane_refill_thresholds_identical(gpu_ctx, walking, guess, pen):
+    """The refilled lane grid (finished lanes take new pairs once at most k
+    lanes of their wavefront still run), signed distance and penetration
+    (closed forms among the pairs): every result record is byte-identical
+    for k = 1, 48, 63 and without refill (one pair per lane), on scanning and
+    on walking hulls; so are the work counters."""
     if walking:
         h = W.hull_pairs(3000, 24, faces=True)
         gpu_ctx.upload_shapes(h["types"], h["params"], h["vert_off"], h["vert_cnt"], h["verts"],
@@ -127,7 +129,10 @@ def test_lane_refill_thresholds_identical(gpu_ctx, walking, guess):
     else:
         w = _mixed(23)
         gpu_ctx.upload_shapes(w["types"], w["params"], w["vert_off"], w["vert_cnt"], w["verts"])
-    req = mfcl.request(mfcl.MODE_SIGNED_DISTANCE, use_guess=int(guess))
+    if pen:
+        req = mfcl.request(mfcl.MODE_COLLIDE, max_contacts=4, penetration=1, use_guess=int(guess))
+    else:
+        req = mfcl.request(mfcl.MODE_SIGNED_DISTANCE, use_guess=int(guess))
     runs = []
     try:
         for k in (0, 1, 48, 63):
@@ -136,7 +141,7 @@ def test_lane_refill_thresholds_identical(gpu_ctx, walking, guess):
     finally:
         autotune.set_knob(gpu_ctx, autotune.LANE_REFILL, 48)
     ref, cref = runs[0]
-    print(f"\n[lane refill walking={walking} guess={guess}] pairs={len(ref)} "
+    print(f"\n[lane refill walking={walking} guess={guess} pen={pen}] pairs={len(ref)} "
           f"EPA queued {int((ref['epa_status'] >= 0).sum())} counters {cref[:5].tolist()}")
     for out, cnt in runs[1:]:
         assert out.tobytes() == ref.tobytes()
