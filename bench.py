@@ -179,7 +179,12 @@ def main():
 
         t_group = autotune.tune(ctx, autotune.GJK_GROUP, [1, 32], run, reps=2, log=log)
         t_epa = autotune.tune(ctx, autotune.EPA_THREADS, [32768, 131072], run, reps=2, log=log)
-        tuned = {"gjk_group": min(t_group, key=t_group.get), "epa_threads": min(t_epa, key=t_epa.get)}
+        t_lds = autotune.tune(ctx, autotune.EPA_LDS_FACES, [48, 64], run, reps=3, log=log)
+        tuned = {"gjk_group": min(t_group, key=t_group.get), "epa_threads": min(t_epa, key=t_epa.get),
+                 "epa_lds_faces": min(t_lds, key=t_lds.get)}
+        if tuned["gjk_group"] == 1:
+            t_refill = autotune.tune(ctx, autotune.LANE_REFILL, [40, 48], run, reps=3, log=log)
+            tuned["lane_refill"] = min(t_refill, key=t_refill.get)
     ctx.set_timing(True)
     for _ in range(args.warmup):
         ctx.convex_query_dev(dpa, dpb, dqa, dqb, n, req, dout)
@@ -431,6 +436,15 @@ def bench_mesh(ctx, args, threads):
     dqa, dqb = ctx.to_device(pairs["pose_a"]), ctx.to_device(pairs["pose_b"])
     dres = ctx.alloc(n * mfcl.MESH_RESULT_DTYPE.itemsize)
     dcon = ctx.alloc(n * 16 * 8)
+    tuned = {}
+    if not args.no_tune:
+        from mfcl import autotune
+
+        def run():
+            ctx.mesh_collide_dev(dpa, dpb, dqa, dqb, n, 16, dres, dcon)
+
+        t_bpc = autotune.tune(ctx, autotune.MESH_BLOCKS_PER_CU, [2, 3], run, reps=5, log=log)
+        tuned = {"mesh_blocks_per_cu": min(t_bpc, key=t_bpc.get)}
     for _ in range(2):
         ctx.mesh_collide_dev(dpa, dpb, dqa, dqb, n, 16, dres, dcon)
     ctx.sync()
@@ -446,7 +460,7 @@ def bench_mesh(ctx, args, threads):
     bv, lf = res["bv_tests"].astype(np.float64).sum(), res["leaf_tests"].astype(np.float64).sum()
     alg = 2 * 64 * bv + 2 * 36 * lf + 64 * n
     kms = float(np.mean(km))
-    out = {"value": n * args.steps / el, "unit": "pairs/s", "pairs_per_launch": n, "kernel_ms": kms,
+    out = {"value": n * args.steps / el, "unit": "pairs/s", "pairs_per_launch": n, "kernel_ms": kms, "launch": tuned,
            "hit_fraction": float((res["hit"] == 1).mean()), "overflow_pairs": int((res["hit"] < 0).sum()),
            "bv_tests_mean": bv / n, "bv_tests_max": int(res["bv_tests"].max()), "leaf_tests_mean": lf / n,
            "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
