@@ -565,7 +565,9 @@ __device__ __attribute__((always_inline)) inline void laneRefillBody(
   if (md.lane == 0 && v) atomicAdd(scanned_total, v);
 }
 
-__global__ void __launch_bounds__(kBlock) laneKernelRSd(ShapeTables T, PairIO io, const uint32_t* order, uint64_t n,
+// three wavefronts per SIMD: the spills this costs (168 VGPRs) are cheaper
+// than the latency left exposed at two; four spill too much
+__global__ void __launch_bounds__(kBlock, 3) laneKernelRSd(ShapeTables T, PairIO io, const uint32_t* order, uint64_t n,
                                                         Params P, QEntry* queue, uint32_t* qcount,
                                                         unsigned long long* scanned_total, double* dir_ws,
                                                         uint32_t* cursor, int refill_at) {
@@ -1049,8 +1051,11 @@ int convexQueryLaunch(mfcl_ctx* ctx, const uint32_t* pa, const uint32_t* pb, con
       double* dws = ws.lane_dirs;
       const int refill = ctx->tune.lane_refill;
       if (refill > 0) {
-        // persistent: two blocks (8 wavefronts) per CU, at most one lane per pair
-        const unsigned rb = static_cast<unsigned>(std::min<uint64_t>(lb, 2ull * static_cast<uint64_t>(ctx->n_cu)));
+        // persistent: as many blocks per CU as the kernel's launch bound
+        // keeps resident (3 for distance, 2 for penetration), at most one
+        // lane per pair
+        const uint64_t per_cu = mode == kModeSd ? 3ull : 2ull;
+        const unsigned rb = static_cast<unsigned>(std::min<uint64_t>(lb, per_cu * static_cast<uint64_t>(ctx->n_cu)));
         uint32_t* cursor = qcount + 8;  // zeroed with the histogram
         if (mode == kModeSd)
           laneKernelRSd<<<rb, kBlock, 0, s>>>(T, io, order, n, P, queue, qcount, sc, dws, cursor, refill);
